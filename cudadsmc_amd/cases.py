@@ -1,0 +1,103 @@
+"""Case definitions: gas constants, the benchmark configurations and body
+polygons.  The role of the reference's compile-time src/Param.h:5-37 and of
+the geometry choice in Run::initialize (Run.cpp:45-48), as runtime values.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+BOLTZ = 1.380649e-23          # J/K
+
+# argon, VHS (Bird 1994, Appendix A)
+ARGON = dict(mass=6.63e-26, diam=4.17e-10, Vtl=0.81, Tref=273.0)
+
+
+def vhs_collision_frequency(n, T, gas=ARGON, boltz=BOLTZ):
+    """Equilibrium VHS collision frequency (Bird 1994 eq. 4.64):
+    nu = 4 d^2 n sqrt(pi k Tref / m) (T/Tref)^(1-omega)."""
+    d, m, w, Tr = gas["diam"], gas["mass"], gas["Vtl"], gas["Tref"]
+    return 4.0 * d * d * n * math.sqrt(math.pi * boltz * Tr / m) * (T / Tr) ** (1.0 - w)
+
+
+def hard_sphere_density_for_mfp(lam, gas=ARGON):
+    """n such that the hard-sphere mean free path 1/(sqrt(2) pi d^2 n) is lam."""
+    return 1.0 / (math.sqrt(2.0) * math.pi * gas["diam"] ** 2 * lam)
+
+
+def circle_polygon(n, cx, cy, r):
+    """Counter-clockwise regular polygon (Circle::Initialize, Circle.cpp:10-19)."""
+    a = 2.0 * math.pi * np.arange(n) / n
+    return np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], axis=1)
+
+
+def wedge_polygon(apex_x, cy, length, half_angle_deg):
+    """Counter-clockwise isosceles wedge, apex upstream at (apex_x, cy)."""
+    h = length * math.tan(math.radians(half_angle_deg))
+    return np.array([[apex_x, cy], [apex_x + length, cy - h], [apex_x + length, cy + h]], dtype=np.float64)
+
+
+def make_params(**kw):
+    """dsmc_params with defaults; keyword names are the struct's fields."""
+    p = abi.dsmc_params()
+    p.boltz = BOLTZ
+    for k, v in ARGON.items():
+        setattr(p, k, v)
+    p.L3 = 1.0
+    p.V_y = 0.0
+    p.T_wall = 300.0
+    p.wall_bc = abi.BC_DIFFUSE
+    for f in range(4):
+        p.face_bc[f] = abi.BC_STREAM
+        p.face_T[f] = 0.0
+    p.Fn = 0.0
+    p.seed = 20251203
+    p.capacity = 0
+    p.sample_start = -1
+    for k, v in kw.items():
+        if k in ("face_bc", "face_T"):
+            arr = getattr(p, k)
+            for f in range(4):
+                arr[f] = v[f]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def box_equilibrium(ppc=20, n_cells=64, steps=1000, seed=20251203):
+    """Plain DSMC, 2D closed box at equilibrium: argon, 1 mm x 1 mm, 64x64
+    cells, n = 1e20 m^-3, T = 300 K, 20 particles per cell, specular walls,
+    dt = 0.2 / nu(VHS), 1000 steps."""
+    n, T = 1e20, 300.0
+    nu = vhs_collision_frequency(n, T)
+    p = make_params(L1=1e-3, L2=1e-3, N1=n_cells, N2=n_cells, n_inf=n, T=T, V_jet=0.0,
+                    wall_bc=abi.BC_SPECULAR, face_bc=[abi.BC_SPECULAR] * 4,
+                    tau=0.2 / nu, ppc=float(ppc), seed=seed, sample_start=0)
+    return dict(name="box_equilibrium_64x64", params=p, body=None, steps=steps, nu=nu)
+
+
+def cylinder_mach10(ppc=30, nx=400, ny=200, n_points=256, steps=20000, sample_from=10000,
+                    seed=20251203):
+    """Plain DSMC, 2D supersonic argon flow over a cut-cell cylinder: Mach 10,
+    T_inf = 200 K, Kn = 0.01 on the diameter (hard-sphere mean free path),
+    R = 0.1 m centred in a 0.6 m x 0.3 m domain, 400x200 cells, diffuse wall
+    at 500 K, freestream inflow/outflow on all faces, 30 particles per fluid
+    cell, dt = 0.3 dx / (U + 3 vmp), 20k steps, sampling from step 10k."""
+    T = 200.0
+    m = ARGON["mass"]
+    a = math.sqrt(5.0 / 3.0 * BOLTZ * T / m)
+    U = 10.0 * a
+    R, D = 0.1, 0.2
+    lam = 0.01 * D
+    n = hard_sphere_density_for_mfp(lam)
+    L1, L2 = 0.6, 0.3
+    dx = L1 / nx
+    vmp = math.sqrt(2.0 * BOLTZ * T / m)
+    dt = 0.3 * dx / (U + 3.0 * vmp)
+    p = make_params(L1=L1, L2=L2, N1=nx, N2=ny, n_inf=n, T=T, V_jet=U, T_wall=500.0,
+                    wall_bc=abi.BC_DIFFUSE, face_bc=[abi.BC_STREAM] * 4, tau=dt,
+                    ppc=float(ppc), seed=seed, sample_start=sample_from)
+    body = circle_polygon(n_points, 0.5 * L1, 0.5 * L2, R)
+    return dict(name=f"cylinder_mach10_{nx}x{ny}", params=p, body=body, steps=steps, U=U,
+                sample_from=sample_from)

@@ -1,0 +1,727 @@
+// C ABI implementation (include/dsmc_abi.h): handle, device buffers, the
+// step phases, hipGraph replay, per-kernel event timing and field readout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dsmc_abi.h"
+#include "dsmc_kernels.h"
+#include "host/case_setup.h"
+
+using namespace dsmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(DSMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+}  // namespace
+
+struct dsmc_handle_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Param P;
+    CutTables T;
+    DevCase cs{};
+    int ncell = 0, ntile_scan = 0;
+    int64_t cap = 0;
+    SoA A{}, B{};
+    int2 *cellslot = nullptr;
+    int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;
+    int *start = nullptr, *tile_first = nullptr, *ncoll_cell = nullptr;
+    float *sigcmax = nullptr, *mrem = nullptr;
+    double *acc = nullptr;
+    Ctrl *ctrl = nullptr;
+    float *d_seg = nullptr, *d_vol_ntc = nullptr, *d_ux = nullptr, *d_uy = nullptr, *d_sig = nullptr;
+    int *d_cseg_start = nullptr, *d_cseg_idx = nullptr;
+    uint8_t *d_cut = nullptr;
+    double *d_expected = nullptr;
+    int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
+    int keep_perm = 0, timing = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev[5]{};
+    double t_ms[4]{}, t_bytes[4]{};
+    int64_t t_launches = 0;
+    std::vector<void *> allocs;
+
+    template <class T>
+    hipError_t alloc(T **p, size_t n)
+    {
+        hipError_t e = dalloc(p, n);
+        if (e == hipSuccess) allocs.push_back((void *)*p);
+        return e;
+    }
+};
+
+namespace {
+
+int set_device(dsmc_handle h)
+{
+    HIPCHK(hipSetDevice(h->device));
+    return DSMC_OK;
+}
+
+int launch_step(dsmc_handle h, int phase_mask)
+{
+    hipStream_t st = h->stream;
+    if (phase_mask & 1) HIPCHK(launch_move(h->cs, h->A, h->B, h->cellslot, h->count, h->ctrl, h->grid_move, st));
+    if (phase_mask & 2) {
+        HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
+        HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
+                              h->ctrl, h->grid_scatter, st));
+    }
+    if (phase_mask & 4)
+        HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->tile_first, h->sigcmax, h->mrem,
+                           h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->keep_perm, h->grid_cell, st));
+    return DSMC_OK;
+}
+
+int read_ctrl(dsmc_handle h, Ctrl &c)
+{
+    HIPCHK(hipMemcpyAsync(&c, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+void drop_graph(dsmc_handle h)
+{
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->graph_steps = 0;
+}
+
+int reset_cell_state(dsmc_handle h)
+{
+    std::vector<float> s0(h->ncell, (float)h->P.sigcmax0);
+    HIPCHK(hipMemcpyAsync(h->sigcmax, s0.data(), sizeof(float) * h->ncell, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemsetAsync(h->mrem, 0, sizeof(float) * h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * 6 * (size_t)h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->count, 0, sizeof(int) * h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->ncoll_cell, 0, sizeof(int) * h->ncell, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+int64_t default_capacity(const Param &P, const CutTables &T)
+{
+    if (P.in.capacity > 0) return P.in.capacity;
+    double fill_est = 0;
+    for (int c = 0; c < T.ncell; ++c) fill_est += P.ppc_eff * T.frac[c];
+    return (int64_t)std::ceil(4.0 * fill_est) + 65536;
+}
+
+// Scalar constants of the kernels (device table pointers are set by the caller).
+DevCase make_devcase(const Param &P, const CutTables &T, int64_t cap)
+{
+    const dsmc_params *p = &P.in;
+    DevCase cs{};
+    cs.nx = p->N1; cs.ny = p->N2; cs.ncell = T.ncell; cs.nseg = T.nseg;
+    cs.L1 = (float)p->L1; cs.L2 = (float)p->L2; cs.dx = (float)P.dx; cs.dy = (float)P.dy;
+    cs.dt = (float)p->tau;
+    cs.ntc_coef = (float)P.ntc_coef; cs.K1 = (float)P.K1; cs.e_exp = (float)P.e_exp; cs.margin = T.margin;
+    cs.body_bc = p->wall_bc; cs.body_vmp = (float)P.body_vmp; cs.body_sig = (float)P.body_sig;
+    for (int f = 0; f < 4; ++f) {
+        cs.face_bc[f] = p->face_bc[f];
+        cs.face_vmp[f] = (float)P.face_vmp[f];
+        cs.face_sig[f] = (float)P.face_sig[f];
+        cs.inj_e[f] = P.inj_e[f];
+        cs.inj_a[f] = (float)P.inj_a[f];
+    }
+    cs.inj_vmp = (float)P.inj_vmp; cs.inj_sig = (float)P.inj_sig;
+    cs.inj_ux = (float)p->V_jet; cs.inj_uy = (float)p->V_y;
+    cs.seed = P.seed32; cs.cap = (uint32_t)cap;
+    return cs;
+}
+
+void make_info(const Param &P, const CutTables &T, const DevCase &cs, int64_t cap, dsmc_case_info *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->ncell = T.ncell; o->nseg = T.nseg; o->ncut = T.ncut; o->cseg_len = (int)T.cseg_idx.size();
+    o->Fn = P.Fn; o->dt = P.in.tau;
+    o->L1 = cs.L1; o->L2 = cs.L2; o->dx = cs.dx; o->dy = cs.dy;
+    o->ntc_coef = cs.ntc_coef; o->K1 = cs.K1; o->e_exp = cs.e_exp; o->sigcmax0 = (float)P.sigcmax0;
+    o->body_vmp = cs.body_vmp; o->body_sig = cs.body_sig;
+    for (int f = 0; f < 4; ++f) {
+        o->face_vmp[f] = cs.face_vmp[f]; o->face_sig[f] = cs.face_sig[f];
+        o->inj_e[f] = cs.inj_e[f]; o->inj_a[f] = cs.inj_a[f]; o->face_bc[f] = cs.face_bc[f];
+    }
+    o->inj_vmp = cs.inj_vmp; o->inj_sig = cs.inj_sig; o->inj_ux = cs.inj_ux; o->inj_uy = cs.inj_uy;
+    o->margin = cs.margin; o->body_bc = cs.body_bc; o->seed = cs.seed; o->cell_volume = P.Vcell;
+    o->ppc_eff = P.ppc_eff;
+    o->capacity = cap;
+}
+
+void copy_tables(const CutTables &T, float *seg, int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc,
+                 uint8_t *cut, double *frac)
+{
+    if (seg) std::copy(T.seg.begin(), T.seg.end(), seg);
+    if (cseg_start) std::copy(T.cseg_start.begin(), T.cseg_start.end(), cseg_start);
+    if (cseg_idx) std::copy(T.cseg_idx.begin(), T.cseg_idx.end(), cseg_idx);
+    if (vol) std::copy(T.vol.begin(), T.vol.end(), vol);
+    if (vol_ntc) std::copy(T.vol_ntc.begin(), T.vol_ntc.end(), vol_ntc);
+    if (cut) std::copy(T.cut.begin(), T.cut.end(), cut);
+    if (frac) std::copy(T.frac.begin(), T.frac.end(), frac);
+}
+
+int fill(dsmc_handle h, const std::vector<double> &expected, const std::vector<float> &ux,
+         const std::vector<float> &uy, const std::vector<float> &sig)
+{
+    double tot = 0;
+    for (double e : expected) tot += e;
+    if (tot + 6.0 * std::sqrt(tot + 1.0) + (double)h->ncell > (double)h->cap)
+        return fail(DSMC_ERR_CAPACITY, "initial fill does not fit the particle capacity");
+    int r = reset_cell_state(h);
+    if (r) return r;
+    const size_t C = h->ncell;
+    HIPCHK(hipMemcpyAsync(h->d_expected, expected.data(), sizeof(double) * C, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_ux, ux.data(), sizeof(float) * C, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_uy, uy.data(), sizeof(float) * C, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_sig, sig.data(), sizeof(float) * C, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(launch_fill(h->cs, h->A, h->d_expected, h->d_cut, h->d_ux, h->d_uy, h->d_sig, h->count, h->lstart,
+                       h->tsum, h->toff, h->ctrl, h->grid_fill, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dsmc_last_error(void) { return g_err.c_str(); }
+
+int dsmc_device_count(int *n)
+{
+    if (!n) return fail(DSMC_ERR_ARG, "null argument");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return DSMC_OK;
+}
+
+int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
+{
+    if (!p || !out) return fail(DSMC_ERR_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(DSMC_ERR_NODEVICE, "no HIP device");
+    dsmc_handle h = new dsmc_handle_s();
+    std::string msg = h->P.derive(*p);
+    if (!msg.empty()) { delete h; return fail(DSMC_ERR_ARG, msg); }
+    msg = build_cut_tables(h->P, body, h->T);
+    if (!msg.empty()) { delete h; return fail(DSMC_ERR_GEOMETRY, msg); }
+    const Param &P = h->P;
+    const CutTables &T = h->T;
+    h->ncell = T.ncell;
+    h->ntile_scan = (T.ncell + SCAN_TILE - 1) / SCAN_TILE;
+    h->cap = default_capacity(P, T);
+    if (h->cap > (int64_t)0x7fffffff - 1) { delete h; return fail(DSMC_ERR_CAPACITY, "capacity above 2^31"); }
+
+    auto bad = [&](hipError_t e, const char *what) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        dsmc_destroy(h);
+        return fail(DSMC_ERR_HIP, m);
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&h->device)) != hipSuccess) return bad(e, "hipGetDevice");
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
+    for (int k = 0; k < 5; ++k)
+        if ((e = hipEventCreate(&h->ev[k])) != hipSuccess) return bad(e, "event");
+    const size_t N = (size_t)h->cap, C = (size_t)T.ncell;
+    SoA *bufs[2] = { &h->A, &h->B };
+    for (SoA *s : bufs) {
+        if ((e = h->alloc(&s->x, N)) || (e = h->alloc(&s->y, N)) || (e = h->alloc(&s->vx, N)) ||
+            (e = h->alloc(&s->vy, N)) || (e = h->alloc(&s->vz, N)) || (e = h->alloc(&s->pid, N)))
+            return bad(e, "particle buffers");
+    }
+    const int ntiles_cell = (int)((N + CELL_P - 1) / CELL_P) + 1;
+    if ((e = h->alloc(&h->cellslot, N)) || (e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
+        (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||
+        (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, C + 1)) ||
+        (e = h->alloc(&h->tile_first, ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
+        (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, 6 * C)) ||
+        (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
+        (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||
+        (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) ||
+        (e = h->alloc(&h->d_expected, C)) || (e = h->alloc(&h->d_ux, C)) || (e = h->alloc(&h->d_uy, C)) ||
+        (e = h->alloc(&h->d_sig, C)))
+        return bad(e, "cell buffers");
+    auto up = [&](void *d, const void *s, size_t bytes) {
+        return bytes ? hipMemcpy(d, s, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    if ((e = up(h->d_seg, T.seg.data(), sizeof(float) * T.seg.size())) ||
+        (e = up(h->d_cseg_start, T.cseg_start.data(), sizeof(int) * (C + 1))) ||
+        (e = up(h->d_cseg_idx, T.cseg_idx.data(), sizeof(int) * T.cseg_idx.size())) ||
+        (e = up(h->d_vol_ntc, T.vol_ntc.data(), sizeof(float) * C)) ||
+        (e = up(h->d_cut, T.cut.data(), C)) || (e = hipMemset(h->ctrl, 0, sizeof(Ctrl))))
+        return bad(e, "table upload");
+
+    DevCase &cs = h->cs;
+    cs = make_devcase(P, T, h->cap);
+    cs.seg = h->d_seg; cs.cseg_start = h->d_cseg_start; cs.cseg_idx = h->d_cseg_idx; cs.vol_ntc = h->d_vol_ntc;
+
+    int cu = 0;
+    if ((e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device)) != hipSuccess)
+        return bad(e, "device attribute");
+    h->grid_move = cu * occupancy_move();
+    h->grid_scatter = cu * occupancy_scatter();
+    h->grid_cell = cu * occupancy_cell();
+    h->grid_fill = cu * 4;
+    int r = reset_cell_state(h);
+    if (r) { std::string m = g_err; dsmc_destroy(h); return fail(r, m); }
+    *out = h;
+    return DSMC_OK;
+}
+
+int dsmc_destroy(dsmc_handle h)
+{
+    if (!h) return DSMC_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    for (void *p : h->allocs) (void)hipFree(p);
+    for (int k = 0; k < 5; ++k)
+        if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return DSMC_OK;
+}
+
+int dsmc_case_info_get(dsmc_handle h, dsmc_case_info *o)
+{
+    if (!h || !o) return fail(DSMC_ERR_ARG, "null argument");
+    make_info(h->P, h->T, h->cs, h->cap, o);
+    return DSMC_OK;
+}
+
+int dsmc_case_tables(dsmc_handle h, float *seg, int *cseg_start, int *cseg_idx, double *vol,
+                     float *vol_ntc, uint8_t *cut)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    copy_tables(h->T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, nullptr);
+    return DSMC_OK;
+}
+
+int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_info *info, float *seg,
+                       int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc, uint8_t *cut,
+                       double *frac)
+{
+    if (!p || !info) return fail(DSMC_ERR_ARG, "null argument");
+    Param P;
+    CutTables T;
+    std::string msg = P.derive(*p);
+    if (!msg.empty()) return fail(DSMC_ERR_ARG, msg);
+    msg = build_cut_tables(P, body, T);
+    if (!msg.empty()) return fail(DSMC_ERR_GEOMETRY, msg);
+    int64_t cap = default_capacity(P, T);
+    DevCase cs = make_devcase(P, T, cap);
+    make_info(P, T, cs, cap, info);
+    copy_tables(T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, frac);
+    return DSMC_OK;
+}
+
+int dsmc_init_freestream(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    const size_t C = h->ncell;
+    std::vector<double> e(C);
+    for (size_t c = 0; c < C; ++c) e[c] = h->P.ppc_eff * h->T.frac[c];
+    std::vector<float> ux(C, (float)h->P.in.V_jet), uy(C, (float)h->P.in.V_y), sg(C, (float)h->P.inj_sig);
+    return fill(h, e, ux, uy, sg);
+}
+
+int dsmc_init_fields(dsmc_handle h, const double *n, const double *ux, const double *uy, const double *T)
+{
+    if (!h || !n || !ux || !uy || !T) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(h);
+    if (r) return r;
+    const size_t C = h->ncell;
+    std::vector<double> e(C);
+    std::vector<float> fx(C), fy(C), sg(C);
+    for (size_t c = 0; c < C; ++c) {
+        if (!(n[c] >= 0.0) || !(T[c] > 0.0) || !std::isfinite(ux[c]) || !std::isfinite(uy[c]))
+            return fail(DSMC_ERR_ARG, "init_fields: n must be >= 0, T > 0, velocities finite");
+        e[c] = h->P.ppc_eff * h->T.frac[c] * (n[c] / h->P.in.n_inf);
+        fx[c] = (float)ux[c];
+        fy[c] = (float)uy[c];
+        sg[c] = (float)std::sqrt(h->P.in.boltz * T[c] / h->P.in.mass);
+    }
+    return fill(h, e, fx, fy, sg);
+}
+
+int dsmc_upload_particles(dsmc_handle h, int64_t n, const float *x, const float *y, const float *vx,
+                          const float *vy, const float *vz, const uint32_t *pid)
+{
+    if (!h || n < 0 || (n > 0 && (!x || !y || !vx || !vy || !vz))) return fail(DSMC_ERR_ARG, "bad argument");
+    if (n > h->cap) return fail(DSMC_ERR_CAPACITY, "upload exceeds particle capacity");
+    int r = set_device(h);
+    if (r) return r;
+    std::vector<uint32_t> ids;
+    uint32_t next = (uint32_t)n;
+    if (!pid) {
+        ids.resize(n);
+        for (int64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+        pid = ids.data();
+    } else {
+        next = 0;
+        for (int64_t i = 0; i < n; ++i) next = std::max(next, pid[i] + 1u);
+    }
+    const size_t b = sizeof(float) * (size_t)n;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(h->A.x, x, b, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->A.y, y, b, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->A.vx, vx, b, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->A.vy, vy, b, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->A.vz, vz, b, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->A.pid, pid, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    }
+    HIPCHK(hipMemsetAsync(h->count, 0, sizeof(int) * h->ncell, h->stream));
+    HIPCHK(launch_ctrl_reset(h->ctrl, (uint32_t)n, next, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+namespace {
+
+int download_range(dsmc_handle h, bool moved, int64_t cap, float *x, float *y, float *vx, float *vy, float *vz,
+                   uint32_t *pid, int64_t *n_out)
+{
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    const uint32_t n = moved ? c.n_move : c.n_live;
+    if (n_out) *n_out = n;
+    if ((int64_t)n > cap) return fail(DSMC_ERR_ARG, "download buffer too small");
+    const SoA &S = (c.step & 1u) ? h->B : h->A;
+    const size_t b = sizeof(float) * n;
+    if (n) {
+        if (x) HIPCHK(hipMemcpyAsync(x, S.x, b, hipMemcpyDeviceToHost, h->stream));
+        if (y) HIPCHK(hipMemcpyAsync(y, S.y, b, hipMemcpyDeviceToHost, h->stream));
+        if (vx) HIPCHK(hipMemcpyAsync(vx, S.vx, b, hipMemcpyDeviceToHost, h->stream));
+        if (vy) HIPCHK(hipMemcpyAsync(vy, S.vy, b, hipMemcpyDeviceToHost, h->stream));
+        if (vz) HIPCHK(hipMemcpyAsync(vz, S.vz, b, hipMemcpyDeviceToHost, h->stream));
+        if (pid) HIPCHK(hipMemcpyAsync(pid, S.pid, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+}  // namespace
+
+int dsmc_download_particles(dsmc_handle h, int64_t cap, float *x, float *y, float *vx, float *vy, float *vz,
+                            uint32_t *pid, int64_t *n_out)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    return download_range(h, false, cap, x, y, vx, vy, vz, pid, n_out);
+}
+
+int dsmc_download_moved(dsmc_handle h, int64_t cap, float *x, float *y, float *vx, float *vy, float *vz,
+                        uint32_t *pid, int64_t *n_out)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    return download_range(h, true, cap, x, y, vx, vy, vz, pid, n_out);
+}
+
+int dsmc_move(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    return r ? r : launch_step(h, 1);
+}
+
+int dsmc_reassign(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    return r ? r : launch_step(h, 2);
+}
+
+int dsmc_collide(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    return r ? r : launch_step(h, 4);
+}
+
+namespace {
+
+constexpr int kGraphSteps = 8;
+
+int build_graph(dsmc_handle h)
+{
+    drop_graph(h);
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int r = DSMC_OK;
+    for (int s = 0; s < kGraphSteps && r == DSMC_OK; ++s) r = launch_step(h, 7);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+    if (e != hipSuccess) return fail(DSMC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    h->graph = g;
+    HIPCHK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    h->graph_steps = kGraphSteps;
+    return DSMC_OK;
+}
+
+// Algorithmic bytes of the four kernels of one step (formulas in DESIGN.md).
+void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, double out[4])
+{
+    const double C = h->ncell;
+    const double nl = before.n_live, nm = after.n_move, nn = after.n_live;
+    const double J = nm - nl;
+    const bool sampled = after.nsamples > before.nsamples;
+    out[0] = 40.0 * nl + 24.0 * J + 12.0 * nm + 4.0 * C;
+    out[1] = 8.0 * C;
+    out[2] = 12.0 * nm;
+    out[3] = 52.0 * nn + 32.0 * after.last_cand + 24.0 * after.last_coll + 16.0 * C +
+             (sampled ? 16.0 * nn + 96.0 * C : 0.0);
+}
+
+}  // namespace
+
+int dsmc_run(dsmc_handle h, int nsteps, int graph)
+{
+    if (!h || nsteps < 0) return fail(DSMC_ERR_ARG, "bad argument");
+    int r = set_device(h);
+    if (r) return r;
+    if (graph && !h->timing) {
+        if (!h->gexec) {
+            // capture launches the kernels into the graph only; state is untouched
+            if ((r = build_graph(h))) return r;
+        }
+        int full = nsteps / h->graph_steps;
+        for (int k = 0; k < full; ++k) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+        for (int k = 0; k < nsteps % h->graph_steps; ++k)
+            if ((r = launch_step(h, 7))) return r;
+        return DSMC_OK;
+    }
+    if (!h->timing) {
+        for (int k = 0; k < nsteps; ++k)
+            if ((r = launch_step(h, 7))) return r;
+        return DSMC_OK;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        Ctrl before, after;
+        if ((r = read_ctrl(h, before))) return r;
+        hipStream_t st = h->stream;
+        HIPCHK(hipEventRecord(h->ev[0], st));
+        if ((r = launch_step(h, 1))) return r;
+        HIPCHK(hipEventRecord(h->ev[1], st));
+        HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
+        HIPCHK(hipEventRecord(h->ev[2], st));
+        HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
+                              h->ctrl, h->grid_scatter, st));
+        HIPCHK(hipEventRecord(h->ev[3], st));
+        if ((r = launch_step(h, 4))) return r;
+        HIPCHK(hipEventRecord(h->ev[4], st));
+        HIPCHK(hipEventSynchronize(h->ev[4]));
+        if ((r = read_ctrl(h, after))) return r;
+        double by[4];
+        step_bytes(h, before, after, by);
+        for (int q = 0; q < 4; ++q) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, h->ev[q], h->ev[q + 1]));
+            h->t_ms[q] += ms;
+            h->t_bytes[q] += by[q];
+        }
+        h->t_launches += 1;
+    }
+    return DSMC_OK;
+}
+
+int dsmc_sync(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+int dsmc_status_get(dsmc_handle h, dsmc_status *o)
+{
+    if (!h || !o) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    o->step = c.step; o->n_live = c.n_live; o->next_pid = c.next_pid; o->err = c.err;
+    o->particle_steps = c.psteps; o->collisions = c.ncoll; o->nsamples = c.nsamples;
+    o->last_collisions = c.last_coll; o->n_move = c.n_move;
+    return DSMC_OK;
+}
+
+int dsmc_set_keep_perm(dsmc_handle h, int on)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    h->keep_perm = on ? 1 : 0;
+    drop_graph(h);
+    return DSMC_OK;
+}
+
+int dsmc_download_bins(dsmc_handle h, int64_t cap, int *cell, int *count)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    if (cell) {
+        if ((int64_t)c.n_move > cap) return fail(DSMC_ERR_ARG, "cell buffer too small");
+        std::vector<int2> cs(c.n_move);
+        if (c.n_move)
+            HIPCHK(hipMemcpy(cs.data(), h->cellslot, sizeof(int2) * c.n_move, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < c.n_move; ++i) cell[i] = cs[i].x;
+    }
+    if (count) HIPCHK(hipMemcpy(count, h->count, sizeof(int) * h->ncell, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+int dsmc_download_start(dsmc_handle h, int *start)
+{
+    if (!h || !start) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(start, h->start, sizeof(int) * (h->ncell + 1), hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+int dsmc_download_perm(dsmc_handle h, int64_t cap, int *perm)
+{
+    if (!h || !perm) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    if ((int64_t)c.n_live > cap) return fail(DSMC_ERR_ARG, "perm buffer too small");
+    if (c.n_live) HIPCHK(hipMemcpy(perm, h->perm, sizeof(int) * c.n_live, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+int dsmc_cell_state(dsmc_handle h, float *sigcmax, float *mrem, int *ncoll)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t C = h->ncell;
+    if (sigcmax) HIPCHK(hipMemcpy(sigcmax, h->sigcmax, sizeof(float) * C, hipMemcpyDeviceToHost));
+    if (mrem) HIPCHK(hipMemcpy(mrem, h->mrem, sizeof(float) * C, hipMemcpyDeviceToHost));
+    if (ncoll) HIPCHK(hipMemcpy(ncoll, h->ncoll_cell, sizeof(int) * C, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+int dsmc_set_cell_state(dsmc_handle h, const float *sigcmax, const float *mrem)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    const size_t C = h->ncell;
+    if (sigcmax) HIPCHK(hipMemcpy(h->sigcmax, sigcmax, sizeof(float) * C, hipMemcpyHostToDevice));
+    if (mrem) HIPCHK(hipMemcpy(h->mrem, mrem, sizeof(float) * C, hipMemcpyHostToDevice));
+    return DSMC_OK;
+}
+
+int dsmc_reset_samples(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * 6 * (size_t)h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(&h->ctrl->nsamples, 0, sizeof(unsigned long long), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    if (nsamples) *nsamples = c.nsamples;
+    if (acc) HIPCHK(hipMemcpy(acc, h->acc, sizeof(double) * 6 * (size_t)h->ncell, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy, double *uz, double *T, double *p)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    const size_t C = h->ncell;
+    std::vector<double> a(6 * C);
+    uint64_t ns = 0;
+    int r = dsmc_download_samples(h, a.data(), &ns);
+    if (r) return r;
+    const double k = h->P.in.boltz, m = h->P.in.mass, Fn = h->P.Fn;
+    for (size_t c = 0; c < C; ++c) {
+        const double sn = a[c];
+        double vol = h->T.vol[c];
+        double nn = (ns > 0 && vol > 0) ? sn / (double)ns * Fn / vol : 0.0;
+        double u = 0, v = 0, w = 0, t = 0;
+        if (sn > 0) {
+            u = a[C + c] / sn; v = a[2 * C + c] / sn; w = a[3 * C + c] / sn;
+            t = m / (3.0 * k) * (a[4 * C + c] / sn - (u * u + v * v + w * w));
+        }
+        if (n) n[c] = nn;
+        if (ux) ux[c] = u;
+        if (uy) uy[c] = v;
+        if (uz) uz[c] = w;
+        if (T) T[c] = t;
+        if (p) p[c] = nn * k * t;
+    }
+    return DSMC_OK;
+}
+
+int dsmc_set_timing(dsmc_handle h, int on)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    h->timing = on ? 1 : 0;
+    for (int q = 0; q < 4; ++q) h->t_ms[q] = h->t_bytes[q] = 0.0;
+    h->t_launches = 0;
+    return DSMC_OK;
+}
+
+int dsmc_kernel_times(dsmc_handle h, double ms[4], double bytes[4], int64_t *launches)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    for (int q = 0; q < 4; ++q) {
+        double n = h->t_launches ? (double)h->t_launches : 1.0;
+        if (ms) ms[q] = h->t_ms[q] / n;
+        if (bytes) bytes[q] = h->t_bytes[q] / n;
+    }
+    if (launches) *launches = h->t_launches;
+    return DSMC_OK;
+}
+
+}  // extern "C"

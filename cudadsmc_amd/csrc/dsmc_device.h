@@ -1,0 +1,328 @@
+// Device building blocks of the DSMC hot path on gfx950.
+//
+// Arithmetic contract (shared with the CPU oracle, which is why results are
+// bit-identical): FP32 state, every + - * / sqrt correctly rounded and
+// never fused (compile with -ffp-contract=off), log / exp / sin-cos(2 pi u)
+// from the routines below (IEEE basic operations only), FP64 only for the
+// sampling sums and the inflow remainder.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dsmc {
+
+enum { BC_STREAM = 0, BC_SPECULAR = 1, BC_DIFFUSE = 2 };
+enum { SUB_MOVE = 1, SUB_INJECT = 2, SUB_COLLIDE = 3, SUB_INIT = 4 };
+enum { ERR_BOUNCE = 1, ERR_INFLOW = 2, ERR_CAPACITY = 4, ERR_FILL = 8 };
+constexpr int MAX_BOUNCE = 64;
+constexpr int INFLOW_TRIES = 256;
+constexpr int FILL_TRIES = 256;
+
+// Structure-of-arrays particle state (24 B per particle).
+struct SoA {
+    float *x, *y, *vx, *vy, *vz;
+    uint32_t *pid;
+};
+
+// Per-instance constants, passed by value to every kernel.
+struct DevCase {
+    int nx, ny, ncell, nseg;
+    float L1, L2, dx, dy, dt;
+    float ntc_coef, K1, e_exp, margin;
+    int body_bc;
+    float body_vmp, body_sig;
+    int face_bc[4];
+    float face_vmp[4], face_sig[4];
+    double inj_e[4];
+    float inj_a[4];
+    float inj_vmp, inj_sig, inj_ux, inj_uy;
+    uint32_t seed, cap;
+    const float *seg;
+    const int *cseg_start, *cseg_idx;
+    const float *vol_ntc;
+};
+
+// Device-resident step control: everything a step needs that changes
+// between steps lives here, so a step has no host round trip and can be
+// captured in a hipGraph.
+struct Ctrl {
+    uint32_t step, n_live, n_move, n_next;
+    uint32_t next_pid, err, scan_done, cell_done;
+    uint32_t step_coll, last_coll, step_cand, last_cand;
+    unsigned long long psteps, ncoll, nsamples, ncand;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11), counter-based: key = (id, seed),
+// counter = (draw, step, substream, extra).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+__device__ __forceinline__ float u01(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
+__device__ __forceinline__ float u01o(uint32_t w) { return (float)((w >> 8) + 1u) * 0x1p-24f; }
+__device__ __forceinline__ int mulhi_idx(uint32_t w, int n) { return (int)__umulhi(w, (uint32_t)n); }
+
+// ---------------------------------------------------------------------------
+// log, exp, sin/cos(2 pi u) from IEEE basic operations.
+// ---------------------------------------------------------------------------
+#define DM_LN2_HI 0x1.62e400p-1f
+#define DM_LN2_LO 0x1.7f7d1cp-20f
+#define DM_LOG2E  0x1.715476p+0f
+#define DM_PIO2   0x1.921fb6p+0f
+
+// x = 2^e m, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh((m-1)/(m+1)).
+__device__ __forceinline__ float dm_logf(float x)
+{
+    uint32_t ix = __float_as_uint(x);
+    int e = (int)((ix >> 23) & 0xffu) - 127;
+    float m = __uint_as_float((ix & 0x007fffffu) | 0x3f800000u);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e = e + 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    float p = s2 * (0x1.555556p-2f + s2 * (0x1.99999ap-3f + s2 * (0x1.24924ap-3f + s2 * 0x1.c71c72p-4f)));
+    float two_s = 2.0f * s;
+    float logm = two_s + two_s * p;
+    float ef = (float)e;
+    return ef * DM_LN2_HI + (ef * DM_LN2_LO + logm);
+}
+
+// x = k ln2 + r (Cody-Waite), e^r by Taylor to r^7.
+__device__ __forceinline__ float dm_expf(float x)
+{
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    float kf = floorf(x * DM_LOG2E + 0.5f);
+    float r = (x - kf * DM_LN2_HI) - kf * DM_LN2_LO;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f +
+              r * (0x1.111112p-7f + r * (0x1.6c16c2p-10f + r * 0x1.a01a02p-13f))))));
+    int k = (int)kf;
+    if (k > 127) return p * __uint_as_float((uint32_t)(127 + 127) << 23) * 2.0f;
+    if (k < -126) return p * __uint_as_float((uint32_t)(k + 126 + 127) << 23) * 0x1p-126f;
+    return p * __uint_as_float((uint32_t)(k + 127) << 23);
+}
+
+// sin, cos of 2 pi u, u in [0,1): exact quadrant reduction, Taylor on |t|<=pi/4.
+__device__ __forceinline__ void dm_sincos2pi(float u, float &sn, float &cs)
+{
+    float t = u * 4.0f;
+    float q = floorf(t + 0.5f);
+    float f = t - q;
+    float th = f * DM_PIO2;
+    float t2 = th * th;
+    float sp = th + th * t2 * (-0x1.555556p-3f + t2 * (0x1.111112p-7f + t2 * (-0x1.a01a02p-13f + t2 * 0x1.71de3ap-19f)));
+    float cp = 1.0f + t2 * (-0.5f + t2 * (0x1.555556p-5f + t2 * (-0x1.6c16c2p-10f + t2 * (0x1.a01a02p-16f + t2 * -0x1.27e4fcp-22f))));
+    int qi = ((int)q) & 3;
+    if (qi == 0)      { sn = sp;  cs = cp; }
+    else if (qi == 1) { sn = cp;  cs = -sp; }
+    else if (qi == 2) { sn = -sp; cs = -cp; }
+    else              { sn = -cp; cs = sp; }
+}
+
+// Cell index (cudaDSMC GPUCell.cu:184-189: i = (int)(x/dx), cell = j + i*ny),
+// clamped into the grid.
+__device__ __forceinline__ int cell_of(const DevCase &cs, float x, float y)
+{
+    int i = (int)(x / cs.dx);
+    int j = (int)(y / cs.dy);
+    i = i < 0 ? 0 : (i > cs.nx - 1 ? cs.nx - 1 : i);
+    j = j < 0 ? 0 : (j > cs.ny - 1 ? cs.ny - 1 : j);
+    return j + i * cs.ny;
+}
+
+// Diffuse re-emission from a wall with unit normal (nx, ny) into the fluid:
+// flux-weighted normal speed, independent Gaussian tangential and z parts.
+__device__ __forceinline__ void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
+                                             uint32_t seed, uint32_t step, uint32_t b,
+                                             float &vx, float &vy, float &vz)
+{
+    uint4 w = philox(b, step, SUB_MOVE, 0u, pid, seed);
+    float vn = sqrtf(-dm_logf(u01o(w.x))) * vmp;
+    float r = sqrtf(-2.0f * dm_logf(u01o(w.y))) * sig;
+    float s, c;
+    dm_sincos2pi(u01(w.z), s, c);
+    float vt = r * c;
+    vx = vn * nx + vt * (-ny);
+    vy = vn * ny + vt * nx;
+    vz = r * s;
+}
+
+// Flight of duration t with wall events (faces and body segments), earliest
+// event first.  Returns true if the particle left through a stream face.
+// pid is only read (from *pidp) when a diffuse wall needs random numbers.
+__device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, float &vx, float &vy,
+                                         float &vz, const uint32_t *pidp, uint32_t pid_known,
+                                         bool have_pid, uint32_t step, float t, uint32_t &err,
+                                         bool &vchg)
+{
+    bool dead = false;
+    for (int b = 0;; ++b) {
+        if (b == MAX_BOUNCE) { err |= ERR_BOUNCE; break; }
+        float best = t;
+        int kind = -1;
+        if (vx < 0.0f) {
+            float s = (0.0f - x) / vx;
+            if (s < best) { best = s; kind = 0; }
+        } else if (vx > 0.0f) {
+            float s = (cs.L1 - x) / vx;
+            if (s < best) { best = s; kind = 1; }
+        }
+        if (vy < 0.0f) {
+            float s = (0.0f - y) / vy;
+            if (s < best) { best = s; kind = 2; }
+        } else if (vy > 0.0f) {
+            float s = (cs.L2 - y) / vy;
+            if (s < best) { best = s; kind = 3; }
+        }
+        if (cs.nseg > 0) {
+            float reach = sqrtf(vx * vx + vy * vy) * t;
+            bool all = reach > cs.margin;
+            int cnt, c0 = 0;
+            if (all) {
+                cnt = cs.nseg;
+            } else {
+                int c = cell_of(cs, x, y);
+                c0 = cs.cseg_start[c];
+                cnt = cs.cseg_start[c + 1] - c0;
+            }
+            for (int q = 0; q < cnt; ++q) {
+                int k = all ? q : cs.cseg_idx[c0 + q];
+                const float *S = cs.seg + 6 * k;
+                float snx = S[4], sny = S[5];
+                float vn = vx * snx + vy * sny;
+                if (!(vn < 0.0f)) continue;
+                float ax = S[0], ay = S[1];
+                float ex = S[2] - ax, ey = S[3] - ay;
+                float den = vx * ey - vy * ex;
+                if (den == 0.0f) continue;
+                float wx = ax - x, wy = ay - y;
+                float s = (wx * ey - wy * ex) / den;
+                if (!(s > 0.0f && s < best)) continue;
+                float u = (wx * vy - wy * vx) / den;
+                if (u < 0.0f || u > 1.0f) continue;
+                best = s;
+                kind = 4 + k;
+            }
+        }
+        if (kind < 0) {
+            x = x + vx * t;
+            y = y + vy * t;
+            break;
+        }
+        x = x + vx * best;
+        y = y + vy * best;
+        t = t - best;
+        vchg = true;
+        if (kind < 4) {
+            if (kind == 0) x = 0.0f;
+            else if (kind == 1) x = cs.L1;
+            else if (kind == 2) y = 0.0f;
+            else y = cs.L2;
+            int bc = cs.face_bc[kind];
+            if (bc == BC_STREAM) { dead = true; break; }
+            if (bc == BC_SPECULAR) {
+                if (kind < 2) vx = -vx; else vy = -vy;
+            } else {
+                float nx = kind == 0 ? 1.0f : (kind == 1 ? -1.0f : 0.0f);
+                float ny = kind == 2 ? 1.0f : (kind == 3 ? -1.0f : 0.0f);
+                uint32_t pid = have_pid ? pid_known : *pidp;
+                diffuse_emit(nx, ny, cs.face_vmp[kind], cs.face_sig[kind], pid, cs.seed, step,
+                             (uint32_t)b, vx, vy, vz);
+            }
+        } else {
+            const float *S = cs.seg + 6 * (kind - 4);
+            float snx = S[4], sny = S[5];
+            if (cs.body_bc == BC_SPECULAR) {
+                float vn = vx * snx + vy * sny;
+                vx = vx - 2.0f * vn * snx;
+                vy = vy - 2.0f * vn * sny;
+            } else {
+                uint32_t pid = have_pid ? pid_known : *pidp;
+                diffuse_emit(snx, sny, cs.body_vmp, cs.body_sig, pid, cs.seed, step, (uint32_t)b,
+                             vx, vy, vz);
+            }
+        }
+    }
+    return dead;
+}
+
+// Inflow particles created at `step` on each stream face: exact remainder
+// carry floor((s+1) e) - floor(s e) of the expected flux e per step.
+__device__ __forceinline__ void inject_counts(const DevCase &cs, uint32_t step, int J[4])
+{
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        double e = cs.inj_e[f];
+        J[f] = (cs.face_bc[f] == BC_STREAM && e > 0.0)
+                   ? (int)(floor(((double)step + 1.0) * e) - floor((double)step * e))
+                   : 0;
+    }
+}
+
+// One inflow particle on face f: position on the face, drifting-Maxwellian
+// flux-weighted velocity, then flight for a random fraction of dt.
+__device__ __forceinline__ bool inject_one(const DevCase &cs, int f, uint32_t pid, uint32_t step,
+                                           float &x, float &y, float &vx, float &vy, float &vz,
+                                           uint32_t &err)
+{
+    uint4 w = philox(0u, step, SUB_INJECT, 0u, pid, cs.seed);
+    float upos = u01(w.x);
+    float ufrac = u01(w.y);
+    float r = sqrtf(-2.0f * dm_logf(u01o(w.z)));
+    float sn, cn;
+    dm_sincos2pi(u01(w.w), sn, cn);
+    float g0 = r * cn, g1 = r * sn;
+
+    float a = cs.inj_a[f];
+    float lo, hi;
+    if (a >= -3.5f) {
+        lo = a - 3.5f;
+        if (lo < 0.0f) lo = 0.0f;
+        hi = (a > 0.0f ? a : 0.0f) + 3.5f;
+    } else {
+        lo = 0.0f;
+        hi = -4.0f / a;
+    }
+    float ws = 0.5f * (a + sqrtf(a * a + 2.0f));
+    float dws = ws - a;
+    float ref = dws * dws;
+    float wn = lo;
+    bool ok = false;
+    for (int tr = 0; tr < INFLOW_TRIES && !ok; tr += 2) {
+        uint4 q = philox(1u + (uint32_t)(tr / 2), step, SUB_INJECT, 0u, pid, cs.seed);
+        float cand = lo + (hi - lo) * u01(q.x);
+        float d = cand - a;
+        float acc = (cand / ws) * dm_expf(ref - d * d);
+        wn = cand;
+        if (u01(q.y) < acc) { ok = true; break; }
+        cand = lo + (hi - lo) * u01(q.z);
+        d = cand - a;
+        acc = (cand / ws) * dm_expf(ref - d * d);
+        wn = cand;
+        if (u01(q.w) < acc) { ok = true; break; }
+    }
+    if (!ok) err |= ERR_INFLOW;
+    float vn = wn * cs.inj_vmp;
+    vz = cs.inj_sig * g1;
+    if (f == 0)      { x = 0.0f;         y = upos * cs.L2; vx = vn;  vy = cs.inj_uy + cs.inj_sig * g0; }
+    else if (f == 1) { x = cs.L1;        y = upos * cs.L2; vx = -vn; vy = cs.inj_uy + cs.inj_sig * g0; }
+    else if (f == 2) { x = upos * cs.L1; y = 0.0f;         vy = vn;  vx = cs.inj_ux + cs.inj_sig * g0; }
+    else             { x = upos * cs.L1; y = cs.L2;        vy = -vn; vx = cs.inj_ux + cs.inj_sig * g0; }
+    bool vchg = false;
+    return move_one(cs, x, y, vx, vy, vz, nullptr, pid, true, step, ufrac * cs.dt, err, vchg);
+}
+
+}  // namespace dsmc

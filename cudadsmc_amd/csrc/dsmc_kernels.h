@@ -1,0 +1,37 @@
+// Kernel launchers and tile constants shared by dsmc_kernels.hip and the
+// C ABI implementation.
+#pragma once
+
+#include "dsmc_device.h"
+
+namespace dsmc {
+
+constexpr int SCAN_IPT = 8;                  // cells per thread in k_scan
+constexpr int SCAN_TILE = 256 * SCAN_IPT;    // cells per k_scan block
+constexpr int CELL_P = 2048;                 // particles per k_cell tile (nominal)
+constexpr int CELL_CAP = 4096;               // LDS staging capacity of one tile
+constexpr int CELL_BIG = 64;                 // cells with > 64 particles listed per tile
+constexpr int CELL_WAVES = 4;                // waves per k_cell block
+
+hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, int2 *cellslot, int *count,
+                       Ctrl *ctrl, int grid, hipStream_t st);
+hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell, Ctrl *ctrl,
+                       hipStream_t st);
+hipError_t launch_scatter(const int2 *cellslot, const int *lstart, const int *toff, int *start,
+                          int *perm, int *tile_first, int ncell, const Ctrl *ctrl, int grid,
+                          hipStream_t st);
+hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
+                       const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
+                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int grid,
+                       hipStream_t st);
+hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, const uint8_t *cut,
+                       const float *ux, const float *uy, const float *sig, int *count, int *lstart,
+                       int *tsum, int *toff, Ctrl *ctrl, int grid, hipStream_t st);
+hipError_t launch_ctrl_reset(Ctrl *ctrl, uint32_t n, uint32_t next_pid, hipStream_t st);
+
+// Resident blocks per CU of the persistent kernels (occupancy query).
+int occupancy_move();
+int occupancy_cell();
+int occupancy_scatter();
+
+}  // namespace dsmc

@@ -1,0 +1,58 @@
+// Host-side case construction: the roles of the reference's src/Param.h,
+// src/meshes/CartesianMesh.{h,cpp} and src/object/Geom.{h,cpp}.
+//
+//  - Param:          derived constants of Run::initialize (Run.cpp:29-35:
+//                    Vstd, Vmax, VHS_coe) plus the inflow flux per face;
+//  - CartesianMesh:  uniform grid, cell id = k + j*nz + i*ny*nz with nz = 1
+//                    (CartesianMesh.cpp:171-181), cell fluid volume
+//                    (CartesianMesh.cpp:100-169);
+//  - Geom:           closed body polygon, one wall segment per polygon edge,
+//                    and per-cell lists of the segments a particle starting
+//                    in that cell can reach within one cell width
+//                    (Geom.cpp:7-72 builds one chord per cut cell instead).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/dsmc_abi.h"
+
+namespace dsmc {
+
+struct Param {
+    dsmc_params in{};
+    // derived (double, as in Run::initialize)
+    double dx = 0, dy = 0, Vcell = 0, Fn = 0, ppc_eff = 0;
+    double Vstd = 0, Vmax = 0, VHS_coe = 0;
+    double K1 = 0, e_exp = 0, ntc_coef = 0, sigcmax0 = 0;
+    double body_vmp = 0, body_sig = 0;
+    double face_vmp[4]{}, face_sig[4]{};
+    double inj_e[4]{}, inj_a[4]{};
+    double inj_vmp = 0, inj_sig = 0;
+    uint32_t seed32 = 0;
+
+    // Validates the input and computes the derived constants.
+    // Returns an empty string on success, else the reason.
+    std::string derive(const dsmc_params &p);
+};
+
+struct CutTables {
+    int ncell = 0, nseg = 0, ncut = 0;
+    std::vector<float> seg;          // 6 per segment: ax ay bx by nx ny
+    std::vector<int> cseg_start;     // ncell + 1
+    std::vector<int> cseg_idx;
+    std::vector<double> vol;         // fluid volume (dx*dy*L3 minus body), m^3
+    std::vector<double> frac;        // fluid fraction, exactly 1.0 for uncut cells
+    std::vector<float> vol_ntc;      // volume used by NTC (floored at 0.1 cell)
+    std::vector<uint8_t> cut;        // 0 < frac < 1
+    float margin = 0;
+};
+
+// Builds the cut-cell tables.  body may be null (no immersed body).
+std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T);
+
+// Cell id of cell (i, j): the reference convention with nz = 1.
+inline int cell_id(int i, int j, int ny) { return j + i * ny; }
+
+}  // namespace dsmc

@@ -1,0 +1,196 @@
+"""Run: the driver of one DSMC instance over the C ABI.
+
+Mirrors the reference driver's interface (src/Run.h:30-44 --
+initialize, solver, particlemove, ressignParticle, collision, finalize and
+the Transfer* methods) with the state on the GPU; every call goes to
+libcudadsmc.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import check, ptr
+
+
+def _body_struct(body):
+    if body is None:
+        return None, None
+    xy = np.ascontiguousarray(np.asarray(body, dtype=np.float64).reshape(-1, 2))
+    return abi.dsmc_body(xy.shape[0], xy.ctypes.data_as(C.POINTER(C.c_double))), xy
+
+
+def describe(params, body=None):
+    """Host-only case description (no device needed): derived constants and
+    cut-cell tables, as dsmc_case_describe returns them."""
+    lib = abi.load()
+    b, _xy = _body_struct(body)
+    bptr = C.byref(b) if b is not None else None
+    info = abi.dsmc_case_info()
+    check(lib.dsmc_case_describe(C.byref(params), bptr, C.byref(info), None, None, None, None, None, None, None))
+    nc = info.ncell
+    t = dict(seg=np.empty(6 * info.nseg, np.float32), cseg_start=np.empty(nc + 1, np.int32),
+             cseg_idx=np.empty(max(info.cseg_len, 1), np.int32), vol=np.empty(nc, np.float64),
+             vol_ntc=np.empty(nc, np.float32), cut=np.empty(nc, np.uint8), frac=np.empty(nc, np.float64))
+    check(lib.dsmc_case_describe(C.byref(params), bptr, C.byref(info), ptr(t["seg"], C.c_float),
+                                 ptr(t["cseg_start"], C.c_int), ptr(t["cseg_idx"], C.c_int),
+                                 ptr(t["vol"], C.c_double), ptr(t["vol_ntc"], C.c_float),
+                                 ptr(t["cut"], C.c_uint8), ptr(t["frac"], C.c_double)))
+    return info, t
+
+
+class Run:
+    def __init__(self, params, body=None):
+        self.lib = abi.load()
+        self.params = params
+        self.h = C.c_void_p()
+        self._body_xy = None
+        bptr = None
+        if body is not None:
+            xy = np.ascontiguousarray(np.asarray(body, dtype=np.float64).reshape(-1, 2))
+            self._body_xy = xy
+            b = abi.dsmc_body(xy.shape[0], xy.ctypes.data_as(C.POINTER(C.c_double)))
+            self._body = b
+            bptr = C.byref(b)
+        check(self.lib.dsmc_create(C.byref(params), bptr, C.byref(self.h)))
+        self.info = abi.dsmc_case_info()
+        check(self.lib.dsmc_case_info_get(self.h, C.byref(self.info)))
+        self.ncell = self.info.ncell
+
+    # -- lifecycle (Run::initialize / finalize, Run.cpp:29-106,306-311) --
+    def initialize(self):
+        """Freestream fill of every fluid cell (Run::assignParticle)."""
+        check(self.lib.dsmc_init_freestream(self.h))
+
+    def initialize_fields(self, n, ux, uy, T):
+        """Fill from per-cell number density, velocity and temperature."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (n, ux, uy, T)]
+        check(self.lib.dsmc_init_fields(self.h, *[ptr(v, C.c_double) for v in a]))
+
+    def finalize(self):
+        if self.h:
+            self.lib.dsmc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    close = finalize
+
+    def __del__(self):
+        try:
+            self.finalize()
+        except Exception:
+            pass
+
+    # -- the three phases of Run::solver (Run.cpp:159-162,245-256) --
+    def particlemove(self):
+        check(self.lib.dsmc_move(self.h))
+
+    def ressignParticle(self):
+        check(self.lib.dsmc_reassign(self.h))
+
+    def collision(self):
+        check(self.lib.dsmc_collide(self.h))
+
+    def solver(self, nsteps, graph=True):
+        check(self.lib.dsmc_run(self.h, int(nsteps), 1 if graph else 0))
+
+    def sync(self):
+        check(self.lib.dsmc_sync(self.h))
+
+    # -- state transfer (Run::Transfer*, Run.cpp:164-238) --
+    def status(self):
+        s = abi.dsmc_status()
+        check(self.lib.dsmc_status_get(self.h, C.byref(s)))
+        return s
+
+    def upload(self, x, y, vx, vy, vz, pid=None):
+        a = [abi.f32(v) for v in (x, y, vx, vy, vz)]
+        n = a[0].shape[0]
+        p = None if pid is None else np.ascontiguousarray(pid, dtype=np.uint32)
+        check(self.lib.dsmc_upload_particles(self.h, n, *[ptr(v, C.c_float) for v in a],
+                                             ptr(p, C.c_uint32)))
+
+    def download(self, moved=False):
+        """Current state (n_live particles), or with moved=True the n_move
+        particles processed by the last particlemove()."""
+        s = self.status()
+        n = s.n_move if moved else s.n_live
+        out = [np.empty(max(n, 1), np.float32) for _ in range(5)]
+        pid = np.empty(max(n, 1), np.uint32)
+        nn = C.c_int64()
+        fn = self.lib.dsmc_download_moved if moved else self.lib.dsmc_download_particles
+        check(fn(self.h, n, *[ptr(v, C.c_float) for v in out], ptr(pid, C.c_uint32), C.byref(nn)))
+        return dict(x=out[0][:n], y=out[1][:n], vx=out[2][:n], vy=out[3][:n], vz=out[4][:n], pid=pid[:n])
+
+    def tables(self):
+        i = self.info
+        seg = np.empty(6 * i.nseg, np.float32)
+        cs = np.empty(i.ncell + 1, np.int32)
+        ci = np.empty(max(i.cseg_len, 1), np.int32)
+        vol = np.empty(i.ncell, np.float64)
+        vn = np.empty(i.ncell, np.float32)
+        cut = np.empty(i.ncell, np.uint8)
+        check(self.lib.dsmc_case_tables(self.h, ptr(seg, C.c_float), ptr(cs, C.c_int), ptr(ci, C.c_int),
+                                        ptr(vol, C.c_double), ptr(vn, C.c_float), ptr(cut, C.c_uint8)))
+        return dict(seg=seg, cseg_start=cs, cseg_idx=ci, vol=vol, vol_ntc=vn, cut=cut)
+
+    def bins(self):
+        s = self.status()
+        cell = np.empty(max(s.n_move, 1), np.int32)
+        count = np.empty(self.ncell, np.int32)
+        check(self.lib.dsmc_download_bins(self.h, s.n_move, ptr(cell, C.c_int), ptr(count, C.c_int)))
+        return cell[:s.n_move], count
+
+    def start(self):
+        st = np.empty(self.ncell + 1, np.int32)
+        check(self.lib.dsmc_download_start(self.h, ptr(st, C.c_int)))
+        return st
+
+    def perm(self):
+        s = self.status()
+        p = np.empty(max(s.n_live, 1), np.int32)
+        check(self.lib.dsmc_download_perm(self.h, s.n_live, ptr(p, C.c_int)))
+        return p[:s.n_live]
+
+    def set_keep_perm(self, on=True):
+        check(self.lib.dsmc_set_keep_perm(self.h, 1 if on else 0))
+
+    def cell_state(self):
+        sm = np.empty(self.ncell, np.float32)
+        mr = np.empty(self.ncell, np.float32)
+        nc = np.empty(self.ncell, np.int32)
+        check(self.lib.dsmc_cell_state(self.h, ptr(sm, C.c_float), ptr(mr, C.c_float), ptr(nc, C.c_int)))
+        return sm, mr, nc
+
+    def set_cell_state(self, sigcmax=None, mrem=None):
+        a = None if sigcmax is None else abi.f32(sigcmax)
+        b = None if mrem is None else abi.f32(mrem)
+        check(self.lib.dsmc_set_cell_state(self.h, ptr(a, C.c_float), ptr(b, C.c_float)))
+
+    # -- sampling (GPUCells::Sample / DownloadToHost) --
+    def reset_samples(self):
+        check(self.lib.dsmc_reset_samples(self.h))
+
+    def samples(self):
+        acc = np.empty(6 * self.ncell, np.float64)
+        ns = C.c_uint64()
+        check(self.lib.dsmc_download_samples(self.h, ptr(acc, C.c_double), C.byref(ns)))
+        return acc.reshape(6, self.ncell), ns.value
+
+    def fields(self):
+        out = [np.empty(self.ncell, np.float64) for _ in range(6)]
+        check(self.lib.dsmc_download_fields(self.h, *[ptr(v, C.c_double) for v in out]))
+        shape = (self.params.N1, self.params.N2)
+        names = ("n", "ux", "uy", "uz", "T", "p")
+        return {k: v.reshape(shape) for k, v in zip(names, out)}
+
+    # -- per-kernel timing --
+    def set_timing(self, on=True):
+        check(self.lib.dsmc_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(4)
+        by = np.zeros(4)
+        n = C.c_int64()
+        check(self.lib.dsmc_kernel_times(self.h, ptr(ms, C.c_double), ptr(by, C.c_double), C.byref(n)))
+        names = ("k_move", "k_scan", "k_scatter", "k_cell")
+        return {k: dict(ms=float(m), bytes=float(b)) for k, m, b in zip(names, ms, by)}, n.value

@@ -1,0 +1,197 @@
+/*
+ * dsmc_abi.h -- C ABI of the MI355X DSMC hot path (libcudadsmc.so).
+ *
+ * Replaces the two device-container classes the reference driver calls
+ * directly (cudaDSMC src/particle/Particle.cuh:7-38 `GPUParticles`,
+ * src/cell/Cell.cuh:7-35 `GPUCells`) and their kernels.  Plain pointers and
+ * sizes only.  Every call returns 0 on success or a negative status; the
+ * message of the last failure on the calling thread is dsmc_last_error().
+ * A handle owns its device memory and one HIP stream; calls on one handle
+ * are not thread-safe (one host thread per handle, as in the reference).
+ * Host pointers are borrowed for the duration of the call only.
+ */
+#ifndef DSMC_ABI_H
+#define DSMC_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSMC_OK             0
+#define DSMC_ERR_ARG       -1   /* invalid argument or state */
+#define DSMC_ERR_HIP       -2   /* HIP runtime failure */
+#define DSMC_ERR_CAPACITY  -3   /* particle capacity exceeded */
+#define DSMC_ERR_GEOMETRY  -4   /* invalid body polygon / cut-cell table */
+#define DSMC_ERR_NODEVICE  -5   /* no HIP device */
+
+/* boundary models (face_bc[], wall_bc) */
+#define DSMC_BC_STREAM    0     /* freestream: particles leave, Maxwellian flux enters */
+#define DSMC_BC_SPECULAR  1
+#define DSMC_BC_DIFFUSE   2
+
+/* device error word bits (dsmc_status.err) */
+#define DSMC_DEV_ERR_BOUNCE   1  /* a particle needed more than 64 wall events in one step */
+#define DSMC_DEV_ERR_INFLOW   2  /* inflow speed sampler hit its try limit */
+#define DSMC_DEV_ERR_CAPACITY 4  /* injection clamped at capacity */
+#define DSMC_DEV_ERR_FILL     8  /* fill could not place a particle in a cut cell's fluid part */
+
+/* Case parameters.  Field names follow the reference's src/Param.h:5-37
+ * (boltz, mass, diam, Vtl, L1, L2, L3, N1, N2, T, T_wall, V_jet, tau, Fn);
+ * the rest are the runtime knobs the BASELINE configs sweep. */
+typedef struct dsmc_params {
+    double boltz;        /* J/K */
+    double mass;         /* kg */
+    double diam;         /* VHS reference diameter d_ref, m */
+    double Vtl;          /* VHS viscosity index omega */
+    double Tref;         /* VHS reference temperature, K */
+    double L1, L2, L3;   /* domain x, y extent and unit depth, m */
+    int    N1, N2;       /* cells in x and y */
+    double n_inf;        /* freestream / initial number density, 1/m^3 */
+    double T;            /* freestream / initial temperature, K */
+    double V_jet;        /* freestream x velocity, m/s */
+    double V_y;          /* freestream y velocity, m/s */
+    double T_wall;       /* body wall temperature, K */
+    int    wall_bc;      /* body wall model: DSMC_BC_DIFFUSE or DSMC_BC_SPECULAR */
+    int    face_bc[4];   /* x=0, x=L1, y=0, y=L2 */
+    double face_T[4];    /* wall temperature of diffuse faces */
+    double tau;          /* time step, s */
+    double ppc;          /* simulated particles per full fluid cell at n_inf (sets Fn) */
+    double Fn;           /* real molecules per simulated particle; > 0 overrides ppc */
+    uint64_t seed;       /* Philox key of this instance */
+    int64_t capacity;    /* particle capacity; 0 = 4x the initial fill estimate */
+    int    sample_start; /* first step whose state is sampled; < 0 = never */
+} dsmc_params;
+
+/* Immersed body: closed polygon, counter-clockwise, n_points >= 3
+ * (reference src/object/Geom.h `m_points`, Circle.cpp:10-38). */
+typedef struct dsmc_body {
+    int n_points;
+    const double *xy;    /* 2*n_points */
+} dsmc_body;
+
+/* Derived constants of a created case (what the kernels use). */
+typedef struct dsmc_case_info {
+    int    ncell, nseg, ncut;
+    int    cseg_len;             /* length of the per-cell segment index list */
+    double Fn, dt;
+    float  L1, L2, dx, dy;
+    float  ntc_coef, K1, e_exp;  /* NTC: Fn*dt/2; VHS sigma*c_r = K1*(c_r^2)^e_exp */
+    float  sigcmax0;             /* initial (sigma*c_r)max, reference Vmax*pi*d^2 */
+    float  body_vmp, body_sig;
+    float  face_vmp[4], face_sig[4];
+    double inj_e[4];             /* expected inflow particles per step per face */
+    float  inj_a[4];             /* inward normal speed ratio per face */
+    float  inj_vmp, inj_sig, inj_ux, inj_uy;
+    float  margin;
+    int    body_bc, face_bc[4];
+    uint32_t seed;
+    double cell_volume;          /* full cell volume dx*dy*L3 */
+    double ppc_eff;              /* particles per full fluid cell at n_inf */
+    int64_t capacity;            /* particle capacity */
+} dsmc_case_info;
+
+typedef struct dsmc_status {
+    uint32_t step;               /* steps completed */
+    uint32_t n_live;             /* particles in the current state */
+    uint32_t next_pid;           /* next particle id to assign */
+    uint32_t err;                /* DSMC_DEV_ERR_* bits */
+    uint64_t particle_steps;     /* sum over completed steps of particles moved */
+    uint64_t collisions;         /* collisions over all completed steps */
+    uint64_t nsamples;           /* sampled steps */
+    uint32_t last_collisions;    /* collisions in the last completed step */
+    uint32_t n_move;             /* particles processed by the last move (incl. inflow) */
+} dsmc_status;
+
+typedef struct dsmc_handle_s *dsmc_handle;
+
+const char *dsmc_last_error(void);
+int dsmc_device_count(int *n);
+
+/* Lifecycle.  Replaces GPUParticles(N) / GPUCells(C) construction and
+ * GPUCells::UploadFromHost's cut-cell tables (Particle.cuh:11, Cell.cuh:10-12;
+ * Run.cpp:52-56,102-105,197-223).  body may be NULL (no immersed body). */
+int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out);
+int dsmc_destroy(dsmc_handle h);
+int dsmc_case_info_get(dsmc_handle h, dsmc_case_info *out);
+/* Host-built cut-cell tables: seg[6*nseg] (ax ay bx by nx ny, n into the
+ * fluid), cseg_start[ncell+1], cseg_idx[cseg_len], vol[ncell] fluid volume,
+ * vol_ntc[ncell] NTC volume, cut[ncell] partial-cell flag.  NULL skips. */
+int dsmc_case_tables(dsmc_handle h, float *seg, int *cseg_start, int *cseg_idx,
+                     double *vol, float *vol_ntc, uint8_t *cut);
+/* Host only (no device needed): the same derived constants and tables for a
+ * parameter set, plus frac[ncell] fluid fraction.  Call with NULL tables to
+ * get the sizes in info first. */
+int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_info *info,
+                       float *seg, int *cseg_start, int *cseg_idx, double *vol,
+                       float *vol_ntc, uint8_t *cut, double *frac);
+
+/* Initial state.  Replaces Run::assignParticle + TransferParticlesFromHostToDevice
+ * (Run.cpp:108-191).  init_freestream fills every fluid cell at n_inf, T,
+ * (V_jet, V_y); init_fields fills from per-cell n, ux, uy, T (warm start).
+ * Both reset the step counter, the particle ids and the cell state. */
+int dsmc_init_freestream(dsmc_handle h);
+int dsmc_init_fields(dsmc_handle h, const double *n, const double *ux,
+                     const double *uy, const double *T);
+
+/* Particle state I/O (GPUParticles::UploadFromHost, GPUParticle.cu:41-58).
+ * upload replaces the state with n particles; pid may be NULL (0..n-1). */
+int dsmc_upload_particles(dsmc_handle h, int64_t n, const float *x, const float *y,
+                          const float *vx, const float *vy, const float *vz,
+                          const uint32_t *pid);
+int dsmc_download_particles(dsmc_handle h, int64_t cap, float *x, float *y, float *vx,
+                            float *vy, float *vz, uint32_t *pid, int64_t *n_out);
+
+/* The three phases of Run::solver (Run.cpp:159-162,245-256):
+ *  dsmc_move     = Run::particlemove (GPUParticles::Move + DeleteInvalid) plus
+ *                  the inflow of GPUParticles::Injet and the atomic per-cell
+ *                  count of GPUCells::CalparticleNum -- one fused kernel;
+ *  dsmc_reassign = GPUCells::CalparticleStartIndex + GPUParticles::Sort
+ *                  (exclusive scan, scatter into the cell-sorted permutation);
+ *  dsmc_collide  = GPUCells::Collision (+ GPUCells::Sample when the step is
+ *                  >= sample_start): stable order within each cell, state
+ *                  reordered and compacted into cell order, NTC/VHS collisions.
+ * A step is move, reassign, collide, in that order. */
+int dsmc_move(dsmc_handle h);
+int dsmc_reassign(dsmc_handle h);
+int dsmc_collide(dsmc_handle h);
+/* nsteps full steps, launched eagerly (graph = 0) or by replaying a captured
+ * hipGraph of the step (graph = 1).  Asynchronous; dsmc_sync waits. */
+int dsmc_run(dsmc_handle h, int nsteps, int graph);
+int dsmc_sync(dsmc_handle h);
+int dsmc_status_get(dsmc_handle h, dsmc_status *out);
+
+/* Intermediate buffers for parity checks.  cell/slot are valid after
+ * dsmc_move for the n_move processed particles (cell -1 = deleted);
+ * count after dsmc_move; start[ncell+1] after dsmc_reassign; perm after
+ * dsmc_collide when keep_perm is set (ascending particle index per cell). */
+int dsmc_set_keep_perm(dsmc_handle h, int on);
+/* The n_move particles processed by the last dsmc_move (moved + inflow, in
+ * processing order, deleted ones included); valid until dsmc_collide. */
+int dsmc_download_moved(dsmc_handle h, int64_t cap, float *x, float *y, float *vx, float *vy,
+                        float *vz, uint32_t *pid, int64_t *n_out);
+int dsmc_download_bins(dsmc_handle h, int64_t cap, int *cell, int *count);
+int dsmc_download_start(dsmc_handle h, int *start);
+int dsmc_download_perm(dsmc_handle h, int64_t cap, int *perm);
+int dsmc_cell_state(dsmc_handle h, float *sigcmax, float *mrem, int *ncoll);
+int dsmc_set_cell_state(dsmc_handle h, const float *sigcmax, const float *mrem);
+
+/* Sampling (GPUCells::Sample + DownloadToHost, GPUCell.cu:83-91,147-167).
+ * acc[6*ncell]: six SoA blocks sum n, sum vx, sum vy, sum vz, sum v^2,
+ * collisions.  Fields: number density n, velocity, temperature, pressure. */
+int dsmc_reset_samples(dsmc_handle h);
+int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples);
+int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,
+                         double *uz, double *T, double *p);
+
+/* Per-kernel timing with HIP events on the handle's stream (eager steps
+ * only).  ms[k] = mean duration of kernel k over timed steps, k = 0 move,
+ * 1 scan, 2 scatter, 3 cell; bytes[k] = algorithmic bytes of the last launch. */
+int dsmc_set_timing(dsmc_handle h, int on);
+int dsmc_kernel_times(dsmc_handle h, double ms[4], double bytes[4], int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

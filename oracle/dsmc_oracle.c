@@ -1,0 +1,695 @@
+/*
+ * dsmc_oracle.c -- CPU restatement of the reference DSMC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see dsmc_oracle.h): the checker for the HIP
+ * kernels and the timed CPU baseline.  Never linked into the product.
+ *
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fno-math-errno -fopenmp).
+ * FP contraction must stay off: the GPU side rounds every product and sum
+ * separately and bit-exact comparison depends on it.
+ */
+#include "dsmc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as   */
+/* easy as 1, 2, 3", SC'11).  Replaces the reference's clock-seeded cuRAND   */
+/* XORWOW streams (GPUParticle.cu:242,389; GPUCell.cu:219 -- defect D10).    */
+/* ------------------------------------------------------------------------- */
+void or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* uniform in [0,1) and (0,1] with 24 random bits (exact in FP32) */
+static inline float u01(uint32_t w)  { return (float)(w >> 8) * 0x1p-24f; }
+static inline float u01o(uint32_t w) { return (float)((w >> 8) + 1u) * 0x1p-24f; }
+/* floor(w * n / 2^32): unbiased-enough integer in [0,n) without FP */
+static inline int mulhi_idx(uint32_t w, int n) { return (int)(((uint64_t)w * (uint64_t)(uint32_t)n) >> 32); }
+
+/* ------------------------------------------------------------------------- */
+/* Elementary functions from IEEE basic operations only.                      */
+/* ------------------------------------------------------------------------- */
+#define OR_LN2_HI  0x1.62e400p-1f      /* 0.693145751953125, 16 significant bits */
+#define OR_LN2_LO  0x1.7f7d1cp-20f     /* ln2 - OR_LN2_HI */
+#define OR_LOG2E   0x1.715476p+0f
+#define OR_PIO2    0x1.921fb6p+0f
+
+/* natural log for finite normal x > 0: x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+ * log m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716, odd series to s^9. */
+float or_logf(float x)
+{
+    uint32_t ix;
+    memcpy(&ix, &x, 4);
+    int e = (int)((ix >> 23) & 0xffu) - 127;
+    uint32_t mb = (ix & 0x007fffffu) | 0x3f800000u;
+    float m;
+    memcpy(&m, &mb, 4);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e = e + 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    float p = s2 * (0x1.555556p-2f + s2 * (0x1.99999ap-3f + s2 * (0x1.24924ap-3f + s2 * 0x1.c71c72p-4f)));
+    float two_s = 2.0f * s;
+    float logm = two_s + two_s * p;
+    float ef = (float)e;
+    return ef * OR_LN2_HI + (ef * OR_LN2_LO + logm);
+}
+
+/* exp for x in [-87, 88]: x = k ln2 + r (Cody-Waite), Taylor to r^7. */
+float or_expf(float x)
+{
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) x = 88.0f;
+    float kf = floorf(x * OR_LOG2E + 0.5f);
+    float r = (x - kf * OR_LN2_HI) - kf * OR_LN2_LO;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f +
+              r * (0x1.111112p-7f + r * (0x1.6c16c2p-10f + r * 0x1.a01a02p-13f))))));
+    int k = (int)kf;
+    uint32_t sb;
+    float scale;
+    if (k > 127) {
+        sb = (uint32_t)(127 + 127) << 23; memcpy(&scale, &sb, 4);
+        return p * scale * 2.0f;
+    }
+    if (k < -126) {
+        sb = (uint32_t)(k + 126 + 127) << 23; memcpy(&scale, &sb, 4);
+        return p * scale * 0x1p-126f;
+    }
+    sb = (uint32_t)(k + 127) << 23;
+    memcpy(&scale, &sb, 4);
+    return p * scale;
+}
+
+/* sin and cos of 2*pi*u for u in [0,1): quadrant reduction is exact
+ * (4u and its nearest integer are exact), |theta| <= pi/4, Taylor series. */
+void or_sincos2pi(float u, float *sn, float *cs)
+{
+    float t = u * 4.0f;
+    float q = floorf(t + 0.5f);
+    float f = t - q;
+    float th = f * OR_PIO2;
+    float t2 = th * th;
+    float sp = th + th * t2 * (-0x1.555556p-3f + t2 * (0x1.111112p-7f + t2 * (-0x1.a01a02p-13f + t2 * 0x1.71de3ap-19f)));
+    float cp = 1.0f + t2 * (-0.5f + t2 * (0x1.555556p-5f + t2 * (-0x1.6c16c2p-10f + t2 * (0x1.a01a02p-16f + t2 * -0x1.27e4fcp-22f))));
+    int qi = ((int)q) & 3;
+    if (qi == 0)      { *sn = sp;  *cs = cp; }
+    else if (qi == 1) { *sn = cp;  *cs = -sp; }
+    else if (qi == 2) { *sn = -sp; *cs = -cp; }
+    else              { *sn = -cp; *cs = sp; }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Binning index.  Reference: GPUCell.cu:184-189 and CartesianMesh.cpp:171-181 */
+/* (i = (int)(x/dx), cell = k + j*nz + i*ny*nz with nz = 1).  Deviation D14:  */
+/* out-of-range indices are clamped instead of leaving stale ids.            */
+/* ------------------------------------------------------------------------- */
+static inline int cell_of(const or_case *cs, float x, float y)
+{
+    int i = (int)(x / cs->dx);
+    int j = (int)(y / cs->dy);
+    if (i < 0) i = 0;
+    if (i > cs->nx - 1) i = cs->nx - 1;
+    if (j < 0) j = 0;
+    if (j > cs->ny - 1) j = cs->ny - 1;
+    return j + i * cs->ny;
+}
+
+/* Diffuse re-emission from a wall with unit normal n (into the fluid).
+ * Reference: GPUSegment.cuh:59-84 and WallBoundary.cpp:317-327.  D7 fix:
+ * independent draws -- normal speed from the flux-weighted (Rayleigh) law
+ * v_n = sqrt(-ln U) sqrt(2kTw/m), tangential and z components N(0, kTw/m). */
+static void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
+                         uint32_t seed, uint32_t step, uint32_t b,
+                         float *vx, float *vy, float *vz)
+{
+    uint32_t ctr[4] = { b, step, OR_SUB_MOVE, 0u }, w[4];
+    or_philox(ctr, pid, seed, w);
+    float vn = sqrtf(-or_logf(u01o(w[0]))) * vmp;
+    float r = sqrtf(-2.0f * or_logf(u01o(w[1]))) * sig;
+    float s, c;
+    or_sincos2pi(u01(w[2]), &s, &c);
+    float vt = r * c;
+    *vx = vn * nx + vt * (-ny);
+    *vy = vn * ny + vt * nx;
+    *vz = r * s;
+}
+
+/* One particle's flight of duration t.  Reference: moveParticles
+ * (GPUParticle.cu:244-350): x += v dt in 2D, cut-cell wall reflection, and
+ * deletion outside [0,L1]x[0,L2].  Deviations: D6 (the hit point is found on
+ * the actual path, not extrapolated forward), D8 (finite ray-segment test
+ * against every body segment the path can reach, earliest hit first, repeated
+ * for further hits), D17 (per-face boundary types: stream faces delete,
+ * specular faces mirror as WallBoundary::apply in Boundary.cuh:40-62, diffuse
+ * faces re-emit). */
+static void move_one(const or_case *cs, float *px, float *py, float *pvx, float *pvy,
+                     float *pvz, uint32_t pid, uint32_t step, float t,
+                     uint8_t *dead, uint32_t *err)
+{
+    float x = *px, y = *py, vx = *pvx, vy = *pvy, vz = *pvz;
+    uint8_t dd = 0;
+    for (int b = 0;; ++b) {
+        if (b == OR_MAX_BOUNCE) { *err |= OR_ERR_BOUNCE; break; }
+        float best = t;
+        int kind = -1;
+        if (vx < 0.0f) {
+            float s = (0.0f - x) / vx;
+            if (s < best) { best = s; kind = 0; }
+        } else if (vx > 0.0f) {
+            float s = (cs->L1 - x) / vx;
+            if (s < best) { best = s; kind = 1; }
+        }
+        if (vy < 0.0f) {
+            float s = (0.0f - y) / vy;
+            if (s < best) { best = s; kind = 2; }
+        } else if (vy > 0.0f) {
+            float s = (cs->L2 - y) / vy;
+            if (s < best) { best = s; kind = 3; }
+        }
+        if (cs->nseg > 0) {
+            float reach = sqrtf(vx * vx + vy * vy) * t;
+            int all = reach > cs->margin;
+            int cnt, c0 = 0;
+            if (all) {
+                cnt = cs->nseg;
+            } else {
+                int c = cell_of(cs, x, y);
+                c0 = cs->cseg_start[c];
+                cnt = cs->cseg_start[c + 1] - c0;
+            }
+            for (int q = 0; q < cnt; ++q) {
+                int k = all ? q : cs->cseg_idx[c0 + q];
+                const float *S = cs->seg + 6 * k;
+                float vn = vx * S[4] + vy * S[5];
+                if (!(vn < 0.0f)) continue;
+                float ex = S[2] - S[0], ey = S[3] - S[1];
+                float den = vx * ey - vy * ex;
+                if (den == 0.0f) continue;
+                float wx = S[0] - x, wy = S[1] - y;
+                float s = (wx * ey - wy * ex) / den;
+                if (!(s > 0.0f && s < best)) continue;
+                float u = (wx * vy - wy * vx) / den;
+                if (u < 0.0f || u > 1.0f) continue;
+                best = s;
+                kind = 4 + k;
+            }
+        }
+        if (kind < 0) {
+            x = x + vx * t;
+            y = y + vy * t;
+            break;
+        }
+        x = x + vx * best;
+        y = y + vy * best;
+        t = t - best;
+        if (kind < 4) {
+            if (kind == 0) x = 0.0f;
+            else if (kind == 1) x = cs->L1;
+            else if (kind == 2) y = 0.0f;
+            else y = cs->L2;
+            int bc = cs->face_bc[kind];
+            if (bc == OR_BC_STREAM) { dd = 1; break; }
+            if (bc == OR_BC_SPECULAR) {
+                if (kind < 2) vx = -vx; else vy = -vy;
+            } else {
+                float nx = kind == 0 ? 1.0f : (kind == 1 ? -1.0f : 0.0f);
+                float ny = kind == 2 ? 1.0f : (kind == 3 ? -1.0f : 0.0f);
+                diffuse_emit(nx, ny, cs->face_vmp[kind], cs->face_sig[kind], pid,
+                             cs->seed, step, (uint32_t)b, &vx, &vy, &vz);
+            }
+        } else {
+            const float *S = cs->seg + 6 * (kind - 4);
+            if (cs->body_bc == OR_BC_SPECULAR) {
+                /* Segment.cpp:152 specular law v' = v - 2 (v.n) n */
+                float vn = vx * S[4] + vy * S[5];
+                vx = vx - 2.0f * vn * S[4];
+                vy = vy - 2.0f * vn * S[5];
+            } else {
+                diffuse_emit(S[4], S[5], cs->body_vmp, cs->body_sig, pid, cs->seed,
+                             step, (uint32_t)b, &vx, &vy, &vz);
+            }
+        }
+    }
+    *px = x; *py = y; *pvx = vx; *pvy = vy; *pvz = vz;
+    *dead = dd;
+}
+
+void or_move(const or_case *cs, int n, float *x, float *y, float *vx, float *vy,
+             float *vz, const uint32_t *pid, uint32_t step, uint8_t *dead,
+             uint32_t *err)
+{
+    uint32_t e = 0;
+    for (int i = 0; i < n; ++i)
+        move_one(cs, &x[i], &y[i], &vx[i], &vy[i], &vz[i], pid[i], step, cs->dt,
+                 &dead[i], &e);
+    *err |= e;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Inflow.  Reference: Injet/InjectParticles (GPUParticle.cu:149-171,375-422) */
+/* fills a slab of width V_jet*tau on x=0 with drifting Maxwellians (D18).    */
+/* Restated as the Maxwellian flux through each stream face (Bird 1994 eq.    */
+/* 4.22), number per step with exact remainder carry floor((s+1)e)-floor(s e)*/
+/* and positions advanced by a random fraction of dt along their path.       */
+/* ------------------------------------------------------------------------- */
+void or_inject_counts(const or_case *cs, uint32_t step, int J[4])
+{
+    for (int f = 0; f < 4; ++f) {
+        double e = cs->inj_e[f];
+        if (cs->face_bc[f] == OR_BC_STREAM && e > 0.0)
+            J[f] = (int)(floor(((double)step + 1.0) * e) - floor((double)step * e));
+        else
+            J[f] = 0;
+    }
+}
+
+static void inject_one(const or_case *cs, int f, uint32_t pid, uint32_t step,
+                       float *ox, float *oy, float *ovx, float *ovy, float *ovz,
+                       uint8_t *dead, uint32_t *err)
+{
+    uint32_t ctr[4] = { 0u, step, OR_SUB_INJECT, 0u }, w[4];
+    or_philox(ctr, pid, cs->seed, w);
+    float upos = u01(w[0]);
+    float ufrac = u01(w[1]);
+    float r = sqrtf(-2.0f * or_logf(u01o(w[2])));
+    float sn, cn;
+    or_sincos2pi(u01(w[3]), &sn, &cn);
+    float g0 = r * cn, g1 = r * sn;
+
+    /* normal speed ratio w = v_n/vmp with density ~ w exp(-(w-a)^2), w > 0:
+     * acceptance-rejection against its maximum at w* = (a + sqrt(a^2+2))/2 */
+    float a = cs->inj_a[f];
+    float lo, hi;
+    if (a >= -3.5f) {
+        lo = a - 3.5f;
+        if (lo < 0.0f) lo = 0.0f;
+        hi = (a > 0.0f ? a : 0.0f) + 3.5f;
+    } else {
+        lo = 0.0f;
+        hi = -4.0f / a;
+    }
+    float ws = 0.5f * (a + sqrtf(a * a + 2.0f));
+    float dws = ws - a;
+    float ref = dws * dws;
+    float wn = lo;
+    int ok = 0;
+    for (int tr = 0; tr < OR_INFLOW_TRIES && !ok; tr += 2) {
+        uint32_t c2[4] = { 1u + (uint32_t)(tr / 2), step, OR_SUB_INJECT, 0u };
+        or_philox(c2, pid, cs->seed, w);
+        for (int h = 0; h < 2; ++h) {
+            float cand = lo + (hi - lo) * u01(w[2 * h]);
+            float d = cand - a;
+            float acc = (cand / ws) * or_expf(ref - d * d);
+            wn = cand;
+            if (u01(w[2 * h + 1]) < acc) { ok = 1; break; }
+        }
+    }
+    if (!ok) *err |= OR_ERR_INFLOW;
+    float vn = wn * cs->inj_vmp;
+    float x, y, vx, vy;
+    float vz = cs->inj_sig * g1;
+    if (f == 0)      { x = 0.0f;          y = upos * cs->L2; vx = vn;  vy = cs->inj_uy + cs->inj_sig * g0; }
+    else if (f == 1) { x = cs->L1;        y = upos * cs->L2; vx = -vn; vy = cs->inj_uy + cs->inj_sig * g0; }
+    else if (f == 2) { x = upos * cs->L1; y = 0.0f;          vy = vn;  vx = cs->inj_ux + cs->inj_sig * g0; }
+    else             { x = upos * cs->L1; y = cs->L2;        vy = -vn; vx = cs->inj_ux + cs->inj_sig * g0; }
+    move_one(cs, &x, &y, &vx, &vy, &vz, pid, step, ufrac * cs->dt, dead, err);
+    *ox = x; *oy = y; *ovx = vx; *ovy = vy; *ovz = vz;
+}
+
+void or_inject(const or_case *cs, uint32_t step, uint32_t next_pid, int J_total,
+               float *x, float *y, float *vx, float *vy, float *vz,
+               uint32_t *pid, uint8_t *dead, uint32_t *err)
+{
+    int J[4];
+    or_inject_counts(cs, step, J);
+    uint32_t e = 0;
+    for (int q = 0; q < J_total; ++q) {
+        int f = 0, acc = J[0];
+        while (f < 3 && q >= acc) { ++f; acc += J[f]; }
+        uint32_t p = next_pid + (uint32_t)q;
+        pid[q] = p;
+        inject_one(cs, f, p, step, &x[q], &y[q], &vx[q], &vy[q], &vz[q], &dead[q], &e);
+    }
+    *err |= e;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Bin / scan / scatter.  Reference: CalparticleNumAndLocalID                */
+/* (GPUCell.cu:169-198), thrust::exclusive_scan (GPUCell.cu:104-120),        */
+/* sortParticles (GPUParticle.cu:365-373).  The serial loop makes the        */
+/* atomicAdd rank equal to index order, i.e. the scatter is stable.          */
+/* ------------------------------------------------------------------------- */
+void or_bin(const or_case *cs, int n, const float *x, const float *y,
+            const uint8_t *dead, int *cell, int *count, int *slot)
+{
+    for (int i = 0; i < n; ++i) {
+        if (dead && dead[i]) { cell[i] = -1; slot[i] = 0; continue; }
+        int c = cell_of(cs, x[i], y[i]);
+        cell[i] = c;
+        slot[i] = count[c]++;
+    }
+}
+
+void or_scan(const int *count, int ncell, int *start)
+{
+    int s = 0;
+    for (int c = 0; c < ncell; ++c) { start[c] = s; s += count[c]; }
+    start[ncell] = s;
+}
+
+void or_scatter(int n, const int *cell, const int *slot, const int *start, int *perm)
+{
+    for (int i = 0; i < n; ++i)
+        if (cell[i] >= 0) perm[start[cell[i]] + slot[i]] = i;
+}
+
+/* ------------------------------------------------------------------------- */
+/* NTC/VHS collisions of one cell.  Reference: collisionInCells              */
+/* (GPUCell.cu:200-333) with the host semantics of Cell::collision           */
+/* (Cell.cpp:335-409) and Particle::Collision (Particle.cpp:82-101).         */
+/*  - D2/D25: remainder carry r' = (expected + r) - M  (the GPU kernel has   */
+/*    none; the host version carries expected - M, which loses candidates);  */
+/*  - D3: (sigma c_r)max persists per cell;                                  */
+/*  - D4: distinct pair drawn without the off-by-one (i2 skips i1);          */
+/*  - D5: Gamma(5/2 - omega) from tgamma, folded into K1 by the host;        */
+/*  - candidate count uses the pair number np(np-1) (D24; np^2 over-counts   */
+/*    collisions by (1 + 1/<np>)), T_ref in the VHS cross-section.          */
+/* ------------------------------------------------------------------------- */
+static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
+                        const int *perm, float *vx, float *vy, float *vz,
+                        float *sigcmax, float *mrem)
+{
+    if (np < 2) return 0;
+    float vol = cs->vol_ntc[c];
+    if (!(vol > 0.0f)) return 0;
+    float smax = sigcmax[c];
+    float e = (float)((long long)np * (long long)(np - 1)) * cs->ntc_coef;
+    e = e * smax;
+    e = e / vol;
+    float tot = e + mrem[c];
+    int M = (int)tot;
+    mrem[c] = tot - (float)M;
+    int nc = 0;
+    for (int m = 0; m < M; ++m) {
+        uint32_t ctr[4] = { (uint32_t)m, step, OR_SUB_COLLIDE, 0u }, w[4];
+        or_philox(ctr, (uint32_t)c, cs->seed, w);
+        int i1 = mulhi_idx(w[0], np);
+        int i2 = mulhi_idx(w[1], np - 1);
+        if (i2 >= i1) i2 = i2 + 1;
+        int p1 = perm[s0 + i1], p2 = perm[s0 + i2];
+        float gx = vx[p1] - vx[p2], gy = vy[p1] - vy[p2], gz = vz[p1] - vz[p2];
+        float g2 = gx * gx + gy * gy + gz * gz;
+        float sc = g2 > 0.0f ? cs->K1 * or_expf(cs->e_exp * or_logf(g2)) : 0.0f;
+        if (sc > smax) smax = sc;
+        if (u01(w[2]) < sc / smax) {
+            float cosr = 2.0f * u01(w[3]) - 1.0f;
+            float sinr = sqrtf(1.0f - cosr * cosr);
+            uint32_t c2[4] = { (uint32_t)m, step, OR_SUB_COLLIDE, 1u }, w2[4];
+            or_philox(c2, (uint32_t)c, cs->seed, w2);
+            float sp, cp;
+            or_sincos2pi(u01(w2[0]), &sp, &cp);
+            float g = sqrtf(g2);
+            float rx = g * cosr, ry = g * (sinr * cp), rz = g * (sinr * sp);
+            float mx = 0.5f * (vx[p1] + vx[p2]);
+            float my = 0.5f * (vy[p1] + vy[p2]);
+            float mz = 0.5f * (vz[p1] + vz[p2]);
+            vx[p1] = mx + 0.5f * rx; vx[p2] = mx - 0.5f * rx;
+            vy[p1] = my + 0.5f * ry; vy[p2] = my - 0.5f * ry;
+            vz[p1] = mz + 0.5f * rz; vz[p2] = mz - 0.5f * rz;
+            ++nc;
+        }
+    }
+    sigcmax[c] = smax;
+    return nc;
+}
+
+void or_collide(const or_case *cs, uint32_t step, const int *start, const int *perm,
+                float *vx, float *vy, float *vz, float *sigcmax, float *mrem,
+                int *ncoll_cell, int nthreads)
+{
+    int C = cs->ncell;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 1 ? nthreads : 1) if (nthreads > 1)
+#endif
+    for (int c = 0; c < C; ++c)
+        ncoll_cell[c] = collide_cell(cs, c, step, start[c], start[c + 1] - start[c], perm,
+                                     vx, vy, vz, sigcmax, mrem);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sampling.  Reference: samplingInCells (GPUCell.cu:336-402) and            */
+/* Cell::sample (Cell.cpp:410-447), restated as time-averaged FP64           */
+/* accumulators (D15): counts, sum v, sum v^2, collisions per cell.          */
+/* ------------------------------------------------------------------------- */
+void or_sample(int ncell, const int *start, const int *perm, const float *vx,
+               const float *vy, const float *vz, const int *ncoll_cell, double *acc)
+{
+    for (int c = 0; c < ncell; ++c) {
+        int s0 = start[c], np = start[c + 1] - s0;
+        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+        for (int k = 0; k < np; ++k) {
+            int p = perm[s0 + k];
+            double a = (double)vx[p], b = (double)vy[p], d = (double)vz[p];
+            sx += a; sy += b; sz += d;
+            s2 += a * a + b * b + d * d;
+        }
+        acc[0 * ncell + c] += (double)np;
+        acc[1 * ncell + c] += sx;
+        acc[2 * ncell + c] += sy;
+        acc[3 * ncell + c] += sz;
+        acc[4 * ncell + c] += s2;
+        acc[5 * ncell + c] += (double)(ncoll_cell ? ncoll_cell[c] : 0);
+    }
+}
+
+void or_reorder(int n, const int *perm,
+                const float *x, const float *y, const float *vx, const float *vy,
+                const float *vz, const uint32_t *pid,
+                float *ox, float *oy, float *ovx, float *ovy, float *ovz, uint32_t *opid)
+{
+    for (int k = 0; k < n; ++k) {
+        int p = perm[k];
+        ox[k] = x[p]; oy[k] = y[p]; ovx[k] = vx[p]; ovy[k] = vy[p]; ovz[k] = vz[p];
+        opid[k] = pid[p];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Initial fill.  Reference: Run::assignParticle (Run.cpp:108-157) with the  */
+/* Maxwellian of Random::MaxwellDistribution (Random.cpp:30-46).  Per-cell   */
+/* count floor(expected + U); positions uniform in the cell, rejected inside */
+/* the body for cut cells (D16); velocities drift + N(0, kT/m) by            */
+/* Box-Muller; pid = slot.                                                   */
+/* ------------------------------------------------------------------------- */
+static int inside_body(const or_case *cs, float px, float py)
+{
+    int in = 0;
+    for (int k = 0; k < cs->nseg; ++k) {
+        const float *S = cs->seg + 6 * k;
+        float ax = S[0], ay = S[1], bx = S[2], by = S[3];
+        if ((ay > py) != (by > py)) {
+            float xi = ax + ((py - ay) * (bx - ax)) / (by - ay);
+            if (px < xi) in ^= 1;
+        }
+    }
+    return in;
+}
+
+int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
+                 const float *ux, const float *uy, const float *sig, int cap,
+                 float *x, float *y, float *vx, float *vy, float *vz, uint32_t *pid,
+                 int *count, uint32_t *err)
+{
+    int C = cs->ncell;
+    long long n = 0;
+    for (int c = 0; c < C; ++c) {
+        uint32_t ctr[4] = { 0xFFFFFFFFu, 0u, OR_SUB_INIT, 0u }, w[4];
+        or_philox(ctr, (uint32_t)c, cs->seed, w);
+        double U = (double)w[0] * 0x1p-32;
+        int k = expected[c] > 0.0 ? (int)floor(expected[c] + U) : 0;
+        count[c] = k;
+        n += k;
+    }
+    if (n > cap) { *err |= OR_ERR_CAPACITY; return -1; }
+    int i = 0;
+    uint32_t e = 0;
+    for (int c = 0; c < C; ++c) {
+        int ci = c / cs->ny, cj = c % cs->ny;
+        float x0 = (float)ci * cs->dx, y0 = (float)cj * cs->dy;
+        for (int k = 0; k < count[c]; ++k, ++i) {
+            uint32_t w[4];
+            float px = x0, py = y0;
+            int ok = 0;
+            for (int tr = 0; tr < OR_FILL_TRIES && !ok; tr += 2) {
+                uint32_t ctr[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 1u + (uint32_t)(tr / 2) };
+                or_philox(ctr, (uint32_t)c, cs->seed, w);
+                for (int h = 0; h < 2; ++h) {
+                    px = x0 + cs->dx * u01(w[2 * h]);
+                    py = y0 + cs->dy * u01(w[2 * h + 1]);
+                    if (!cut[c] || !inside_body(cs, px, py)) { ok = 1; break; }
+                }
+            }
+            if (!ok) e |= OR_ERR_FILL;
+            uint32_t cv[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 0u };
+            or_philox(cv, (uint32_t)c, cs->seed, w);
+            float r1 = sqrtf(-2.0f * or_logf(u01o(w[0])));
+            float s1, c1, s2, c2;
+            or_sincos2pi(u01(w[1]), &s1, &c1);
+            float r2 = sqrtf(-2.0f * or_logf(u01o(w[2])));
+            or_sincos2pi(u01(w[3]), &s2, &c2);
+            x[i] = px; y[i] = py;
+            vx[i] = ux[c] + sig[c] * (r1 * c1);
+            vy[i] = uy[c] + sig[c] * (r1 * s1);
+            vz[i] = sig[c] * (r2 * c2);
+            pid[i] = (uint32_t)i;
+        }
+    }
+    *err |= e;
+    return (int)n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Full step: move -> inject -> bin -> scan -> scatter -> collide ->         */
+/* (sample) -> reorder.  Reference phase order: Run::solver                  */
+/* (Run.cpp:259-299): particlemove, ressignParticle, collision, Sample.      */
+/* The parallel version gives bit-identical results to the serial one.       */
+/* ------------------------------------------------------------------------- */
+int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
+{
+    int nt = nthreads > 1 ? nthreads : 1;
+    int n = st->n;
+    uint32_t err = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(| : err) if (nt > 1)
+#endif
+    for (int i = 0; i < n; ++i) {
+        uint32_t e = 0;
+        move_one(cs, &st->x[i], &st->y[i], &st->vx[i], &st->vy[i], &st->vz[i],
+                 st->pid[i], st->step, cs->dt, &st->dead[i], &e);
+        err |= e;
+    }
+    int J[4];
+    or_inject_counts(cs, st->step, J);
+    int Jt = J[0] + J[1] + J[2] + J[3];
+    if (n + Jt > st->cap) { Jt = st->cap - n; err |= OR_ERR_CAPACITY; }
+    uint32_t einj = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt) reduction(| : einj) if (nt > 1)
+#endif
+    for (int q = 0; q < Jt; ++q) {
+        int f = 0, acc = J[0];
+        while (f < 3 && q >= acc) { ++f; acc += J[f]; }
+        uint32_t p = st->next_pid + (uint32_t)q;
+        uint32_t e = 0;
+        st->pid[n + q] = p;
+        inject_one(cs, f, p, st->step, &st->x[n + q], &st->y[n + q], &st->vx[n + q],
+                   &st->vy[n + q], &st->vz[n + q], &st->dead[n + q], &e);
+        einj |= e;
+    }
+    err |= einj;
+    int ntot = n + Jt;
+    int C = cs->ncell;
+
+    /* bin: cell ids in parallel, ranks by a stable two-pass counting sort */
+    if (nt == 1) {
+        memset(st->count, 0, sizeof(int) * (size_t)C);
+        or_bin(cs, ntot, st->x, st->y, st->dead, st->cell, st->count, st->slot);
+    } else {
+#ifdef _OPENMP
+        int *hist = (int *)calloc((size_t)nt * (size_t)C, sizeof(int));
+#pragma omp parallel num_threads(nt)
+        {
+            int tid = omp_get_thread_num();
+            int lo = (int)((long long)ntot * tid / nt), hi = (int)((long long)ntot * (tid + 1) / nt);
+            int *h = hist + (size_t)tid * C;
+            for (int i = lo; i < hi; ++i) {
+                if (st->dead[i]) { st->cell[i] = -1; st->slot[i] = 0; continue; }
+                int c = cell_of(cs, st->x[i], st->y[i]);
+                st->cell[i] = c;
+                st->slot[i] = h[c]++;
+            }
+#pragma omp barrier
+#pragma omp for schedule(static)
+            for (int c = 0; c < C; ++c) {
+                int s = 0;
+                for (int t = 0; t < nt; ++t) { int v = hist[(size_t)t * C + c]; hist[(size_t)t * C + c] = s; s += v; }
+                st->count[c] = s;
+            }
+#pragma omp barrier
+            for (int i = lo; i < hi; ++i)
+                if (st->cell[i] >= 0) st->slot[i] += h[st->cell[i]];
+        }
+        free(hist);
+#endif
+    }
+    or_scan(st->count, C, st->start);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+#endif
+    for (int i = 0; i < ntot; ++i)
+        if (st->cell[i] >= 0) st->perm[st->start[st->cell[i]] + st->slot[i]] = i;
+
+    or_collide(cs, st->step, st->start, st->perm, st->vx, st->vy, st->vz,
+               st->sigcmax, st->mrem, st->ncoll_cell, nt);
+    long long tc = 0;
+    for (int c = 0; c < C; ++c) tc += st->ncoll_cell[c];
+    st->total_coll += tc;
+    if (sample) {
+        or_sample(C, st->start, st->perm, st->vx, st->vy, st->vz, st->ncoll_cell, st->acc);
+        st->nsamples += 1;
+    }
+    int nn = st->start[C];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+#endif
+    for (int k = 0; k < nn; ++k) {
+        int p = st->perm[k];
+        st->bx[k] = st->x[p]; st->by[k] = st->y[p];
+        st->bvx[k] = st->vx[p]; st->bvy[k] = st->vy[p]; st->bvz[k] = st->vz[p];
+        st->bpid[k] = st->pid[p];
+    }
+    float *t;
+    uint32_t *tp;
+    t = st->x; st->x = st->bx; st->bx = t;
+    t = st->y; st->y = st->by; st->by = t;
+    t = st->vx; st->vx = st->bvx; st->bvx = t;
+    t = st->vy; st->vy = st->bvy; st->bvy = t;
+    t = st->vz; st->vz = st->bvz; st->bvz = t;
+    tp = st->pid; st->pid = st->bpid; st->bpid = tp;
+    st->n = nn;
+    st->next_pid += (uint32_t)Jt;
+    st->step += 1;
+    st->err |= err;
+    return nn;
+}
+
+/* ---- batch helpers for the tests ------------------------------------------ */
+void or_logf_n(int n, const float *in, float *out) { for (int i = 0; i < n; ++i) out[i] = or_logf(in[i]); }
+void or_expf_n(int n, const float *in, float *out) { for (int i = 0; i < n; ++i) out[i] = or_expf(in[i]); }
+void or_sincos2pi_n(int n, const float *in, float *s, float *c)
+{
+    for (int i = 0; i < n; ++i) or_sincos2pi(in[i], &s[i], &c[i]);
+}
+/* out[4*i..] = philox(ctr[4*i..], k0[i], k1[i]) */
+void or_philox_n(int n, const uint32_t *ctr, const uint32_t *k0, const uint32_t *k1, uint32_t *out)
+{
+    for (int i = 0; i < n; ++i) or_philox(ctr + 4 * i, k0[i], k1[i], out + 4 * i);
+}

@@ -1,0 +1,138 @@
+/*
+ * dsmc_oracle.h -- CPU restatement of the reference DSMC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * kernels of cudadsmc_amd are compared against, and the timed CPU baseline
+ * of bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links or calls it.
+ *
+ * Every routine cites the reference file:line whose behaviour it restates
+ * (paths relative to the cudaDSMC source tree).  Deliberate deviations from
+ * the reference are the defect fixes named D<n> in the comments; each one is
+ * listed in oracle/README.md with the reason.
+ *
+ * Arithmetic contract shared with the GPU path (this is what makes
+ * bit-exact comparison possible):
+ *   - particle state is FP32 SoA; all per-particle arithmetic is FP32 with
+ *     correctly rounded + - * / sqrt, no FMA contraction (-ffp-contract=off);
+ *   - log, exp and sin/cos(2*pi*u) are the routines in this file, built only
+ *     from IEEE basic operations (no libm), so both sides round identically;
+ *   - sampling sums are FP64.
+ */
+#ifndef DSMC_ORACLE_H
+#define DSMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary condition of a domain face or of the body wall */
+enum { OR_BC_STREAM = 0, OR_BC_SPECULAR = 1, OR_BC_DIFFUSE = 2 };
+
+/* Philox substreams: counter word 2 */
+enum { OR_SUB_MOVE = 1, OR_SUB_INJECT = 2, OR_SUB_COLLIDE = 3, OR_SUB_INIT = 4 };
+
+/* error bits */
+enum { OR_ERR_BOUNCE = 1, OR_ERR_INFLOW = 2, OR_ERR_CAPACITY = 4, OR_ERR_FILL = 8 };
+
+#define OR_MAX_BOUNCE 64
+#define OR_INFLOW_TRIES 256
+#define OR_FILL_TRIES 256
+
+/* One DSMC case: grid, gas, walls, inflow and the cut-cell tables.  The
+ * scalar fields are the derived constants the GPU path uses (see
+ * dsmc_case_info in include/dsmc_abi.h); the tables are the ones the host
+ * geometry layer builds. */
+typedef struct or_case {
+    int   nx, ny, ncell;
+    float L1, L2, dx, dy;
+    float dt;
+    float ntc_coef;      /* Fn*dt/2 */
+    float K1, e_exp;     /* sigma_T*c_r = K1 * (c_r^2)^e_exp  (VHS) */
+    int   body_bc;
+    float body_vmp, body_sig;          /* sqrt(2kTw/m), sqrt(kTw/m) */
+    int   face_bc[4];                  /* x=0, x=L1, y=0, y=L2 */
+    float face_vmp[4], face_sig[4];
+    double inj_e[4];                   /* expected injections per step per face */
+    float inj_a[4];                    /* inward normal speed ratio per face */
+    float inj_vmp, inj_sig, inj_ux, inj_uy;
+    float margin;                      /* reach covered by a cell's segment list */
+    int   nseg;
+    const float *seg;                  /* nseg*6: ax ay bx by nx ny (n into fluid) */
+    const int   *cseg_start;           /* ncell+1 */
+    const int   *cseg_idx;
+    const float *vol_ntc;              /* per-cell volume used by NTC */
+    uint32_t seed;
+} or_case;
+
+/* ---- primitives ---------------------------------------------------------- */
+void  or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]);
+float or_logf(float x);
+float or_expf(float x);
+void  or_sincos2pi(float u, float *s, float *c);
+
+/* ---- hot-path phases ----------------------------------------------------- */
+/* Free flight + walls + body + outflow for particles [0,n). */
+void or_move(const or_case *cs, int n, float *x, float *y, float *vx, float *vy,
+             float *vz, const uint32_t *pid, uint32_t step, uint8_t *dead,
+             uint32_t *err);
+/* Number of inflow particles created at `step` on each face. */
+void or_inject_counts(const or_case *cs, uint32_t step, int J[4]);
+/* Create this step's inflow particles at slots [0,J) of the output arrays. */
+void or_inject(const or_case *cs, uint32_t step, uint32_t next_pid, int J_total,
+               float *x, float *y, float *vx, float *vy, float *vz,
+               uint32_t *pid, uint8_t *dead, uint32_t *err);
+/* Binning: cell id and stable rank (serial atomicAdd order). */
+void or_bin(const or_case *cs, int n, const float *x, const float *y,
+            const uint8_t *dead, int *cell, int *count, int *slot);
+void or_scan(const int *count, int ncell, int *start /* ncell+1 */);
+void or_scatter(int n, const int *cell, const int *slot, const int *start, int *perm);
+/* NTC/VHS collisions of every cell, through perm; updates velocities in place. */
+void or_collide(const or_case *cs, uint32_t step, const int *start, const int *perm,
+                float *vx, float *vy, float *vz, float *sigcmax, float *mrem,
+                int *ncoll_cell, int nthreads);
+/* FP64 sampling accumulators: acc[6*ncell] as 6 SoA blocks n,vx,vy,vz,v2,coll. */
+void or_sample(int ncell, const int *start, const int *perm, const float *vx,
+               const float *vy, const float *vz, const int *ncoll_cell, double *acc);
+void or_reorder(int n, const int *perm,
+                const float *x, const float *y, const float *vx, const float *vy,
+                const float *vz, const uint32_t *pid,
+                float *ox, float *oy, float *ovx, float *ovy, float *ovz, uint32_t *opid);
+
+/* Initial fill (Run::assignParticle): expected[c] = mean particle count of
+ * cell c; cut[c] != 0 marks a partially solid cell (rejection sampling).
+ * Writes into arrays of capacity cap; returns particle count (or -1). */
+int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
+                 const float *ux, const float *uy, const float *sig, int cap,
+                 float *x, float *y, float *vx, float *vy, float *vz, uint32_t *pid,
+                 int *count, uint32_t *err);
+
+/* ---- batch helpers for the tests ------------------------------------------ */
+void or_logf_n(int n, const float *in, float *out);
+void or_expf_n(int n, const float *in, float *out);
+void or_sincos2pi_n(int n, const float *in, float *s, float *c);
+void or_philox_n(int n, const uint32_t *ctr, const uint32_t *k0, const uint32_t *k1, uint32_t *out);
+
+/* ---- full step ------------------------------------------------------------ */
+typedef struct or_state {
+    int n, cap;
+    uint32_t step, next_pid, err;
+    float *x, *y, *vx, *vy, *vz; uint32_t *pid;      /* current state */
+    float *bx, *by, *bvx, *bvy, *bvz; uint32_t *bpid; /* scratch */
+    uint8_t *dead; int *cell, *slot, *perm;
+    int *count, *start, *ncoll_cell;
+    float *sigcmax, *mrem;
+    double *acc; long long nsamples;
+    long long total_coll;
+} or_state;
+
+/* One timestep; sample != 0 accumulates sampling sums.  nthreads <= 1 runs
+ * serially.  Returns the number of particles after the step. */
+int or_step(const or_case *cs, or_state *st, int sample, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

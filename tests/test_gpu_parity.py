@@ -1,0 +1,203 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on
+identical inputs.  Integer results (cell ids, counts, scan offsets, the
+stable permutation) must be bit-exact; because both sides follow the same
+FP32 arithmetic contract, particle states, (sigma c_r)max and remainders are
+required bit-exact too.  Sampling sums (FP64, different summation order)
+must agree to 1e-12 relative."""
+import numpy as np
+import pytest
+
+from cudadsmc_amd import abi, cases, run
+from cudadsmc_amd.run import Run
+from oracle import oracle as O
+
+import dsmc_testlib as T
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "box16": lambda: T.small_box(16, 8, seed=7),
+    "cyl80": lambda: T.small_cylinder(80, 40, 10, 64, seed=11),
+    "cyl80_specular": lambda: T.small_cylinder(80, 40, 10, 64, seed=13, wall=abi.BC_SPECULAR),
+}
+
+
+def _make(name):
+    p, body = CASES[name]()
+    print(f"[case] {name} N1={p.N1} N2={p.N2} ppc={p.ppc} seed={p.seed}", flush=True)
+    oc, info, tables = O.case_from_params(p, body)
+    return p, body, oc, info, tables
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fill_matches_oracle(gpu_lib, name):
+    p, body, oc, info, tables = _make(name)
+    r = Run(p, body)
+    r.initialize()
+    g = r.download()
+    o, count, err = T.oracle_fill(oc, info, tables)
+    assert err == 0 and r.status().err == 0
+    T.assert_same_particles(g, o, what="fill ")
+    r.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_phase_parity(gpu_lib, name):
+    """move / reassign / collide each against the oracle, bit-exact."""
+    p, body, oc, info, tables = _make(name)
+    r = Run(p, body)
+    r.initialize()
+    r.set_keep_perm(True)
+    for step in range(3):
+        cur = r.download()
+        sm, mr, _ = r.cell_state()
+        st = r.status()
+        ref = T.oracle_phases(oc, cur, step, st.next_pid, sm, mr, int(info.capacity))
+        r.particlemove()
+        moved = r.download(moved=True)
+        T.assert_same_particles(moved, ref["moved"], what=f"step {step} move ")
+        cell, count = r.bins()
+        np.testing.assert_array_equal(cell, ref["cell"])
+        np.testing.assert_array_equal(count, ref["count"])
+        r.ressignParticle()
+        np.testing.assert_array_equal(r.start(), ref["start"])
+        r.collision()
+        np.testing.assert_array_equal(r.perm(), ref["perm"])
+        T.assert_same_particles(r.download(), ref["new"], what=f"step {step} collide ")
+        sm2, mr2, nc2 = r.cell_state()
+        np.testing.assert_array_equal(sm2, ref["sigcmax"])
+        np.testing.assert_array_equal(mr2, ref["mrem"])
+        np.testing.assert_array_equal(nc2, ref["ncoll"])
+        s2 = r.status()
+        assert s2.step == step + 1 and s2.err == 0
+        assert s2.last_collisions == int(ref["ncoll"].sum())
+    r.close()
+
+
+@pytest.mark.parametrize("name,steps", [("box16", 30), ("cyl80", 30)])
+def test_trajectory_bitexact(gpu_lib, name, steps):
+    """Many full steps on both sides from the same fill: identical states."""
+    p, body, oc, info, tables = _make(name)
+    p.sample_start = 5
+    r = Run(p, body)
+    r.initialize()
+    r.solver(steps, graph=False)
+    g = r.download()
+    acc_g, ns_g = r.samples()
+    parts, _, _ = T.oracle_fill(oc, info, tables)
+    st = O.OracleState(oc, parts, int(info.capacity), info.sigcmax0)
+    for s in range(steps):
+        st.step(sample=s >= 5, nthreads=4)
+    o = st.particles()
+    T.assert_same_particles(g, o, what=f"{name} after {steps} steps ")
+    s = r.status()
+    assert s.collisions == st.st.total_coll
+    assert s.err == 0 and st.st.err == 0
+    assert ns_g == steps - 5
+    acc_o = st.acc.reshape(6, -1)
+    np.testing.assert_allclose(acc_g, acc_o, rtol=1e-12, atol=1e-9)
+    r.close()
+
+
+def test_graph_equals_eager(gpu_lib):
+    p, body = T.small_cylinder(80, 40, 10, 64, seed=21)
+    p.sample_start = 0
+    a, b = Run(p, body), Run(p, body)
+    a.initialize()
+    b.initialize()
+    a.solver(37, graph=False)
+    b.solver(37, graph=True)
+    T.assert_same_particles(a.download(), b.download(), what="graph vs eager ")
+    np.testing.assert_array_equal(a.samples()[0], b.samples()[0])
+    a.close()
+    b.close()
+
+
+def test_heavy_cell_and_empty(gpu_lib):
+    """All particles in a single cell (beyond the LDS tile capacity) and an
+    empty state: the global-memory path and the n = 0 path."""
+    p, body = T.small_box(8, 1, seed=3)
+    p.n_inf = 1.35e16  # ~1000 NTC candidates for 6000 particles in one cell
+    oc, info, tables = O.case_from_params(p, body)
+    rng = np.random.default_rng(5)
+    n = 6000
+    dx = np.float32(info.dx)
+    parts = dict(x=(rng.random(n, dtype=np.float32) * dx * 0.99).astype(np.float32) + 3 * dx,
+                 y=(rng.random(n, dtype=np.float32) * dx * 0.99).astype(np.float32) + 5 * dx,
+                 vx=rng.normal(0, 1e-3, n).astype(np.float32), vy=rng.normal(0, 1e-3, n).astype(np.float32),
+                 vz=rng.normal(0, 300, n).astype(np.float32), pid=np.arange(n, dtype=np.uint32))
+    r = Run(p, body)
+    r.upload(*[parts[k] for k in ("x", "y", "vx", "vy", "vz")], parts["pid"])
+    r.set_keep_perm(True)
+    sm, mr, _ = r.cell_state()
+    ref = T.oracle_phases(oc, parts, 0, n, sm, mr, int(info.capacity))
+    r.particlemove()
+    r.ressignParticle()
+    r.collision()
+    np.testing.assert_array_equal(r.perm(), ref["perm"])
+    T.assert_same_particles(r.download(), ref["new"], what="heavy cell ")
+    np.testing.assert_array_equal(r.cell_state()[0], ref["sigcmax"])
+    assert ref["ncoll"].max() > 100
+    # empty
+    r.upload(*[np.zeros(0, np.float32)] * 5)
+    r.solver(3, graph=False)
+    s = r.status()
+    assert s.n_live == 0 and s.step == 3 and s.err == 0
+    r.close()
+
+
+def test_box_full_rate_and_conservation(gpu_lib):
+    """Closed box at equilibrium, full size (64x64, 81,920 particles, 1000
+    steps): collision rate against VHS theory, energy and particle
+    conservation, T = 300 K."""
+    cfg = cases.box_equilibrium()
+    p = cfg["params"]
+    r = Run(p)
+    r.initialize()
+    g0 = r.download()
+    e0 = float((g0["vx"].astype(np.float64) ** 2 + g0["vy"] ** 2 + g0["vz"] ** 2).sum())
+    r.solver(cfg["steps"], graph=True)
+    s = r.status()
+    assert s.err == 0 and s.n_live == 81920 and s.step == 1000
+    rate = s.collisions / cfg["steps"]
+    theory = 0.5 * 81920 * cfg["nu"] * r.info.dt
+    # NTC collision counts per step are sub-Poisson; use the Poisson sigma of
+    # the 1000-step mean as a conservative 3-sigma band.
+    sigma = np.sqrt(theory) / np.sqrt(cfg["steps"])
+    assert abs(rate - theory) < 3 * sigma + 0.002 * theory, (rate, theory)
+    g = r.download()
+    e1 = float((g["vx"].astype(np.float64) ** 2 + g["vy"] ** 2 + g["vz"] ** 2).sum())
+    assert abs(e1 - e0) / e0 < 1e-4
+    assert np.unique(g["pid"]).size == 81920
+    f = r.fields()
+    T_mean = float((f["T"] * f["n"]).sum() / f["n"].sum())
+    assert abs(T_mean - 300.0) < 1.5, T_mean
+    r.close()
+
+
+def test_cylinder_full_invariants(gpu_lib):
+    """Mach-10 cylinder at full size for 300 steps: no particle inside the
+    body, cell-sorted state, unique ids, inflow count as specified, no
+    device error."""
+    cfg = cases.cylinder_mach10()
+    p = cfg["params"]
+    r = Run(p, cfg["body"])
+    r.initialize()
+    n0 = r.status().n_live
+    r.solver(300, graph=True)
+    s = r.status()
+    assert s.err == 0
+    g = r.download()
+    assert g["x"].shape[0] == s.n_live
+    assert np.unique(g["pid"]).size == s.n_live
+    inj = sum(r.info.inj_e[f] for f in range(4)) * 300
+    assert abs((s.next_pid - n0) - inj) <= 4
+    cx, cy, R = 0.3, 0.15, 0.1
+    rr = np.hypot(g["x"].astype(np.float64) - cx, g["y"].astype(np.float64) - cy)
+    # polygon of 256 points is inscribed: its apothem is R cos(pi/256)
+    assert (rr < R * np.cos(np.pi / 256) * (1 - 1e-5)).sum() == 0
+    i = np.minimum((g["x"] / np.float32(r.info.dx)).astype(np.int64), p.N1 - 1)
+    j = np.minimum((g["y"] / np.float32(r.info.dy)).astype(np.int64), p.N2 - 1)
+    cell = j + i * p.N2
+    assert (np.diff(cell) >= 0).all()
+    r.close()

@@ -1,0 +1,44 @@
+"""CPU: the oracle's phase functions (used by the GPU parity tests) compose
+to exactly its full step, and the OpenMP step equals the serial one."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+import dsmc_testlib as T
+
+
+@pytest.mark.parametrize("case", ["box", "cyl"])
+def test_phases_compose_to_step(case):
+    p, body = T.small_box(16, 8, seed=7) if case == "box" else T.small_cylinder(80, 40, 10, 64, seed=11)
+    oc, info, tables = O.case_from_params(p, body)
+    parts, _, err = T.oracle_fill(oc, info, tables)
+    assert err == 0
+    st = O.OracleState(oc, parts, int(info.capacity), info.sigcmax0)
+    cur = parts
+    sm = np.full(info.ncell, np.float32(info.sigcmax0), np.float32)
+    mr = np.zeros(info.ncell, np.float32)
+    next_pid = parts["pid"].size
+    for step in range(4):
+        ref = T.oracle_phases(oc, cur, step, next_pid, sm, mr, int(info.capacity))
+        st.step(sample=True)
+        T.assert_same_particles(st.particles(), ref["new"], what=f"step {step} ")
+        np.testing.assert_array_equal(st.sigcmax, ref["sigcmax"])
+        np.testing.assert_array_equal(st.mrem, ref["mrem"])
+        cur, sm, mr = ref["new"], ref["sigcmax"], ref["mrem"]
+        next_pid += ref["Jt"]
+    assert st.st.err == 0
+
+
+def test_openmp_step_is_bit_identical():
+    p, body = T.small_cylinder(80, 40, 10, 64, seed=17)
+    oc, info, tables = O.case_from_params(p, body)
+    parts, _, _ = T.oracle_fill(oc, info, tables)
+    a = O.OracleState(oc, parts, int(info.capacity), info.sigcmax0)
+    b = O.OracleState(oc, parts, int(info.capacity), info.sigcmax0)
+    for _ in range(10):
+        a.step(sample=True, nthreads=1)
+        b.step(sample=True, nthreads=4)
+    T.assert_same_particles(a.particles(), b.particles())
+    np.testing.assert_array_equal(a.acc, b.acc)
+    assert a.st.total_coll == b.st.total_coll
