@@ -76,9 +76,9 @@ SIGNATURES = {
     "dsmc_create": [_P(dsmc_params), _P(dsmc_body), _P(C.c_void_p)],
     "dsmc_destroy": [_h],
     "dsmc_case_info_get": [_h, _P(dsmc_case_info)],
-    "dsmc_case_tables": [_h, _f32p, _i32p, _i32p, _f64p, _f32p, _u8p],
+    "dsmc_case_tables": [_h, _f32p, _i32p, _i32p, _f64p, _f32p, _u8p, _f32p],
     "dsmc_case_describe": [_P(dsmc_params), _P(dsmc_body), _P(dsmc_case_info), _f32p, _i32p, _i32p,
-                           _f64p, _f32p, _u8p, _f64p],
+                           _f64p, _f32p, _u8p, _f64p, _f32p],
     "dsmc_init_freestream": [_h],
     "dsmc_init_fields": [_h, _f64p, _f64p, _f64p, _f64p],
     "dsmc_upload_particles": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p],

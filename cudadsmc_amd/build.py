@@ -14,6 +14,7 @@ LIB = os.path.join(PKG, "libcudadsmc.so")
 
 SOURCES = [
     os.path.join(CSRC, "dsmc_kernels.hip"),
+    os.path.join(CSRC, "dsmc_cell.hip"),
     os.path.join(CSRC, "dsmc_abi.hip"),
     os.path.join(CSRC, "host", "case_setup.cpp"),
 ]

@@ -58,6 +58,7 @@ struct dsmc_handle_s {
     float *d_seg = nullptr, *d_vol_ntc = nullptr, *d_ux = nullptr, *d_uy = nullptr, *d_sig = nullptr;
     int *d_cseg_start = nullptr, *d_cseg_idx = nullptr;
     uint8_t *d_cut = nullptr;
+    float *d_fbox = nullptr;
     double *d_expected = nullptr;
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
     int keep_perm = 0, timing = 0;
@@ -179,7 +180,7 @@ void make_info(const Param &P, const CutTables &T, const DevCase &cs, int64_t ca
 }
 
 void copy_tables(const CutTables &T, float *seg, int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc,
-                 uint8_t *cut, double *frac)
+                 uint8_t *cut, double *frac, float *fbox)
 {
     if (seg) std::copy(T.seg.begin(), T.seg.end(), seg);
     if (cseg_start) std::copy(T.cseg_start.begin(), T.cseg_start.end(), cseg_start);
@@ -188,6 +189,7 @@ void copy_tables(const CutTables &T, float *seg, int *cseg_start, int *cseg_idx,
     if (vol_ntc) std::copy(T.vol_ntc.begin(), T.vol_ntc.end(), vol_ntc);
     if (cut) std::copy(T.cut.begin(), T.cut.end(), cut);
     if (frac) std::copy(T.frac.begin(), T.frac.end(), frac);
+    if (fbox) std::copy(T.fbox.begin(), T.fbox.end(), fbox);
 }
 
 int fill(dsmc_handle h, const std::vector<double> &expected, const std::vector<float> &ux,
@@ -204,6 +206,7 @@ int fill(dsmc_handle h, const std::vector<double> &expected, const std::vector<f
     HIPCHK(hipMemcpyAsync(h->d_ux, ux.data(), sizeof(float) * C, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_uy, uy.data(), sizeof(float) * C, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_sig, sig.data(), sizeof(float) * C, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemsetAsync(h->ctrl, 0, sizeof(Ctrl), h->stream));
     HIPCHK(launch_fill(h->cs, h->A, h->d_expected, h->d_cut, h->d_ux, h->d_uy, h->d_sig, h->count, h->lstart,
                        h->tsum, h->toff, h->ctrl, h->grid_fill, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -268,7 +271,7 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
         (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, 6 * C)) ||
         (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
         (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||
-        (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) ||
+        (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) || (e = h->alloc(&h->d_fbox, 4 * C)) ||
         (e = h->alloc(&h->d_expected, C)) || (e = h->alloc(&h->d_ux, C)) || (e = h->alloc(&h->d_uy, C)) ||
         (e = h->alloc(&h->d_sig, C)))
         return bad(e, "cell buffers");
@@ -279,12 +282,14 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
         (e = up(h->d_cseg_start, T.cseg_start.data(), sizeof(int) * (C + 1))) ||
         (e = up(h->d_cseg_idx, T.cseg_idx.data(), sizeof(int) * T.cseg_idx.size())) ||
         (e = up(h->d_vol_ntc, T.vol_ntc.data(), sizeof(float) * C)) ||
-        (e = up(h->d_cut, T.cut.data(), C)) || (e = hipMemset(h->ctrl, 0, sizeof(Ctrl))))
+        (e = up(h->d_cut, T.cut.data(), C)) || (e = up(h->d_fbox, T.fbox.data(), sizeof(float) * 4 * C)) ||
+        (e = hipMemset(h->ctrl, 0, sizeof(Ctrl))))
         return bad(e, "table upload");
 
     DevCase &cs = h->cs;
     cs = make_devcase(P, T, h->cap);
     cs.seg = h->d_seg; cs.cseg_start = h->d_cseg_start; cs.cseg_idx = h->d_cseg_idx; cs.vol_ntc = h->d_vol_ntc;
+    cs.fbox = h->d_fbox;
 
     int cu = 0;
     if ((e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device)) != hipSuccess)
@@ -321,16 +326,16 @@ int dsmc_case_info_get(dsmc_handle h, dsmc_case_info *o)
 }
 
 int dsmc_case_tables(dsmc_handle h, float *seg, int *cseg_start, int *cseg_idx, double *vol,
-                     float *vol_ntc, uint8_t *cut)
+                     float *vol_ntc, uint8_t *cut, float *fbox)
 {
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
-    copy_tables(h->T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, nullptr);
+    copy_tables(h->T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, nullptr, fbox);
     return DSMC_OK;
 }
 
 int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_info *info, float *seg,
                        int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc, uint8_t *cut,
-                       double *frac)
+                       double *frac, float *fbox)
 {
     if (!p || !info) return fail(DSMC_ERR_ARG, "null argument");
     Param P;
@@ -342,7 +347,7 @@ int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_in
     int64_t cap = default_capacity(P, T);
     DevCase cs = make_devcase(P, T, cap);
     make_info(P, T, cs, cap, info);
-    copy_tables(T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, frac);
+    copy_tables(T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, frac, fbox);
     return DSMC_OK;
 }
 

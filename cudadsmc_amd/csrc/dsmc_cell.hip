@@ -1,0 +1,422 @@
+// k_cell: the per-cell half of a DSMC step on gfx950.
+//
+// One block (256 threads) takes a tile of consecutive cells whose particles
+// occupy [p0, p0+n) of the cell-sorted order (n <= CELL_CAP: staged in LDS;
+// larger tiles stage in the destination arrays in HBM).  Per tile:
+//   1. stable order inside each cell: the permutation segment of a cell is
+//      sorted ascending (insertion sort by one lane, block bitonic sort for
+//      big cells), so the result equals the serial reference order;
+//   2. gather of the state into cell order -- the outflow compaction of
+//      GPUParticles::DeleteInvalid (cudaDSMC GPUParticle.cu:76-140);
+//   3. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
+//      host semantics of Cell::collision, Cell.cpp:335-409): one lane per cell
+//      runs the serial candidate loop; cells with many candidates are done by
+//      a whole wave in conflict-free rounds with identical results;
+//   4. FP64 sampling sums, one lane per cell, serial order (samplingInCells,
+//      GPUCell.cu:336-402, as time-averaged accumulators);
+//   5. velocities and cell state written back coalesced.
+// The last block to finish advances the device step control.
+#include "dsmc_kernels.h"
+
+namespace dsmc {
+
+namespace {
+
+constexpr int CELL_MAXC = 256;       // cells whose metadata is staged at once
+constexpr int SORT_LANE_MAX = 96;    // insertion sort by one lane up to this many particles
+constexpr int NTC_LANE_MAX = 24;     // candidates one lane handles serially
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+struct CellShared {
+    int st[CELL_MAXC + 1];           // cell starts relative to the tile
+    float sm[CELL_MAXC], mr[CELL_MAXC], vol[CELL_MAXC];
+    int nc[CELL_MAXC], M[CELL_MAXC];
+    int big[CELL_MAXC], heavy[CELL_MAXC];
+    int nbig, nheavy;
+    uint32_t coll, cand;
+};
+
+// Block-wide ascending sort of p[0..k) (flip bitonic network: every
+// comparator is ascending, so indices >= k act as +inf without storage).
+template <bool GLOBAL>
+__device__ void block_sort(int *p, int k)
+{
+    int P2 = 1;
+    while (P2 < k) P2 <<= 1;
+    const int half = P2 >> 1;
+    for (int size = 2; size <= P2; size <<= 1) {
+        for (int j = size >> 1; j > 0; j >>= 1) {
+            for (int ci = threadIdx.x; ci < half; ci += blockDim.x) {
+                int a, b;
+                if (j == (size >> 1)) {
+                    int hs = size >> 1, blk = ci / hs, pos = ci - blk * hs;
+                    a = blk * size + pos;
+                    b = blk * size + size - 1 - pos;
+                } else {
+                    a = (ci / j) * 2 * j + (ci % j);
+                    b = a + j;
+                }
+                if (b < k) {
+                    int va = p[a], vb = p[b];
+                    if (vb < va) { p[a] = vb; p[b] = va; }
+                }
+            }
+            if (GLOBAL) __threadfence();
+            __syncthreads();
+        }
+    }
+}
+
+// Number of NTC candidates of a cell with remainder carry:
+// M = (int)(np(np-1) Fn dt/2 (sigma c)max / V + r),  r' = that - M.
+__device__ __forceinline__ int ntc_count(const DevCase &cs, int np, float vol, float smax, float &mrem)
+{
+    float e = (float)((long long)np * (long long)(np - 1)) * cs.ntc_coef;
+    e = e * smax;
+    e = e / vol;
+    const float tot = e + mrem;
+    const int M = (int)tot;
+    mrem = tot - (float)M;
+    return M;
+}
+
+// Post-collision velocities: isotropic (VHS) scattering of the relative
+// velocity about the centre of mass (Particle.cpp:84-100, GPUCell.cu:278-310).
+__device__ __forceinline__ void scatter_pair(const DevCase &cs, int c, uint32_t step, int m, uint32_t w3,
+                                             float g2, float &v1x, float &v1y, float &v1z, float &v2x,
+                                             float &v2y, float &v2z)
+{
+    float cosr = 2.0f * u01(w3) - 1.0f;
+    float sinr = sqrtf(1.0f - cosr * cosr);
+    uint4 w2 = philox((uint32_t)m, step, SUB_COLLIDE, 1u, (uint32_t)c, cs.seed);
+    float sp, cp;
+    dm_sincos2pi(u01(w2.x), sp, cp);
+    float g = sqrtf(g2);
+    float rx = g * cosr, ry = g * (sinr * cp), rz = g * (sinr * sp);
+    float mx = 0.5f * (v1x + v2x), my = 0.5f * (v1y + v2y), mz = 0.5f * (v1z + v2z);
+    v1x = mx + 0.5f * rx; v2x = mx - 0.5f * rx;
+    v1y = my + 0.5f * ry; v2y = my - 0.5f * ry;
+    v1z = mz + 0.5f * rz; v2z = mz - 0.5f * rz;
+}
+
+__device__ __forceinline__ float vhs_sigc(const DevCase &cs, float g2)
+{
+    return g2 > 0.0f ? cs.K1 * dm_expf(cs.e_exp * dm_logf(g2)) : 0.0f;
+}
+
+// Serial candidate loop of one cell by one lane.
+__device__ __forceinline__ int ntc_serial(const DevCase &cs, int c, uint32_t step, int np, int M, float *Vx,
+                                          float *Vy, float *Vz, float &smax)
+{
+    int nc = 0;
+    for (int m = 0; m < M; ++m) {
+        uint4 w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+        int i1 = mulhi_idx(w.x, np);
+        int i2 = mulhi_idx(w.y, np - 1);
+        if (i2 >= i1) i2 = i2 + 1;
+        float v1x = Vx[i1], v1y = Vy[i1], v1z = Vz[i1];
+        float v2x = Vx[i2], v2y = Vy[i2], v2z = Vz[i2];
+        float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
+        float g2 = gx * gx + gy * gy + gz * gz;
+        float sc = vhs_sigc(cs, g2);
+        if (sc > smax) smax = sc;
+        if (u01(w.z) < sc / smax) {
+            scatter_pair(cs, c, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+            Vx[i1] = v1x; Vy[i1] = v1y; Vz[i1] = v1z;
+            Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
+            ++nc;
+        }
+    }
+    return nc;
+}
+
+// The same loop by a whole wave: each round takes the longest prefix of the
+// next <= 64 candidates sharing no particle with an earlier candidate of the
+// round; (sigma c)max becomes a prefix max.  Identical results.
+template <bool GLOBAL>
+__device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M, float *Vx, float *Vy,
+                          float *Vz, float &smax)
+{
+    const int lane = lane_id();
+    int nc = 0;
+    for (int mb = 0; mb < M;) {
+        const int m = mb + lane;
+        const int nact = min(64, M - mb);
+        const bool act = lane < nact;
+        int i1 = -1, i2 = -2;
+        uint4 w = make_uint4(0u, 0u, 0u, 0u);
+        if (act) {
+            w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+            i1 = mulhi_idx(w.x, np);
+            i2 = mulhi_idx(w.y, np - 1);
+            if (i2 >= i1) i2 = i2 + 1;
+        }
+        bool conf = false;
+        for (int j = 0; j < nact; ++j) {
+            int a = __shfl(i1, j, 64), b = __shfl(i2, j, 64);
+            if (j < lane && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
+        }
+        const unsigned long long cm = __ballot(act && conf);
+        const int f = cm ? (int)__ffsll((long long)cm) - 1 : nact;
+        const bool proc = lane < f;
+        float sc = 0.0f, g2 = 0.0f;
+        float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
+        if (proc) {
+            v1x = Vx[i1]; v1y = Vy[i1]; v1z = Vz[i1];
+            v2x = Vx[i2]; v2y = Vy[i2]; v2z = Vz[i2];
+            float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
+            g2 = gx * gx + gy * gy + gz * gz;
+            sc = vhs_sigc(cs, g2);
+        }
+        float pm = sc;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            float t = __shfl_up(pm, d, 64);
+            if (lane >= d) pm = fmaxf(pm, t);
+        }
+        const float sml = fmaxf(smax, pm);
+        const bool accd = proc && (u01(w.z) < sc / sml);
+        if (accd) {
+            scatter_pair(cs, c, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+            Vx[i1] = v1x; Vy[i1] = v1y; Vz[i1] = v1z;
+            Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
+        }
+        nc += __popcll(__ballot(accd));
+        smax = fmaxf(smax, __shfl(pm, f - 1, 64));
+        mb += f;
+        if (GLOBAL) __threadfence_block();
+        else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    return nc;
+}
+
+struct CellArgs {
+    DevCase cs;
+    SoA A, B;
+    const int *start;
+    int *perm;
+    const int *tile_first;
+    float *sigcmax, *mrem;
+    int *ncoll_cell;
+    double *acc;
+    Ctrl *ctrl;
+    int sample_start;
+    int keep_perm;
+};
+
+// One tile: cells [ca, cb), particles [p0, p0+n).  LDS selects where the
+// permutation and velocities are staged (block LDS, or HBM in place).
+template <bool LDS>
+__device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uint32_t step, bool sampling,
+                          int ca, int cb, int p0, int n, int *Sp, float *Vx, float *Vy, float *Vz,
+                          CellShared &sh, uint32_t &tcoll, uint32_t &tcand)
+{
+    const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
+    const int nwaves = bs >> 6;
+    const DevCase &cs = a.cs;
+    if (LDS)
+        for (int k = tid; k < n; k += bs) Sp[k] = a.perm[p0 + k];
+    __syncthreads();
+
+    // 1. stable order within each cell
+    for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
+        const int nc = min(cb - c0, CELL_MAXC);
+        for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
+        if (tid == 0) sh.nbig = 0;
+        __syncthreads();
+        for (int i = tid; i < nc; i += bs) {
+            const int o = sh.st[i], k = sh.st[i + 1] - o;
+            if (k <= 1) continue;
+            if (k > SORT_LANE_MAX) {
+                sh.big[atomicAdd(&sh.nbig, 1)] = i;
+                continue;
+            }
+            int *s = Sp + o;
+            for (int q = 1; q < k; ++q) {
+                const int v = s[q];
+                int r = q - 1;
+                while (r >= 0 && s[r] > v) { s[r + 1] = s[r]; --r; }
+                s[r + 1] = v;
+            }
+        }
+        if (!LDS) __threadfence();
+        __syncthreads();
+        const int nb = sh.nbig;
+        for (int b = 0; b < nb; ++b) {
+            const int i = sh.big[b];
+            block_sort<!LDS>(Sp + sh.st[i], sh.st[i + 1] - sh.st[i]);
+        }
+        __syncthreads();
+    }
+    if (LDS && a.keep_perm)
+        for (int k = tid; k < n; k += bs) a.perm[p0 + k] = Sp[k];
+
+    // 2. gather into cell order
+    for (int k = tid; k < n; k += bs) {
+        const int q = Sp[k];
+        nxt.x[p0 + k] = cur.x[q];
+        nxt.y[p0 + k] = cur.y[q];
+        nxt.pid[p0 + k] = cur.pid[q];
+        Vx[k] = cur.vx[q];
+        Vy[k] = cur.vy[q];
+        Vz[k] = cur.vz[q];
+    }
+    if (!LDS) __threadfence();
+    __syncthreads();
+
+    // 3-4. collisions and sampling, chunk by chunk of cells
+    for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
+        const int nc = min(cb - c0, CELL_MAXC);
+        for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
+        for (int i = tid; i < nc; i += bs) {
+            sh.sm[i] = a.sigcmax[c0 + i];
+            sh.mr[i] = a.mrem[c0 + i];
+            sh.vol[i] = cs.vol_ntc[c0 + i];
+        }
+        if (tid == 0) sh.nheavy = 0;
+        __syncthreads();
+        for (int i = tid; i < nc; i += bs) {
+            const int o = sh.st[i], k = sh.st[i + 1] - o;
+            int ncoll = 0, M = 0;
+            const float vol = sh.vol[i];
+            if (k >= 2 && vol > 0.0f) {
+                float smax = sh.sm[i], mr = sh.mr[i];
+                M = ntc_count(cs, k, vol, smax, mr);
+                sh.mr[i] = mr;
+                if (M <= NTC_LANE_MAX) {
+                    ncoll = ntc_serial(cs, c0 + i, step, k, M, Vx + o, Vy + o, Vz + o, smax);
+                    sh.sm[i] = smax;
+                } else {
+                    sh.heavy[atomicAdd(&sh.nheavy, 1)] = i;
+                }
+            }
+            sh.nc[i] = ncoll;
+            sh.M[i] = M;
+            tcand += (uint32_t)M;
+        }
+        if (!LDS) __threadfence();
+        __syncthreads();
+        const int nh = sh.nheavy;
+        for (int h = wave; h < nh; h += nwaves) {
+            const int i = sh.heavy[h];
+            const int o = sh.st[i], k = sh.st[i + 1] - o;
+            float smax = sh.sm[i];
+            const int ncoll = ntc_rounds<!LDS>(cs, c0 + i, step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax);
+            if (lane == 0) { sh.sm[i] = smax; sh.nc[i] = ncoll; }
+        }
+        if (!LDS) __threadfence();
+        __syncthreads();
+        for (int i = tid; i < nc; i += bs) {
+            const int c = c0 + i;
+            const int o = sh.st[i], k = sh.st[i + 1] - o;
+            a.sigcmax[c] = sh.sm[i];
+            a.mrem[c] = sh.mr[i];
+            a.ncoll_cell[c] = sh.nc[i];
+            tcoll += (uint32_t)sh.nc[i];
+            if (sampling && k > 0) {
+                double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+                for (int q = 0; q < k; ++q) {
+                    const double u = (double)Vx[o + q], v = (double)Vy[o + q], w = (double)Vz[o + q];
+                    sx += u; sy += v; sz += w;
+                    s2 += u * u + v * v + w * w;
+                }
+                const size_t C = (size_t)cs.ncell;
+                a.acc[c] += (double)k;
+                a.acc[C + c] += sx;
+                a.acc[2 * C + c] += sy;
+                a.acc[3 * C + c] += sz;
+                a.acc[4 * C + c] += s2;
+                a.acc[5 * C + c] += (double)sh.nc[i];
+            }
+        }
+        __syncthreads();
+    }
+
+    // 5. velocities back to HBM
+    if (LDS)
+        for (int k = tid; k < n; k += bs) {
+            nxt.vx[p0 + k] = Vx[k];
+            nxt.vy[p0 + k] = Vy[k];
+            nxt.vz[p0 + k] = Vz[k];
+        }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_cell(CellArgs a)
+{
+    __shared__ int s_perm[CELL_CAP];
+    __shared__ float s_vx[CELL_CAP], s_vy[CELL_CAP], s_vz[CELL_CAP];
+    __shared__ CellShared sh;
+    Ctrl *ctrl = a.ctrl;
+    const uint32_t step = ctrl->step;
+    const SoA cur = (step & 1u) ? a.B : a.A;
+    const SoA nxt = (step & 1u) ? a.A : a.B;
+    const int n_next = (int)ctrl->n_next;
+    const int ntiles = (n_next + CELL_P - 1) / CELL_P;
+    const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
+    if (threadIdx.x == 0) { sh.coll = 0u; sh.cand = 0u; }
+    uint32_t tcoll = 0, tcand = 0;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ca = a.tile_first[t];
+        const int cb = (t + 1 < ntiles) ? a.tile_first[t + 1] : a.cs.ncell;
+        if (ca >= cb) continue;
+        const int p0 = a.start[ca], n = a.start[cb] - p0;
+        if (n <= CELL_CAP)
+            cell_tile<true>(a, cur, nxt, step, sampling, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll,
+                            tcand);
+        else
+            cell_tile<false>(a, cur, nxt, step, sampling, ca, cb, p0, n, a.perm + p0, nxt.vx + p0,
+                             nxt.vy + p0, nxt.vz + p0, sh, tcoll, tcand);
+    }
+    if (tcoll) atomicAdd(&sh.coll, tcoll);
+    if (tcand) atomicAdd(&sh.cand, tcand);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (sh.coll) atomicAdd(&ctrl->step_coll, sh.coll);
+        if (sh.cand) atomicAdd(&ctrl->step_cand, sh.cand);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (atomicAdd(&ctrl->cell_done, 1u) == gridDim.x - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint32_t n_live = ctrl->n_live;
+            const uint32_t sc = __hip_atomic_load(&ctrl->step_coll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t sm = __hip_atomic_load(&ctrl->step_cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ctrl->psteps += n_live;
+            ctrl->next_pid += ctrl->n_move - n_live;
+            ctrl->n_live = (uint32_t)n_next;
+            ctrl->last_coll = sc;
+            ctrl->ncoll += sc;
+            ctrl->last_cand = sm;
+            ctrl->ncand += sm;
+            ctrl->step_coll = 0u;
+            ctrl->step_cand = 0u;
+            if (sampling) ctrl->nsamples += 1ull;
+            ctrl->step = step + 1u;
+            ctrl->cell_done = 0u;
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
+                       const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
+                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int grid,
+                       hipStream_t st)
+{
+    CellArgs a{ cs, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
+                keep_perm };
+    hipLaunchKernelGGL(k_cell, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+int occupancy_cell()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell, 256, 0) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
+}
+
+}  // namespace dsmc

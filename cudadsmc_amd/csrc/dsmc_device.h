@@ -18,6 +18,8 @@ enum { ERR_BOUNCE = 1, ERR_INFLOW = 2, ERR_CAPACITY = 4, ERR_FILL = 8 };
 constexpr int MAX_BOUNCE = 64;
 constexpr int INFLOW_TRIES = 256;
 constexpr int FILL_TRIES = 256;
+constexpr float SEG_TOL = 1e-5f;   // relative overlap at body segment ends
+constexpr float SEG_BAND = 1e-4f;  // wall band (fraction of a cell) caught as an immediate hit
 
 // Structure-of-arrays particle state (24 B per particle).
 struct SoA {
@@ -41,6 +43,7 @@ struct DevCase {
     const float *seg;
     const int *cseg_start, *cseg_idx;
     const float *vol_ntc;
+    const float *fbox;      // 4 per cell: fluid-part bounding box of cut cells
 };
 
 // Device-resident step control: everything a step needs that changes
@@ -210,9 +213,18 @@ __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, 
                 if (den == 0.0f) continue;
                 float wx = ax - x, wy = ay - y;
                 float s = (wx * ey - wy * ex) / den;
-                if (!(s > 0.0f && s < best)) continue;
+                if (!(s < best)) continue;
+                if (!(s > 0.0f)) {
+                    // on or just behind the wall line (a previous hit point
+                    // rounded to the solid side) and moving in: hit now
+                    float d = -(wx * snx + wy * sny);
+                    if (d < -(cs.margin * SEG_BAND)) continue;
+                    s = 0.0f;
+                }
+                // small overlap at the segment ends: a ray through a shared
+                // vertex must not slip between two rounded tests
                 float u = (wx * vy - wy * vx) / den;
-                if (u < 0.0f || u > 1.0f) continue;
+                if (u < -SEG_TOL || u > 1.0f + SEG_TOL) continue;
                 best = s;
                 kind = 4 + k;
             }

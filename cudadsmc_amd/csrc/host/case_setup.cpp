@@ -146,6 +146,59 @@ bool segment_hits_rect(V2 a, V2 b, double x0, double y0, double x1, double y1)
     return t0 <= t1;
 }
 
+bool point_in_polygon(const std::vector<V2> &poly, double px, double py)
+{
+    bool in = false;
+    const size_t n = poly.size();
+    for (size_t k = 0; k < n; ++k) {
+        const V2 &a = poly[k], &b = poly[(k + 1) % n];
+        if ((a.y > py) != (b.y > py)) {
+            double xi = a.x + (py - a.y) * (b.x - a.x) / (b.y - a.y);
+            if (px < xi) in = !in;
+        }
+    }
+    return in;
+}
+
+// Bounding box of (cell rectangle minus body): the rectangle corners outside
+// the body, the crossings of body edges with the rectangle boundary and the
+// body vertices inside the rectangle.  Grown by one part in 1e5 of the cell
+// and clipped to the cell.  Used to place fill particles in cut cells.
+void fluid_bbox(const std::vector<V2> &poly, double x0, double y0, double x1, double y1, float *out)
+{
+    double bx0 = 1e300, by0 = 1e300, bx1 = -1e300, by1 = -1e300;
+    auto add = [&](double x, double y) {
+        bx0 = std::min(bx0, x); by0 = std::min(by0, y);
+        bx1 = std::max(bx1, x); by1 = std::max(by1, y);
+    };
+    const double cx[4] = { x0, x1, x1, x0 }, cy[4] = { y0, y0, y1, y1 };
+    for (int k = 0; k < 4; ++k)
+        if (!point_in_polygon(poly, cx[k], cy[k])) add(cx[k], cy[k]);
+    const size_t n = poly.size();
+    for (size_t k = 0; k < n; ++k) {
+        const V2 &a = poly[k], &b = poly[(k + 1) % n];
+        if (a.x >= x0 && a.x <= x1 && a.y >= y0 && a.y <= y1) add(a.x, a.y);
+        // crossings with the four rectangle sides
+        for (int s = 0; s < 4; ++s) {
+            const bool vert = (s & 1) == 0;             // s = 0: x = x0, 2: x = x1; 1: y = y0, 3: y = y1
+            const double L = s == 0 ? x0 : s == 2 ? x1 : s == 1 ? y0 : y1;
+            const double pa = vert ? a.x : a.y, pb = vert ? b.x : b.y;
+            if ((pa - L) * (pb - L) > 0.0 || pa == pb) continue;
+            const double t = (L - pa) / (pb - pa);
+            const double q = vert ? a.y + t * (b.y - a.y) : a.x + t * (b.x - a.x);
+            const double lo = vert ? y0 : x0, hi = vert ? y1 : x1;
+            if (q < lo || q > hi) continue;
+            if (vert) add(L, q); else add(q, L);
+        }
+    }
+    if (bx0 > bx1) { bx0 = x0; bx1 = x1; by0 = y0; by1 = y1; }
+    const double gx = 1e-5 * (x1 - x0), gy = 1e-5 * (y1 - y0);
+    out[0] = (float)std::max(x0, bx0 - gx);
+    out[1] = (float)std::max(y0, by0 - gy);
+    out[2] = (float)std::min(x1, bx1 + gx);
+    out[3] = (float)std::min(y1, by1 + gy);
+}
+
 }  // namespace
 
 std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T)
@@ -159,6 +212,7 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
     T.frac.assign(C, 1.0);
     T.vol_ntc.assign(C, (float)P.Vcell);
     T.cut.assign(C, 0);
+    T.fbox.assign(4 * (size_t)C, 0.0f);
     T.cseg_start.assign(C + 1, 0);
     if (!body || body->n_points == 0) return "";
     if (body->n_points < 3 || !body->xy) return "body polygon needs >= 3 points";
@@ -230,8 +284,12 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
             T.vol[c] = fr * P.Vcell;
             T.vol_ntc[c] = (float)std::max(T.vol[c], 0.1 * P.Vcell);
             T.cut[c] = fr > 0.0 ? 1 : 0;
-            if (fr > 0.0) ++T.ncut;
-            else T.vol_ntc[c] = 0.0f;
+            if (fr > 0.0) {
+                ++T.ncut;
+                fluid_bbox(poly, i * P.dx, j * P.dy, (i + 1) * P.dx, (j + 1) * P.dy, &T.fbox[4 * (size_t)c]);
+            } else {
+                T.vol_ntc[c] = 0.0f;
+            }
         }
     return "";
 }

@@ -46,6 +46,7 @@ struct CutTables {
     std::vector<double> frac;        // fluid fraction, exactly 1.0 for uncut cells
     std::vector<float> vol_ntc;      // volume used by NTC (floored at 0.1 cell)
     std::vector<uint8_t> cut;        // 0 < frac < 1
+    std::vector<float> fbox;         // 4 per cell: bounding box of the fluid part of a cut cell
     float margin = 0;
 };
 

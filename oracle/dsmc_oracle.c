@@ -205,9 +205,15 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
                 if (den == 0.0f) continue;
                 float wx = S[0] - x, wy = S[1] - y;
                 float s = (wx * ey - wy * ex) / den;
-                if (!(s > 0.0f && s < best)) continue;
+                if (!(s < best)) continue;
+                if (!(s > 0.0f)) {
+                    /* on or just behind the wall line and moving in: hit now */
+                    float d = -(wx * S[4] + wy * S[5]);
+                    if (d < -(cs->margin * OR_SEG_BAND)) continue;
+                    s = 0.0f;
+                }
                 float u = (wx * vy - wy * vx) / den;
-                if (u < 0.0f || u > 1.0f) continue;
+                if (u < -OR_SEG_TOL || u > 1.0f + OR_SEG_TOL) continue;
                 best = s;
                 kind = 4 + k;
             }
@@ -533,7 +539,12 @@ int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
     uint32_t e = 0;
     for (int c = 0; c < C; ++c) {
         int ci = c / cs->ny, cj = c % cs->ny;
-        float x0 = (float)ci * cs->dx, y0 = (float)cj * cs->dy;
+        /* uncut cells: the cell; cut cells: the bounding box of the fluid part */
+        const float *fb = cs->fbox + 4 * (size_t)c;
+        float x0 = cut[c] ? fb[0] : (float)ci * cs->dx;
+        float y0 = cut[c] ? fb[1] : (float)cj * cs->dy;
+        float wx = cut[c] ? fb[2] - fb[0] : cs->dx;
+        float wy = cut[c] ? fb[3] - fb[1] : cs->dy;
         for (int k = 0; k < count[c]; ++k, ++i) {
             uint32_t w[4];
             float px = x0, py = y0;
@@ -542,8 +553,8 @@ int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
                 uint32_t ctr[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 1u + (uint32_t)(tr / 2) };
                 or_philox(ctr, (uint32_t)c, cs->seed, w);
                 for (int h = 0; h < 2; ++h) {
-                    px = x0 + cs->dx * u01(w[2 * h]);
-                    py = y0 + cs->dy * u01(w[2 * h + 1]);
+                    px = x0 + wx * u01(w[2 * h]);
+                    py = y0 + wy * u01(w[2 * h + 1]);
                     if (!cut[c] || !inside_body(cs, px, py)) { ok = 1; break; }
                 }
             }

@@ -40,6 +40,13 @@ enum { OR_ERR_BOUNCE = 1, OR_ERR_INFLOW = 2, OR_ERR_CAPACITY = 4, OR_ERR_FILL = 
 #define OR_MAX_BOUNCE 64
 #define OR_INFLOW_TRIES 256
 #define OR_FILL_TRIES 256
+/* Segment ends overlap by this fraction so that a ray through a vertex shared
+ * by two body segments cannot pass between the two rounded tests. */
+#define OR_SEG_TOL 1e-5f
+/* A particle within this band (fraction of a cell) behind a body segment,
+ * moving into it, hits it at once: hit points rounded to the solid side of the
+ * wall line cannot leak on a later step. */
+#define OR_SEG_BAND 1e-4f
 
 /* One DSMC case: grid, gas, walls, inflow and the cut-cell tables.  The
  * scalar fields are the derived constants the GPU path uses (see
@@ -64,6 +71,7 @@ typedef struct or_case {
     const int   *cseg_start;           /* ncell+1 */
     const int   *cseg_idx;
     const float *vol_ntc;              /* per-cell volume used by NTC */
+    const float *fbox;                 /* 4 per cell: fluid-part bounding box of cut cells */
     uint32_t seed;
 } or_case;
 

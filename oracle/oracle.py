@@ -36,6 +36,7 @@ class or_case(C.Structure):
         ("cseg_start", C.POINTER(C.c_int)),
         ("cseg_idx", C.POINTER(C.c_int)),
         ("vol_ntc", C.POINTER(C.c_float)),
+        ("fbox", C.POINTER(C.c_float)),
         ("seed", C.c_uint32),
     ]
 
@@ -165,6 +166,7 @@ class OracleCase:
         c.cseg_start = _p(self.t["cseg_start"], C.c_int)
         c.cseg_idx = _p(self.t["cseg_idx"], C.c_int)
         c.vol_ntc = _p(self.t["vol_ntc"], C.c_float)
+        c.fbox = _p(self.t["fbox"], C.c_float)
         self.c = c
         self.ptr = C.byref(c)
 
