@@ -23,8 +23,8 @@ namespace dsmc {
 namespace {
 
 constexpr int CELL_MAXC = 256;       // cells whose metadata is staged at once
-constexpr int SORT_LANE_MAX = 96;    // insertion sort by one lane up to this many particles
 constexpr int NTC_LANE_MAX = 24;     // candidates one lane handles serially
+constexpr int GATHER_UNROLL = 4;     // particles per thread with loads in flight
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
@@ -33,6 +33,7 @@ struct CellShared {
     float sm[CELL_MAXC], mr[CELL_MAXC], vol[CELL_MAXC];
     int nc[CELL_MAXC], M[CELL_MAXC];
     int big[CELL_MAXC], heavy[CELL_MAXC];
+    int rstart[CELL_WAVES][64], rdst[CELL_WAVES][64];   // per-wave run tables
     int nbig, nheavy;
     uint32_t coll, cand;
 };
@@ -66,6 +67,110 @@ __device__ void block_sort(int *p, int k)
             __syncthreads();
         }
     }
+}
+
+// Ascending bitonic sort of one int per lane across the 64 lanes (flip
+// formulation: every comparator is ascending, so +inf padding is free).
+__device__ __forceinline__ int wave_sort64(int v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+        int partner = lane ^ (size - 1);
+        int o = __shfl(v, partner, 64);
+        v = lane < partner ? min(v, o) : max(v, o);
+#pragma unroll
+        for (int s = size >> 2; s > 0; s >>= 1) {
+            partner = lane ^ s;
+            o = __shfl(v, partner, 64);
+            v = lane < partner ? min(v, o) : max(v, o);
+        }
+    }
+    return v;
+}
+
+template <bool GLOBAL>
+__device__ __forceinline__ void wave_sync()
+{
+    if (GLOBAL) __threadfence_block();
+    else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Run starts of a cell segment S[0..len): element e starts a run when its
+// k_move wave (index >> 6) differs from that of e-1.  Calls f(e, run) for
+// every element, with run = index of its run, chunk by chunk.
+template <class F>
+__device__ __forceinline__ int for_runs(const int *S, int len, F f)
+{
+    const int lane = lane_id();
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    int nseen = 0, prevkey = -1;
+    for (int b = 0; b < len; b += 64) {
+        const int e = b + lane;
+        const int key = e < len ? (S[e] >> 6) : -2;
+        int pk = __shfl_up(key, 1, 64);
+        if (lane == 0) pk = prevkey;
+        const bool st = e < len && key != pk;
+        const unsigned long long m = __ballot(st);
+        if (e < len) f(e, nseen + __popcll(m & upto) - 1, st);
+        nseen += __popcll(m);
+        prevkey = __shfl(key, 63, 64);
+    }
+    return nseen;
+}
+
+// Sorts S[0..len) (len > 64) ascending with one wave.  The segment is a
+// concatenation of runs, one per k_move wave that binned into this cell;
+// each run is ascending and holds one value of (index >> 6).  Sorting the
+// runs by that key sorts the segment.  T is scratch of len ints; rstart and
+// rdst hold 64 ints.  Returns false (nothing moved) above 64 runs.
+template <bool GLOBAL>
+__device__ bool wave_run_sort(int *S, int *T, int len, int *rstart, int *rdst)
+{
+    const int lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    // 1. run starts
+    int nr = 0, prevkey = -1;
+    for (int b = 0; b < len; b += 64) {
+        const int e = b + lane;
+        const int key = e < len ? (S[e] >> 6) : -2;
+        int pk = __shfl_up(key, 1, 64);
+        if (lane == 0) pk = prevkey;
+        const bool st = e < len && key != pk;
+        const unsigned long long m = __ballot(st);
+        const int cnt = __popcll(m);
+        if (nr + cnt > 64) return false;
+        if (st) rstart[nr + __popcll(m & lt)] = e;
+        nr += cnt;
+        prevkey = __shfl(key, 63, 64);
+    }
+    if (nr <= 1) return true;
+    wave_sync<GLOBAL>();
+    // 2. order the runs by key; destinations by an exclusive scan of lengths
+    const int rs = lane < nr ? rstart[lane] : len;
+    int re = __shfl_down(rs, 1, 64);
+    if (lane + 1 >= nr) re = len;
+    const int rlen = lane < nr ? re - rs : 0;
+    const int packed = lane < nr ? (((S[rs] >> 6) << 6) | lane) : 0x7fffffff;
+    const int sorted = wave_sort64(packed);
+    const int rj = sorted & 63;
+    int lj = __shfl(rlen, rj, 64);
+    if (lane >= nr) lj = 0;
+    int inc = lj;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane < nr) rdst[rj] = inc - lj;
+    wave_sync<GLOBAL>();
+    // 3. move every element to its run's destination, then copy back
+    for_runs(S, len, [&](int e, int r, bool) { T[rdst[r] + e - rstart[r]] = S[e]; });
+    wave_sync<GLOBAL>();
+    for (int e = lane; e < len; e += 64) S[e] = T[e];
+    wave_sync<GLOBAL>();
+    return true;
 }
 
 // Number of NTC candidates of a cell with remainder carry:
@@ -136,10 +241,16 @@ __device__ __forceinline__ int ntc_serial(const DevCase &cs, int c, uint32_t ste
 // round; (sigma c)max becomes a prefix max.  Identical results.
 template <bool GLOBAL>
 __device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M, float *Vx, float *Vy,
-                          float *Vz, float &smax)
+                          float *Vz, float &smax, int *mark)
 {
     const int lane = lane_id();
     int nc = 0;
+    if (!GLOBAL) {
+        // mark[q] = lowest lane of the round that uses particle q (64 = none)
+        for (int q = lane; q < np; q += 64) mark[q] = 64;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
     for (int mb = 0; mb < M;) {
         const int m = mb + lane;
         const int nact = min(64, M - mb);
@@ -153,9 +264,19 @@ __device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M
             if (i2 >= i1) i2 = i2 + 1;
         }
         bool conf = false;
-        for (int j = 0; j < nact; ++j) {
-            int a = __shfl(i1, j, 64), b = __shfl(i2, j, 64);
-            if (j < lane && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
+        if (!GLOBAL) {
+            if (act) { atomicMin(&mark[i1], lane); atomicMin(&mark[i2], lane); }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (act) conf = mark[i1] < lane || mark[i2] < lane;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (act) { mark[i1] = 64; mark[i2] = 64; }
+        } else {
+            for (int j = 0; j < nact; ++j) {
+                int a = __shfl(i1, j, 64), b = __shfl(i2, j, 64);
+                if (j < lane && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
+            }
         }
         const unsigned long long cm = __ballot(act && conf);
         const int f = cm ? (int)__ffsll((long long)cm) - 1 : nact;
@@ -220,25 +341,25 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
         for (int k = tid; k < n; k += bs) Sp[k] = a.perm[p0 + k];
     __syncthreads();
 
-    // 1. stable order within each cell
+    // 1. stable order within each cell, one wave per cell: up to 64 particles
+    //    sorted in registers, longer segments by their k_move runs; more than
+    //    64 runs fall back to the block network.  The velocity staging array
+    //    is free until the gather and serves as scratch.
+    int *R = reinterpret_cast<int *>(Vx);
     for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
         const int nc = min(cb - c0, CELL_MAXC);
         for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
         if (tid == 0) sh.nbig = 0;
         __syncthreads();
-        for (int i = tid; i < nc; i += bs) {
-            const int o = sh.st[i], k = sh.st[i + 1] - o;
-            if (k <= 1) continue;
-            if (k > SORT_LANE_MAX) {
-                sh.big[atomicAdd(&sh.nbig, 1)] = i;
-                continue;
-            }
-            int *s = Sp + o;
-            for (int q = 1; q < k; ++q) {
-                const int v = s[q];
-                int r = q - 1;
-                while (r >= 0 && s[r] > v) { s[r + 1] = s[r]; --r; }
-                s[r + 1] = v;
+        for (int i = wave; i < nc; i += nwaves) {
+            const int o = sh.st[i], len = sh.st[i + 1] - o;
+            if (len <= 1) continue;
+            if (len <= 64) {
+                int v = lane < len ? Sp[o + lane] : 0x7fffffff;
+                v = wave_sort64(v);
+                if (lane < len) Sp[o + lane] = v;
+            } else if (!wave_run_sort<!LDS>(Sp + o, R + o, len, sh.rstart[wave], sh.rdst[wave])) {
+                if (lane == 0) sh.big[atomicAdd(&sh.nbig, 1)] = i;
             }
         }
         if (!LDS) __threadfence();
@@ -253,15 +374,37 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
     if (LDS && a.keep_perm)
         for (int k = tid; k < n; k += bs) a.perm[p0 + k] = Sp[k];
 
-    // 2. gather into cell order
-    for (int k = tid; k < n; k += bs) {
-        const int q = Sp[k];
-        nxt.x[p0 + k] = cur.x[q];
-        nxt.y[p0 + k] = cur.y[q];
-        nxt.pid[p0 + k] = cur.pid[q];
-        Vx[k] = cur.vx[q];
-        Vy[k] = cur.vy[q];
-        Vz[k] = cur.vz[q];
+    // 2. gather into cell order, GATHER_UNROLL particles per thread in flight
+    {
+        const float *__restrict__ cx = cur.x, *__restrict__ cy = cur.y;
+        const float *__restrict__ cvx = cur.vx, *__restrict__ cvy = cur.vy, *__restrict__ cvz = cur.vz;
+        const uint32_t *__restrict__ cp = cur.pid;
+        float *__restrict__ ox = nxt.x + p0, *__restrict__ oy = nxt.y + p0;
+        uint32_t *__restrict__ op = nxt.pid + p0;
+        for (int k0 = tid; k0 < n; k0 += GATHER_UNROLL * bs) {
+            int q[GATHER_UNROLL];
+            float gx[GATHER_UNROLL], gy[GATHER_UNROLL], gvx[GATHER_UNROLL], gvy[GATHER_UNROLL],
+                gvz[GATHER_UNROLL];
+            uint32_t gp[GATHER_UNROLL];
+#pragma unroll
+            for (int j = 0; j < GATHER_UNROLL; ++j) {
+                const int k = k0 + j * bs;
+                q[j] = k < n ? Sp[k] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < GATHER_UNROLL; ++j)
+                if (q[j] >= 0) {
+                    gx[j] = cx[q[j]]; gy[j] = cy[q[j]]; gp[j] = cp[q[j]];
+                    gvx[j] = cvx[q[j]]; gvy[j] = cvy[q[j]]; gvz[j] = cvz[q[j]];
+                }
+#pragma unroll
+            for (int j = 0; j < GATHER_UNROLL; ++j)
+                if (q[j] >= 0) {
+                    const int k = k0 + j * bs;
+                    ox[k] = gx[j]; oy[k] = gy[j]; op[k] = gp[j];
+                    Vx[k] = gvx[j]; Vy[k] = gvy[j]; Vz[k] = gvz[j];
+                }
+        }
     }
     if (!LDS) __threadfence();
     __syncthreads();
@@ -303,7 +446,8 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             const int i = sh.heavy[h];
             const int o = sh.st[i], k = sh.st[i + 1] - o;
             float smax = sh.sm[i];
-            const int ncoll = ntc_rounds<!LDS>(cs, c0 + i, step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax);
+            // the cell's permutation segment is free after the gather: marks
+            const int ncoll = ntc_rounds<!LDS>(cs, c0 + i, step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax, Sp + o);
             if (lane == 0) { sh.sm[i] = smax; sh.nc[i] = ncoll; }
         }
         if (!LDS) __threadfence();
@@ -356,9 +500,16 @@ __global__ __launch_bounds__(256) void k_cell(CellArgs a)
     const int n_next = (int)ctrl->n_next;
     const int ntiles = (n_next + CELL_P - 1) / CELL_P;
     const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
+    __shared__ int s_tile;
     if (threadIdx.x == 0) { sh.coll = 0u; sh.cand = 0u; }
     uint32_t tcoll = 0, tcand = 0;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (;;) {
+        // dynamic tile queue: dense tiles cost more, blocks take the next free one
+        if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u);
+        __syncthreads();
+        const int t = s_tile;
+        __syncthreads();
+        if (t >= ntiles) break;
         const int ca = a.tile_first[t];
         const int cb = (t + 1 < ntiles) ? a.tile_first[t + 1] : a.cs.ncell;
         if (ca >= cb) continue;
@@ -393,6 +544,7 @@ __global__ __launch_bounds__(256) void k_cell(CellArgs a)
             ctrl->step_cand = 0u;
             if (sampling) ctrl->nsamples += 1ull;
             ctrl->step = step + 1u;
+            ctrl->tile_next = 0u;
             ctrl->cell_done = 0u;
         }
     }

@@ -53,6 +53,7 @@ struct Ctrl {
     uint32_t step, n_live, n_move, n_next;
     uint32_t next_pid, err, scan_done, cell_done;
     uint32_t step_coll, last_coll, step_cand, last_cand;
+    uint32_t tile_next, pad0;      // k_cell work queue
     unsigned long long psteps, ncoll, nsamples, ncand;
 };
 

@@ -46,24 +46,28 @@ __device__ __forceinline__ int wave_sort64(int v)
 
 // Per-cell rank by one atomicAdd per distinct cell of the wave (the state is
 // cell-sorted, so a wave touches only a few cells): lanes sharing a cell get
-// consecutive slots in lane order.  Lanes with c < 0 take no slot.
+// consecutive slots in lane order, so each (wave, cell) group is one
+// ascending run of the cell's segment -- k_cell's sort relies on this.  The
+// groups are found in registers first and all their atomics go out together
+// (one memory round trip per wave).  Lanes with c < 0 take no slot.
 __device__ __forceinline__ int agg_slot(int *__restrict__ count, int c)
 {
     const int lane = lane_id();
     const unsigned long long lt = (1ull << lane) - 1ull;
     unsigned long long todo = __ballot(c >= 0);
-    int slot = 0;
+    int leader = lane, rank = 0, cnt = 0;
     while (todo) {
-        const int leader = (int)__ffsll((long long)todo) - 1;
-        const int lc = __shfl(c, leader, 64);
+        const int l = (int)__ffsll((long long)todo) - 1;
+        const int lc = __shfl(c, l, 64);
         const unsigned long long m = __ballot(c == lc);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[lc], __popcll(m));
-        base = __shfl(base, leader, 64);
-        if (c == lc) slot = base + __popcll(m & lt);
+        if (c == lc) { leader = l; rank = __popcll(m & lt); }
+        if (lane == l) cnt = __popcll(m);
         todo &= ~m;
     }
-    return slot;
+    int base = 0;
+    if (c >= 0 && lane == leader) base = atomicAdd(&count[c], cnt);
+    base = __shfl(base, leader, 64);
+    return c >= 0 ? base + rank : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -319,6 +323,7 @@ __global__ void k_ctrl_reset(Ctrl *ctrl, int use_n_next, uint32_t n, uint32_t ne
         if (!use_n_next) ctrl->err = 0u;  // after a fill, keep the fill's error bits
         ctrl->scan_done = 0u;
         ctrl->cell_done = 0u;
+        ctrl->tile_next = 0u;
         ctrl->step_coll = ctrl->last_coll = ctrl->step_cand = ctrl->last_cand = 0u;
         ctrl->psteps = ctrl->ncoll = ctrl->nsamples = ctrl->ncand = 0ull;
     }
