@@ -22,9 +22,10 @@ namespace dsmc {
 
 namespace {
 
-constexpr int CELL_MAXC = 256;       // cells whose metadata is staged at once
-constexpr int NTC_LANE_MAX = 24;     // candidates one lane handles serially
-constexpr int GATHER_UNROLL = 4;     // particles per thread with loads in flight
+constexpr int CELL_MAXC = 128;       // cells whose metadata is staged at once
+constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
+constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
+constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
@@ -173,6 +174,43 @@ __device__ bool wave_run_sort(int *S, int *T, int len, int *rstart, int *rdst)
     return true;
 }
 
+// Sorts a segment of len <= 64 by one wave using its k_move runs: already
+// sorted segments (one run) return at once; otherwise each run is ranked
+// against the others (few runs) and every element moves to
+// (sum of lengths of runs with smaller key) + (position in its run).
+template <bool GLOBAL>
+__device__ __forceinline__ void wave_small_sort(int *S, int len, int *rtab)
+{
+    const int lane = lane_id();
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const bool in = lane < len;
+    const int v = in ? S[lane] : 0;
+    const int pv = (in && lane > 0) ? S[lane - 1] : 0;
+    const bool st = in && (lane == 0 || (v >> 6) != (pv >> 6));
+    const unsigned long long m = __ballot(st);
+    const int nr = __popcll(m);
+    if (nr <= 1) return;
+    const int run = __popcll(m & upto) - 1;
+    if (st) rtab[run] = lane;
+    wave_sync<GLOBAL>();
+    const int rs = lane < nr ? rtab[lane] : len;
+    int re = __shfl_down(rs, 1, 64);
+    if (lane + 1 >= nr) re = len;
+    // every lane takes part in the shuffle (its source lane must be active)
+    const int sv = __shfl(v, rs & 63, 64);
+    const int rkey = lane < nr ? (sv >> 6) : 0x7fffffff;
+    const int rlen = lane < nr ? re - rs : 0;
+    int dst = 0;
+    for (int r = 0; r < nr; ++r) {
+        const int k2 = __shfl(rkey, r, 64), l2 = __shfl(rlen, r, 64);
+        if (k2 < rkey) dst += l2;
+    }
+    const int mydst = __shfl(dst, run, 64) + lane - __shfl(rs, run, 64);
+    wave_sync<GLOBAL>();
+    if (in) S[mydst] = v;
+    wave_sync<GLOBAL>();
+}
+
 // Number of NTC candidates of a cell with remainder carry:
 // M = (int)(np(np-1) Fn dt/2 (sigma c)max / V + r),  r' = that - M.
 __device__ __forceinline__ int ntc_count(const DevCase &cs, int np, float vol, float smax, float &mrem)
@@ -210,30 +248,79 @@ __device__ __forceinline__ float vhs_sigc(const DevCase &cs, float g2)
     return g2 > 0.0f ? cs.K1 * dm_expf(cs.e_exp * dm_logf(g2)) : 0.0f;
 }
 
-// Serial candidate loop of one cell by one lane.
-__device__ __forceinline__ int ntc_serial(const DevCase &cs, int c, uint32_t step, int np, int M, float *Vx,
-                                          float *Vy, float *Vz, float &smax)
+// The serial candidate loop of many cells at once: a wave takes 64/G cells,
+// G lanes per cell, and runs conflict-free rounds of up to G candidates in
+// each cell (the same rule as ntc_rounds, within G-lane segments).  list[]
+// holds the chunk-local cell numbers; results go to sh.sm / sh.nc.
+template <int G, bool GLOBAL>
+__device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *list, int nlist, CellShared &sh,
+                           float *Vx, float *Vy, float *Vz, int wave, int nwaves)
 {
-    int nc = 0;
-    for (int m = 0; m < M; ++m) {
-        uint4 w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
-        int i1 = mulhi_idx(w.x, np);
-        int i2 = mulhi_idx(w.y, np - 1);
-        if (i2 >= i1) i2 = i2 + 1;
-        float v1x = Vx[i1], v1y = Vy[i1], v1z = Vz[i1];
-        float v2x = Vx[i2], v2y = Vy[i2], v2z = Vz[i2];
-        float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
-        float g2 = gx * gx + gy * gy + gz * gz;
-        float sc = vhs_sigc(cs, g2);
-        if (sc > smax) smax = sc;
-        if (u01(w.z) < sc / smax) {
-            scatter_pair(cs, c, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
-            Vx[i1] = v1x; Vy[i1] = v1y; Vz[i1] = v1z;
-            Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
-            ++nc;
+    const int lane = lane_id(), g = lane / G, sub = lane % G;
+    const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (g * G);
+    constexpr int CPW = 64 / G;
+    for (int base = wave * CPW; base < nlist; base += nwaves * CPW) {
+        const int li = base + g;
+        const bool has = li < nlist;
+        const int i = has ? list[li] : 0;
+        const int o = has ? sh.st[i] : 0;
+        const int np = has ? sh.st[i + 1] - o : 2;
+        const int M = has ? sh.M[i] : 0;
+        const int c = c0 + i;
+        float *vx = Vx + o, *vy = Vy + o, *vz = Vz + o;
+        float smax = has ? sh.sm[i] : 0.0f;
+        int nc = 0, mb = 0;
+        while (__ballot(mb < M)) {
+            const int nact = mb < M ? min(G, M - mb) : 0;
+            const bool act = sub < nact;
+            const int m = mb + sub;
+            int i1 = -1, i2 = -2;
+            uint4 w = make_uint4(0u, 0u, 0u, 0u);
+            if (act) {
+                w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+                i1 = mulhi_idx(w.x, np);
+                i2 = mulhi_idx(w.y, np - 1);
+                if (i2 >= i1) i2 = i2 + 1;
+            }
+            bool conf = false;
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const int a = __shfl(i1, j, G), b = __shfl(i2, j, G);
+                if (j < sub && j < nact && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
+            }
+            const unsigned long long cm = __ballot(act && conf) & gmask;
+            const int f = cm ? (int)__ffsll((long long)cm) - 1 - g * G : nact;
+            const bool proc = sub < f;
+            float sc = 0.0f, g2 = 0.0f;
+            float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
+            if (proc) {
+                v1x = vx[i1]; v1y = vy[i1]; v1z = vz[i1];
+                v2x = vx[i2]; v2y = vy[i2]; v2z = vz[i2];
+                const float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
+                g2 = gx * gx + gy * gy + gz * gz;
+                sc = vhs_sigc(cs, g2);
+            }
+            float pm = sc;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                const float t = __shfl_up(pm, d, G);
+                if (sub >= d) pm = fmaxf(pm, t);
+            }
+            const float sml = fmaxf(smax, pm);
+            const bool accd = proc && (u01(w.z) < sc / sml);
+            if (accd) {
+                scatter_pair(cs, c, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+                vx[i1] = v1x; vy[i1] = v1y; vz[i1] = v1z;
+                vx[i2] = v2x; vy[i2] = v2y; vz[i2] = v2z;
+            }
+            nc += __popcll(__ballot(accd) & gmask);
+            const float last = __shfl(pm, f > 0 ? f - 1 : 0, G);
+            if (f > 0) smax = fmaxf(smax, last);
+            mb += f;
+            wave_sync<GLOBAL>();
         }
+        if (has && sub == 0) { sh.sm[i] = smax; sh.nc[i] = nc; }
     }
-    return nc;
 }
 
 // The same loop by a whole wave: each round takes the longest prefix of the
@@ -355,9 +442,7 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             const int o = sh.st[i], len = sh.st[i + 1] - o;
             if (len <= 1) continue;
             if (len <= 64) {
-                int v = lane < len ? Sp[o + lane] : 0x7fffffff;
-                v = wave_sort64(v);
-                if (lane < len) Sp[o + lane] = v;
+                wave_small_sort<!LDS>(Sp + o, len, sh.rstart[wave]);
             } else if (!wave_run_sort<!LDS>(Sp + o, R + o, len, sh.rstart[wave], sh.rdst[wave])) {
                 if (lane == 0) sh.big[atomicAdd(&sh.nbig, 1)] = i;
             }
@@ -418,27 +503,27 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             sh.mr[i] = a.mrem[c0 + i];
             sh.vol[i] = cs.vol_ntc[c0 + i];
         }
-        if (tid == 0) sh.nheavy = 0;
+        if (tid == 0) { sh.nheavy = 0; sh.nbig = 0; }
         __syncthreads();
+        // candidate counts; cells go to the grouped rounds or, with many
+        // candidates, to whole-wave rounds
         for (int i = tid; i < nc; i += bs) {
             const int o = sh.st[i], k = sh.st[i + 1] - o;
-            int ncoll = 0, M = 0;
+            int M = 0;
             const float vol = sh.vol[i];
             if (k >= 2 && vol > 0.0f) {
-                float smax = sh.sm[i], mr = sh.mr[i];
-                M = ntc_count(cs, k, vol, smax, mr);
+                float mr = sh.mr[i];
+                M = ntc_count(cs, k, vol, sh.sm[i], mr);
                 sh.mr[i] = mr;
-                if (M <= NTC_LANE_MAX) {
-                    ncoll = ntc_serial(cs, c0 + i, step, k, M, Vx + o, Vy + o, Vz + o, smax);
-                    sh.sm[i] = smax;
-                } else {
-                    sh.heavy[atomicAdd(&sh.nheavy, 1)] = i;
-                }
+                if (M > NTC_GROUP_MAX) sh.heavy[atomicAdd(&sh.nheavy, 1)] = i;
+                else if (M > 0) sh.big[atomicAdd(&sh.nbig, 1)] = i;
             }
-            sh.nc[i] = ncoll;
+            sh.nc[i] = 0;
             sh.M[i] = M;
             tcand += (uint32_t)M;
         }
+        __syncthreads();
+        ntc_groups<NTC_G, !LDS>(cs, step, c0, sh.big, sh.nbig, sh, Vx, Vy, Vz, wave, nwaves);
         if (!LDS) __threadfence();
         __syncthreads();
         const int nh = sh.nheavy;
@@ -488,7 +573,7 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_cell(CellArgs a)
+__global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
 {
     __shared__ int s_perm[CELL_CAP];
     __shared__ float s_vx[CELL_CAP], s_vy[CELL_CAP], s_vz[CELL_CAP];

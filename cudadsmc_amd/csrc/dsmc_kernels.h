@@ -8,8 +8,8 @@ namespace dsmc {
 
 constexpr int SCAN_IPT = 8;                  // cells per thread in k_scan
 constexpr int SCAN_TILE = 256 * SCAN_IPT;    // cells per k_scan block
-constexpr int CELL_P = 2048;                 // particles per k_cell tile (nominal)
-constexpr int CELL_CAP = 4096;               // LDS staging capacity of one tile
+constexpr int CELL_P = 1024;                 // particles per k_cell tile (nominal)
+constexpr int CELL_CAP = 2048;               // LDS staging capacity of one tile
 constexpr int CELL_BIG = 64;                 // cells with > 64 particles listed per tile
 constexpr int CELL_WAVES = 4;                // waves per k_cell block
 
