@@ -96,6 +96,8 @@ SIGNATURES = {
     "dsmc_download_perm": [_h, C.c_int64, _i32p],
     "dsmc_cell_state": [_h, _f32p, _f32p, _i32p],
     "dsmc_set_cell_state": [_h, _f32p, _f32p],
+    "dsmc_restore_counters": [_h, _P(dsmc_status)],
+    "dsmc_upload_samples": [_h, _f64p, C.c_uint64],
     "dsmc_reset_samples": [_h],
     "dsmc_download_samples": [_h, _f64p, _P(C.c_uint64)],
     "dsmc_download_fields": [_h, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],

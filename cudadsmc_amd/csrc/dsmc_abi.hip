@@ -66,6 +66,8 @@ struct dsmc_handle_s {
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     hipEvent_t ev[5]{};
+    hipEvent_t ev_flight[2]{};   // in-flight throttle of graph replays
+    int flight_k = 0;
     double t_ms[4]{}, t_bytes[4]{};
     int64_t t_launches = 0;
     std::vector<void *> allocs;
@@ -256,6 +258,9 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
     for (int k = 0; k < 5; ++k)
         if ((e = hipEventCreate(&h->ev[k])) != hipSuccess) return bad(e, "event");
+    for (int k = 0; k < 2; ++k)
+        if ((e = hipEventCreateWithFlags(&h->ev_flight[k], hipEventDisableTiming)) != hipSuccess)
+            return bad(e, "event");
     const size_t N = (size_t)h->cap, C = (size_t)T.ncell;
     SoA *bufs[2] = { &h->A, &h->B };
     for (SoA *s : bufs) {
@@ -313,6 +318,8 @@ int dsmc_destroy(dsmc_handle h)
     for (void *p : h->allocs) (void)hipFree(p);
     for (int k = 0; k < 5; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    for (int k = 0; k < 2; ++k)
+        if (h->ev_flight[k]) (void)hipEventDestroy(h->ev_flight[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DSMC_OK;
@@ -523,8 +530,19 @@ int dsmc_run(dsmc_handle h, int nsteps, int graph)
             // capture launches the kernels into the graph only; state is untouched
             if ((r = build_graph(h))) return r;
         }
+        // Replays are throttled to at most 2*kFlight graphs queued ahead of the
+        // device: every kFlight replays an event is recorded after waiting for
+        // the one recorded kFlight replays earlier, so the queue never drains.
+        constexpr int kFlight = 32;
         int full = nsteps / h->graph_steps;
-        for (int k = 0; k < full; ++k) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+        for (int k = 0; k < full; ++k) {
+            HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+            if (++h->flight_k % kFlight == 0) {
+                hipEvent_t e = h->ev_flight[(h->flight_k / kFlight) & 1];
+                HIPCHK(hipEventSynchronize(e));
+                HIPCHK(hipEventRecord(e, h->stream));
+            }
+        }
         for (int k = 0; k < nsteps % h->graph_steps; ++k)
             if ((r = launch_step(h, 7))) return r;
         return DSMC_OK;
@@ -654,6 +672,37 @@ int dsmc_set_cell_state(dsmc_handle h, const float *sigcmax, const float *mrem)
     const size_t C = h->ncell;
     if (sigcmax) HIPCHK(hipMemcpy(h->sigcmax, sigcmax, sizeof(float) * C, hipMemcpyHostToDevice));
     if (mrem) HIPCHK(hipMemcpy(h->mrem, mrem, sizeof(float) * C, hipMemcpyHostToDevice));
+    return DSMC_OK;
+}
+
+int dsmc_restore_counters(dsmc_handle h, const dsmc_status *s)
+{
+    if (!h || !s) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    if (s->next_pid < c.next_pid) return fail(DSMC_ERR_ARG, "next_pid below the uploaded particle ids");
+    c.step = s->step;
+    c.next_pid = s->next_pid;
+    c.psteps = s->particle_steps;
+    c.ncoll = s->collisions;
+    c.nsamples = s->nsamples;
+    c.last_coll = s->last_collisions;
+    HIPCHK(hipMemcpyAsync(h->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples)
+{
+    if (!h || !acc) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(h->acc, acc, sizeof(double) * 6 * (size_t)h->ncell, hipMemcpyHostToDevice, h->stream));
+    unsigned long long ns = nsamples;
+    HIPCHK(hipMemcpyAsync(&h->ctrl->nsamples, &ns, sizeof(ns), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return DSMC_OK;
 }
 

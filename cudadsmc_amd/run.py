@@ -170,6 +170,35 @@ class Run:
         b = None if mrem is None else abi.f32(mrem)
         check(self.lib.dsmc_set_cell_state(self.h, ptr(a, C.c_float), ptr(b, C.c_float)))
 
+    # -- checkpoint / resume (no counterpart in the reference) --
+    def save_state(self, path):
+        """Writes everything a bit-identical continuation needs to an .npz:
+        particles in stored order with ids, cell state, counters, samples."""
+        self.sync()
+        s = self.status()
+        p = self.download()
+        sm, mr, _ = self.cell_state()
+        acc, ns = self.samples()
+        np.savez(path, **p, sigcmax=sm, mrem=mr, acc=acc, nsamples=np.uint64(ns),
+                 counters=np.array([s.step, s.next_pid, s.particle_steps, s.collisions,
+                                    s.last_collisions], np.uint64),
+                 ncell=np.int64(self.ncell))
+
+    def load_state(self, path):
+        """Restores a state written by save_state into this (same-case) run."""
+        with np.load(path, allow_pickle=False) as z:
+            if int(z["ncell"]) != self.ncell:
+                raise ValueError(f"checkpoint has {int(z['ncell'])} cells, this case {self.ncell}")
+            self.upload(z["x"], z["y"], z["vx"], z["vy"], z["vz"], z["pid"])
+            self.set_cell_state(z["sigcmax"], z["mrem"])
+            c = [int(v) for v in z["counters"]]
+            s = abi.dsmc_status()
+            s.step, s.next_pid, s.particle_steps, s.collisions, s.last_collisions = c
+            s.nsamples = int(z["nsamples"])
+            check(self.lib.dsmc_restore_counters(self.h, C.byref(s)))
+            check(self.lib.dsmc_upload_samples(self.h, ptr(np.ascontiguousarray(z["acc"], np.float64).ravel(),
+                                                           C.c_double), int(z["nsamples"])))
+
     # -- sampling (GPUCells::Sample / DownloadToHost) --
     def reset_samples(self):
         check(self.lib.dsmc_reset_samples(self.h))

@@ -178,6 +178,16 @@ int dsmc_download_perm(dsmc_handle h, int64_t cap, int *perm);
 int dsmc_cell_state(dsmc_handle h, float *sigcmax, float *mrem, int *ncoll);
 int dsmc_set_cell_state(dsmc_handle h, const float *sigcmax, const float *mrem);
 
+/* Checkpoint / resume.  A run continues bit-identically from a saved state
+ * given: the particles in their stored order (dsmc_upload_particles with
+ * pids), the cell state (dsmc_set_cell_state), the counters below and the
+ * sample accumulators.  dsmc_restore_counters sets step, next_pid,
+ * particle_steps, collisions and nsamples from *s (n_live stays that of the
+ * last upload; the other fields are ignored).  The reference writes fields
+ * only (Output.cpp:12-154) and has no resume; these replace nothing. */
+int dsmc_restore_counters(dsmc_handle h, const dsmc_status *s);
+int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples);
+
 /* Sampling (GPUCells::Sample + DownloadToHost, GPUCell.cu:83-91,147-167).
  * acc[6*ncell]: six SoA blocks sum n, sum vx, sum vy, sum vz, sum v^2,
  * collisions.  Fields: number density n, velocity, temperature, pressure. */
