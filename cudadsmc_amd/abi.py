@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcudadsmc.so")
 
 DSMC_OK = 0
+DSMC_ERR_ARG, DSMC_ERR_HIP, DSMC_ERR_CAPACITY, DSMC_ERR_GEOMETRY, DSMC_ERR_NODEVICE = -1, -2, -3, -4, -5
 BC_STREAM, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
 DEV_ERR_BOUNCE, DEV_ERR_INFLOW, DEV_ERR_CAPACITY, DEV_ERR_FILL = 1, 2, 4, 8
 
