@@ -1,17 +1,21 @@
 // k_cell: the per-cell half of a DSMC step on gfx950.
 //
-// One block (256 threads) takes a tile of consecutive cells whose particles
-// occupy [p0, p0+n) of the cell-sorted order (n <= CELL_CAP: staged in LDS;
+// One block (4 waves of 64) takes a tile of consecutive cells whose particles
+// occupy [p0, p0+n) of the cell-sorted order; tiles of about CELL_P particles
+// are handed out by a device-wide queue (n <= CELL_CAP: staged in LDS;
 // larger tiles stage in the destination arrays in HBM).  Per tile:
-//   1. stable order inside each cell: the permutation segment of a cell is
-//      sorted ascending (insertion sort by one lane, block bitonic sort for
-//      big cells), so the result equals the serial reference order;
+//   1. stable order inside each cell: a cell's permutation segment is made
+//      ascending by one wave -- the scatter leaves it as a concatenation of
+//      ascending runs, one per k_move wave, so short segments are ranked run
+//      against run, long ones sorted by run key, and a block bitonic sort
+//      takes the rest; the result equals the serial reference order;
 //   2. gather of the state into cell order -- the outflow compaction of
 //      GPUParticles::DeleteInvalid (cudaDSMC GPUParticle.cu:76-140);
 //   3. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
-//      host semantics of Cell::collision, Cell.cpp:335-409): one lane per cell
-//      runs the serial candidate loop; cells with many candidates are done by
-//      a whole wave in conflict-free rounds with identical results;
+//      host semantics of Cell::collision, Cell.cpp:335-409): candidate counts
+//      one lane per cell; cells with up to 64 candidates run in groups of 8
+//      lanes, heavier cells by a whole wave, both in conflict-free rounds
+//      whose results equal the serial candidate loop bit for bit;
 //   4. FP64 sampling sums, one lane per cell, serial order (samplingInCells,
 //      GPUCell.cu:336-402, as time-averaged accumulators);
 //   5. velocities and cell state written back coalesced.
