@@ -102,6 +102,22 @@ SIGNATURES = {
     "dsmc_reset_samples": [_h],
     "dsmc_download_samples": [_h, _f64p, _P(C.c_uint64)],
     "dsmc_download_fields": [_h, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],
+    "dsmc_batch_create": [_P(dsmc_params), _P(dsmc_body), C.c_int, _P(C.c_void_p)],
+    "dsmc_batch_destroy": [_h],
+    "dsmc_batch_case_info": [_h, C.c_int, _P(dsmc_case_info)],
+    "dsmc_batch_init_freestream": [_h],
+    "dsmc_batch_init_fields": [_h, _f64p, _f64p, _f64p, _f64p],
+    "dsmc_batch_run": [_h, C.c_int, C.c_int],
+    "dsmc_batch_sync": [_h],
+    "dsmc_batch_status": [_h, C.c_int, _P(dsmc_status)],
+    "dsmc_batch_download_particles": [_h, C.c_int, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p,
+                                      _P(C.c_int64)],
+    "dsmc_batch_cell_state": [_h, C.c_int, _f32p, _f32p, _i32p],
+    "dsmc_batch_reset_samples": [_h],
+    "dsmc_batch_download_samples": [_h, C.c_int, _f64p, _P(C.c_uint64)],
+    "dsmc_batch_download_fields": [_h, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],
+    "dsmc_batch_set_timing": [_h, C.c_int],
+    "dsmc_batch_kernel_times": [_h, _f64p, _f64p, _P(C.c_int64)],
     "dsmc_set_timing": [_h, C.c_int],
     "dsmc_kernel_times": [_h, _f64p, _f64p, _P(C.c_int64)],
 }

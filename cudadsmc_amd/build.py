@@ -16,6 +16,7 @@ SOURCES = [
     os.path.join(CSRC, "dsmc_kernels.hip"),
     os.path.join(CSRC, "dsmc_cell.hip"),
     os.path.join(CSRC, "dsmc_abi.hip"),
+    os.path.join(CSRC, "dsmc_batch.hip"),
     os.path.join(CSRC, "host", "case_setup.cpp"),
 ]
 

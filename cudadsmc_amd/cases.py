@@ -101,3 +101,63 @@ def cylinder_mach10(ppc=30, nx=400, ny=200, n_points=256, steps=20000, sample_fr
     body = circle_polygon(n_points, 0.5 * L1, 0.5 * L2, R)
     return dict(name=f"cylinder_mach10_{nx}x{ny}", params=p, body=body, steps=steps, U=U,
                 sample_from=sample_from)
+
+
+def flow_case(geometry="cylinder", mach=10.0, kn=0.01, tw_ratio=2.5, nx=400, ny=200, ppc=30.0,
+              half_angle_deg=20.0, T_inf=200.0, steps=20000, sample_from=10000, seed=20251203, n_points=256):
+    """Supersonic argon flow over a cylinder (R = 0.1 m, centred) or a wedge
+    (length 0.3 m, apex at x = 0.15 m) in a 0.6 m x 0.3 m domain:
+    freestream inflow/outflow on all faces, diffuse wall at tw_ratio*T_inf,
+    Kn on the body size (cylinder diameter, wedge length) with the
+    hard-sphere mean free path, dt = 0.3 dx / (U + 3 vmp)."""
+    m = ARGON["mass"]
+    a = math.sqrt(5.0 / 3.0 * BOLTZ * T_inf / m)
+    U = mach * a
+    L1, L2 = 0.6, 0.3
+    if geometry == "cylinder":
+        size = 0.2
+        body = circle_polygon(n_points, 0.5 * L1, 0.5 * L2, 0.1)
+    elif geometry == "wedge":
+        size = 0.3
+        body = wedge_polygon(0.15, 0.5 * L2, 0.3, half_angle_deg)
+    else:
+        raise ValueError(f"unknown geometry {geometry!r}")
+    n = hard_sphere_density_for_mfp(kn * size)
+    dx = L1 / nx
+    vmp = math.sqrt(2.0 * BOLTZ * T_inf / m)
+    dt = 0.3 * dx / (U + 3.0 * vmp)
+    p = make_params(L1=L1, L2=L2, N1=nx, N2=ny, n_inf=n, T=T_inf, V_jet=U, T_wall=tw_ratio * T_inf,
+                    wall_bc=abi.BC_DIFFUSE, face_bc=[abi.BC_STREAM] * 4, tau=dt,
+                    ppc=float(ppc), seed=seed, sample_start=sample_from)
+    return dict(name=f"{geometry}_M{mach:.2f}_Kn{kn:.4f}_Tw{tw_ratio:.2f}_{nx}x{ny}", params=p, body=body,
+                steps=steps, U=U, sample_from=sample_from,
+                point=dict(geometry=geometry, mach=mach, kn=kn, tw_ratio=tw_ratio,
+                           half_angle_deg=half_angle_deg if geometry == "wedge" else 0.0))
+
+
+SWEEP_GRIDS = [(64, 32), (96, 48), (128, 64), (160, 80), (192, 96), (224, 112), (256, 128)]
+
+
+def sweep(n=64, seed=20251203, particles=(2e5, 6e6), steps=20000, sample_from=10000, n_points=128):
+    """The batched parameter sweep, every value drawn from `seed`: geometry
+    cylinder or wedge (half-angle U[10, 30] deg), Mach U[3, 12], Kn
+    log-U[0.005, 0.1], T_w/T_inf U[1, 5], grid 64x32 ... 256x128, particle
+    count log-U over `particles` (uneven instances, about 1.7M on average)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        geom = "cylinder" if rng.random() < 0.5 else "wedge"
+        half = float(rng.uniform(10.0, 30.0))
+        mach = float(rng.uniform(3.0, 12.0))
+        kn = float(math.exp(rng.uniform(math.log(0.005), math.log(0.1))))
+        tw = float(rng.uniform(1.0, 5.0))
+        nx, ny = SWEEP_GRIDS[int(rng.integers(len(SWEEP_GRIDS)))]
+        npart = float(math.exp(rng.uniform(math.log(particles[0]), math.log(particles[1]))))
+        solid = 0.3 * 0.3 * math.tan(math.radians(half)) if geom == "wedge" else math.pi * 0.01
+        fluid_cells = nx * ny * (1.0 - solid / 0.18)
+        cfg = flow_case(geom, mach, kn, tw, nx, ny, ppc=npart / fluid_cells, half_angle_deg=half,
+                        steps=steps, sample_from=sample_from, seed=int(rng.integers(1, 2**62)),
+                        n_points=n_points)
+        cfg["index"] = i
+        out.append(cfg)
+    return out

@@ -10,33 +10,23 @@
 #include <vector>
 
 #include "../../include/dsmc_abi.h"
-#include "dsmc_kernels.h"
-#include "host/case_setup.h"
+#include "host_common.h"
 
 using namespace dsmc;
 
+namespace dsmc {
+
+std::string &last_error_slot()
+{
+    static thread_local std::string err;
+    return err;
+}
+
+}  // namespace dsmc
+
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg)
-{
-    g_err = msg;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(DSMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-template <class T>
-hipError_t dalloc(T **p, size_t n)
-{
-    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
-}
+#define g_err (dsmc::last_error_slot())
 
 }  // namespace
 
@@ -130,37 +120,6 @@ int reset_cell_state(dsmc_handle h)
     HIPCHK(hipMemsetAsync(h->ncoll_cell, 0, sizeof(int) * h->ncell, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return DSMC_OK;
-}
-
-int64_t default_capacity(const Param &P, const CutTables &T)
-{
-    if (P.in.capacity > 0) return P.in.capacity;
-    double fill_est = 0;
-    for (int c = 0; c < T.ncell; ++c) fill_est += P.ppc_eff * T.frac[c];
-    return (int64_t)std::ceil(4.0 * fill_est) + 65536;
-}
-
-// Scalar constants of the kernels (device table pointers are set by the caller).
-DevCase make_devcase(const Param &P, const CutTables &T, int64_t cap)
-{
-    const dsmc_params *p = &P.in;
-    DevCase cs{};
-    cs.nx = p->N1; cs.ny = p->N2; cs.ncell = T.ncell; cs.nseg = T.nseg;
-    cs.L1 = (float)p->L1; cs.L2 = (float)p->L2; cs.dx = (float)P.dx; cs.dy = (float)P.dy;
-    cs.dt = (float)p->tau;
-    cs.ntc_coef = (float)P.ntc_coef; cs.K1 = (float)P.K1; cs.e_exp = (float)P.e_exp; cs.margin = T.margin;
-    cs.body_bc = p->wall_bc; cs.body_vmp = (float)P.body_vmp; cs.body_sig = (float)P.body_sig;
-    for (int f = 0; f < 4; ++f) {
-        cs.face_bc[f] = p->face_bc[f];
-        cs.face_vmp[f] = (float)P.face_vmp[f];
-        cs.face_sig[f] = (float)P.face_sig[f];
-        cs.inj_e[f] = P.inj_e[f];
-        cs.inj_a[f] = (float)P.inj_a[f];
-    }
-    cs.inj_vmp = (float)P.inj_vmp; cs.inj_sig = (float)P.inj_sig;
-    cs.inj_ux = (float)p->V_jet; cs.inj_uy = (float)p->V_y;
-    cs.seed = P.seed32; cs.cap = (uint32_t)cap;
-    return cs;
 }
 
 void make_info(const Param &P, const CutTables &T, const DevCase &cs, int64_t cap, dsmc_case_info *o)
@@ -737,23 +696,7 @@ int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy, doubl
     uint64_t ns = 0;
     int r = dsmc_download_samples(h, a.data(), &ns);
     if (r) return r;
-    const double k = h->P.in.boltz, m = h->P.in.mass, Fn = h->P.Fn;
-    for (size_t c = 0; c < C; ++c) {
-        const double sn = a[c];
-        double vol = h->T.vol[c];
-        double nn = (ns > 0 && vol > 0) ? sn / (double)ns * Fn / vol : 0.0;
-        double u = 0, v = 0, w = 0, t = 0;
-        if (sn > 0) {
-            u = a[C + c] / sn; v = a[2 * C + c] / sn; w = a[3 * C + c] / sn;
-            t = m / (3.0 * k) * (a[4 * C + c] / sn - (u * u + v * v + w * w));
-        }
-        if (n) n[c] = nn;
-        if (ux) ux[c] = u;
-        if (uy) uy[c] = v;
-        if (uz) uz[c] = w;
-        if (T) T[c] = t;
-        if (p) p[c] = nn * k * t;
-    }
+    fields_from_sums(h->P, h->T, a.data(), C, ns, n, ux, uy, uz, T, p);
     return DSMC_OK;
 }
 

@@ -1,0 +1,591 @@
+// Batched instances (include/dsmc_abi.h, dsmc_batch_*): many independent
+// DSMC cases in one set of device arrays, advanced by one launch per phase.
+//
+// Layout: the cells of all instances are concatenated in instance order
+// (instance i owns global cells [cbase_i, cbase_i + C_i)); the particles
+// are kept in global cell order by k_cell, so instance i's particles are a
+// contiguous range whose start is the prefix of the per-instance live
+// counts.  The scan and the scatter run unchanged over all cells; k_move
+// and k_cell find each particle's / cell's instance in LDS offset tables.
+// No instance sees another: every random stream is keyed on the instance's
+// own particle ids, cell ids and seed, so an instance evolves exactly as it
+// would in a dsmc_handle of its own.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/dsmc_abi.h"
+#include "host_common.h"
+
+using namespace dsmc;
+
+struct dsmc_batch_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ninst = 0;
+    std::vector<Param> P;
+    std::vector<CutTables> T;
+    std::vector<DevCase> cs;           // per instance, device table pointers set
+    std::vector<int> cbase;            // ninst + 1
+    std::vector<int64_t> capi;         // per-instance capacity
+    int ncell = 0, ntile_scan = 0;
+    int64_t cap = 0;
+    SoA A{}, B{};
+    int2 *cellslot = nullptr;
+    int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;
+    int *start = nullptr, *tile_first = nullptr, *ncoll_cell = nullptr;
+    float *sigcmax = nullptr, *mrem = nullptr;
+    double *acc = nullptr;
+    Ctrl *ctrl = nullptr, *fill_ctrl = nullptr;
+    InstCtrl *ictl = nullptr;
+    DevCase *d_cases = nullptr;
+    CellConst *d_cc = nullptr;
+    int *d_cbase = nullptr;
+    float *d_seg = nullptr, *d_vol_ntc = nullptr, *d_fbox = nullptr;
+    float *d_ux = nullptr, *d_uy = nullptr, *d_sig = nullptr;
+    int *d_cseg_start = nullptr, *d_cseg_idx = nullptr;
+    uint8_t *d_cut = nullptr;
+    double *d_expected = nullptr;
+    DevCase all{};                     // what k_cell reads of "the case": all cells
+    int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev[5]{};
+    hipEvent_t ev_flight[2]{};
+    int flight_k = 0;
+    int timing = 0;
+    double t_ms[4]{}, t_bytes[4]{};
+    int64_t t_launches = 0;
+    std::vector<void *> allocs;
+
+    template <class U>
+    hipError_t alloc(U **p, size_t n)
+    {
+        hipError_t e = dalloc(p, n);
+        if (e == hipSuccess) allocs.push_back((void *)*p);
+        return e;
+    }
+};
+
+namespace {
+
+constexpr int kGraphSteps = 8;
+
+int set_device(dsmc_batch b)
+{
+    HIPCHK(hipSetDevice(b->device));
+    return DSMC_OK;
+}
+
+int launch_step(dsmc_batch b, int mask)
+{
+    hipStream_t st = b->stream;
+    if (mask & 1)
+        HIPCHK(launch_move_batch(b->d_cases, b->ninst, b->d_cbase, b->A, b->B, b->cellslot, b->count, b->ctrl,
+                                 b->ictl, b->grid_move, st));
+    if (mask & 2) HIPCHK(launch_scan(b->count, b->lstart, b->tsum, b->toff, b->ncell, b->ctrl, st));
+    if (mask & 4)
+        HIPCHK(launch_scatter(b->cellslot, b->lstart, b->toff, b->start, b->perm, b->tile_first, b->ncell, b->ctrl,
+                              b->grid_scatter, st));
+    if (mask & 8)
+        HIPCHK(launch_cell_batch(b->all, b->d_cc, b->d_cbase, b->ninst, b->ictl, b->A, b->B, b->start, b->perm,
+                                 b->tile_first, b->sigcmax, b->mrem, b->ncoll_cell, b->acc, b->ctrl, b->grid_cell,
+                                 st));
+    return DSMC_OK;
+}
+
+int read_ctrl(dsmc_batch b, Ctrl &c, std::vector<InstCtrl> *ic = nullptr)
+{
+    HIPCHK(hipMemcpyAsync(&c, b->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, b->stream));
+    if (ic) {
+        ic->resize(b->ninst);
+        HIPCHK(hipMemcpyAsync(ic->data(), b->ictl, sizeof(InstCtrl) * b->ninst, hipMemcpyDeviceToHost, b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return DSMC_OK;
+}
+
+void drop_graph(dsmc_batch b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_steps = 0;
+}
+
+int build_graph(dsmc_batch b)
+{
+    drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int r = DSMC_OK;
+    for (int s = 0; s < kGraphSteps && r == DSMC_OK; ++s) r = launch_step(b, 15);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+    if (e != hipSuccess) return fail(DSMC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->graph_steps = kGraphSteps;
+    return DSMC_OK;
+}
+
+int reset_cell_state(dsmc_batch b)
+{
+    std::vector<float> s0(b->ncell);
+    for (int i = 0; i < b->ninst; ++i)
+        for (int c = b->cbase[i]; c < b->cbase[i + 1]; ++c) s0[c] = (float)b->P[i].sigcmax0;
+    const size_t C = b->ncell;
+    HIPCHK(hipMemcpyAsync(b->sigcmax, s0.data(), sizeof(float) * C, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemsetAsync(b->mrem, 0, sizeof(float) * C, b->stream));
+    HIPCHK(hipMemsetAsync(b->acc, 0, sizeof(double) * 6 * C, b->stream));
+    HIPCHK(hipMemsetAsync(b->count, 0, sizeof(int) * C, b->stream));
+    HIPCHK(hipMemsetAsync(b->ncoll_cell, 0, sizeof(int) * C, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return DSMC_OK;
+}
+
+// Fills every instance from per-cell expected counts and drifting Maxwellians
+// (the same kernels as a single case, one instance after the other, each
+// writing its particles after the previous instance's).
+int fill(dsmc_batch b, const std::vector<double> &e, const std::vector<float> &ux, const std::vector<float> &uy,
+         const std::vector<float> &sig)
+{
+    for (int i = 0; i < b->ninst; ++i) {
+        double tot = 0;
+        for (int c = b->cbase[i]; c < b->cbase[i + 1]; ++c) tot += e[c];
+        if (tot + 6.0 * std::sqrt(tot + 1.0) + (double)(b->cbase[i + 1] - b->cbase[i]) > (double)b->capi[i])
+            return fail(DSMC_ERR_CAPACITY, "instance " + std::to_string(i) + ": fill exceeds its particle capacity");
+    }
+    drop_graph(b);
+    int r = reset_cell_state(b);
+    if (r) return r;
+    const size_t C = b->ncell;
+    hipStream_t st = b->stream;
+    HIPCHK(hipMemcpyAsync(b->d_expected, e.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_ux, ux.data(), sizeof(float) * C, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_uy, uy.data(), sizeof(float) * C, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_sig, sig.data(), sizeof(float) * C, hipMemcpyHostToDevice, st));
+    std::vector<InstCtrl> ic(b->ninst);
+    uint32_t off = 0, err_all = 0;
+    for (int i = 0; i < b->ninst; ++i) {
+        const int cb = b->cbase[i];
+        SoA Ai{ b->A.x + off, b->A.y + off, b->A.vx + off, b->A.vy + off, b->A.vz + off, b->A.pid + off };
+        HIPCHK(hipMemsetAsync(b->fill_ctrl, 0, sizeof(Ctrl), st));
+        HIPCHK(launch_fill(b->cs[i], Ai, b->d_expected + cb, b->d_cut + cb, b->d_ux + cb, b->d_uy + cb,
+                           b->d_sig + cb, b->count + cb, b->lstart, b->tsum, b->toff, b->fill_ctrl, b->grid_fill, st));
+        Ctrl f;
+        HIPCHK(hipMemcpyAsync(&f, b->fill_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::memset(&ic[i], 0, sizeof(InstCtrl));
+        ic[i].n_live = ic[i].n_move = ic[i].next_pid = f.n_live;
+        ic[i].err = f.err;
+        err_all |= f.err;
+        off += f.n_live;
+    }
+    Ctrl g;
+    std::memset(&g, 0, sizeof(g));
+    g.n_live = g.n_move = g.n_next = off;
+    g.err = err_all;
+    HIPCHK(hipMemcpyAsync(b->ictl, ic.data(), sizeof(InstCtrl) * b->ninst, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->ctrl, &g, sizeof(Ctrl), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return DSMC_OK;
+}
+
+bool valid_index(dsmc_batch b, int i) { return b && i >= 0 && i < b->ninst; }
+
+}  // namespace
+
+extern "C" {
+
+int dsmc_batch_destroy(dsmc_batch b)
+{
+    if (!b) return DSMC_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (void *p : b->allocs) (void)hipFree(p);
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : b->ev_flight)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return DSMC_OK;
+}
+
+int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n, dsmc_batch *out)
+{
+    if (!params || !out || n < 1) return fail(DSMC_ERR_ARG, "batch needs n >= 1 cases");
+    if (n > MAX_INST) return fail(DSMC_ERR_ARG, "batch holds at most " + std::to_string(MAX_INST) + " instances");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(DSMC_ERR_NODEVICE, "no HIP device");
+    dsmc_batch b = new dsmc_batch_s();
+    b->ninst = n;
+    b->P.resize(n);
+    b->T.resize(n);
+    b->cs.resize(n);
+    b->capi.resize(n);
+    b->cbase.assign(n + 1, 0);
+    int64_t C = 0, segs = 0, idx = 0;
+    for (int i = 0; i < n; ++i) {
+        std::string msg = b->P[i].derive(params[i]);
+        if (!msg.empty()) { delete b; return fail(DSMC_ERR_ARG, "instance " + std::to_string(i) + ": " + msg); }
+        const dsmc_body *bd = (bodies && bodies[i].n_points > 0) ? &bodies[i] : nullptr;
+        msg = build_cut_tables(b->P[i], bd, b->T[i]);
+        if (!msg.empty()) { delete b; return fail(DSMC_ERR_GEOMETRY, "instance " + std::to_string(i) + ": " + msg); }
+        b->capi[i] = default_capacity(b->P[i], b->T[i]);
+        b->cap += b->capi[i];
+        C += b->T[i].ncell;
+        segs += (int64_t)b->T[i].seg.size();
+        idx += (int64_t)b->T[i].cseg_idx.size();
+        b->cbase[i + 1] = (int)std::min<int64_t>(C, 0x7fffffff);
+    }
+    if (C >= (1LL << 30) || b->cap > (int64_t)0x7fffffff - 1) {
+        delete b;
+        return fail(DSMC_ERR_CAPACITY, "batch too large: cells or particle capacity above 2^31");
+    }
+    b->ncell = (int)C;
+    b->ntile_scan = (b->ncell + SCAN_TILE - 1) / SCAN_TILE;
+    auto bad = [&](hipError_t e, const char *what) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        dsmc_batch_destroy(b);
+        return fail(DSMC_ERR_HIP, m);
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&b->device)) != hipSuccess) return bad(e, "hipGetDevice");
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
+    for (auto &ev : b->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return bad(e, "event");
+    for (auto &ev : b->ev_flight)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return bad(e, "event");
+    const size_t N = (size_t)b->cap, Cs = (size_t)C;
+    for (SoA *s : { &b->A, &b->B })
+        if ((e = b->alloc(&s->x, N)) || (e = b->alloc(&s->y, N)) || (e = b->alloc(&s->vx, N)) ||
+            (e = b->alloc(&s->vy, N)) || (e = b->alloc(&s->vz, N)) || (e = b->alloc(&s->pid, N)))
+            return bad(e, "particle buffers");
+    const int ntiles_cell = (int)((N + CELL_P - 1) / CELL_P) + 1;
+    if ((e = b->alloc(&b->cellslot, N)) || (e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
+        (e = b->alloc(&b->lstart, Cs)) || (e = b->alloc(&b->tsum, b->ntile_scan)) ||
+        (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, Cs + 1)) ||
+        (e = b->alloc(&b->tile_first, ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
+        (e = b->alloc(&b->sigcmax, Cs)) || (e = b->alloc(&b->mrem, Cs)) || (e = b->alloc(&b->acc, 6 * Cs)) ||
+        (e = b->alloc(&b->ctrl, 1)) || (e = b->alloc(&b->fill_ctrl, 1)) || (e = b->alloc(&b->ictl, n)) ||
+        (e = b->alloc(&b->d_cases, n)) || (e = b->alloc(&b->d_cc, n)) || (e = b->alloc(&b->d_cbase, n + 1)) ||
+        (e = b->alloc(&b->d_seg, segs)) || (e = b->alloc(&b->d_cseg_start, Cs + n)) ||
+        (e = b->alloc(&b->d_cseg_idx, idx)) || (e = b->alloc(&b->d_vol_ntc, Cs)) || (e = b->alloc(&b->d_cut, Cs)) ||
+        (e = b->alloc(&b->d_fbox, 4 * Cs)) || (e = b->alloc(&b->d_expected, Cs)) || (e = b->alloc(&b->d_ux, Cs)) ||
+        (e = b->alloc(&b->d_uy, Cs)) || (e = b->alloc(&b->d_sig, Cs)))
+        return bad(e, "cell buffers");
+
+    // concatenated tables; every instance's DevCase points at its slices
+    std::vector<CellConst> cc(n);
+    int64_t so = 0, io = 0;
+    for (int i = 0; i < n; ++i) {
+        const CutTables &Ti = b->T[i];
+        const int cb = b->cbase[i];
+        const size_t Ci = Ti.ncell;
+        auto up = [&](void *d, const void *s, size_t bytes) {
+            return bytes ? hipMemcpy(d, s, bytes, hipMemcpyHostToDevice) : hipSuccess;
+        };
+        if ((e = up(b->d_seg + so, Ti.seg.data(), sizeof(float) * Ti.seg.size())) ||
+            (e = up(b->d_cseg_start + cb + i, Ti.cseg_start.data(), sizeof(int) * (Ci + 1))) ||
+            (e = up(b->d_cseg_idx + io, Ti.cseg_idx.data(), sizeof(int) * Ti.cseg_idx.size())) ||
+            (e = up(b->d_vol_ntc + cb, Ti.vol_ntc.data(), sizeof(float) * Ci)) ||
+            (e = up(b->d_cut + cb, Ti.cut.data(), Ci)) ||
+            (e = up(b->d_fbox + 4 * (size_t)cb, Ti.fbox.data(), sizeof(float) * 4 * Ci)))
+            return bad(e, "table upload");
+        DevCase &c = b->cs[i];
+        c = make_devcase(b->P[i], Ti, b->capi[i]);
+        c.seg = b->d_seg + so;
+        c.cseg_start = b->d_cseg_start + cb + i;
+        c.cseg_idx = b->d_cseg_idx + io;
+        c.vol_ntc = b->d_vol_ntc + cb;
+        c.fbox = b->d_fbox + 4 * (size_t)cb;
+        cc[i] = CellConst{ c.ntc_coef, c.K1, c.e_exp, c.seed, b->P[i].in.sample_start, 0 };
+        so += (int64_t)Ti.seg.size();
+        io += (int64_t)Ti.cseg_idx.size();
+    }
+    b->all = DevCase{};
+    b->all.ncell = b->ncell;
+    b->all.vol_ntc = b->d_vol_ntc;
+    if ((e = hipMemcpy(b->d_cases, b->cs.data(), sizeof(DevCase) * n, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(b->d_cc, cc.data(), sizeof(CellConst) * n, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(b->d_cbase, b->cbase.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice)) ||
+        (e = hipMemset(b->ctrl, 0, sizeof(Ctrl))) || (e = hipMemset(b->ictl, 0, sizeof(InstCtrl) * n)))
+        return bad(e, "table upload");
+
+    int cu = 0;
+    if ((e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, b->device)) != hipSuccess)
+        return bad(e, "device attribute");
+    b->grid_move = cu * occupancy_move_batch();
+    b->grid_scatter = cu * occupancy_scatter();
+    b->grid_cell = cu * occupancy_cell_batch();
+    b->grid_fill = cu * 4;
+    int r = reset_cell_state(b);
+    if (r) { std::string m = last_error_slot(); dsmc_batch_destroy(b); return fail(r, m); }
+    *out = b;
+    return DSMC_OK;
+}
+
+int dsmc_batch_case_info(dsmc_batch b, int i, dsmc_case_info *o)
+{
+    if (!valid_index(b, i) || !o) return fail(DSMC_ERR_ARG, "bad instance index or null argument");
+    const Param &P = b->P[i];
+    const CutTables &T = b->T[i];
+    const DevCase &cs = b->cs[i];
+    std::memset(o, 0, sizeof(*o));
+    o->ncell = T.ncell; o->nseg = T.nseg; o->ncut = T.ncut; o->cseg_len = (int)T.cseg_idx.size();
+    o->Fn = P.Fn; o->dt = P.in.tau;
+    o->L1 = cs.L1; o->L2 = cs.L2; o->dx = cs.dx; o->dy = cs.dy;
+    o->ntc_coef = cs.ntc_coef; o->K1 = cs.K1; o->e_exp = cs.e_exp; o->sigcmax0 = (float)P.sigcmax0;
+    o->body_vmp = cs.body_vmp; o->body_sig = cs.body_sig;
+    for (int f = 0; f < 4; ++f) {
+        o->face_vmp[f] = cs.face_vmp[f]; o->face_sig[f] = cs.face_sig[f];
+        o->inj_e[f] = cs.inj_e[f]; o->inj_a[f] = cs.inj_a[f]; o->face_bc[f] = cs.face_bc[f];
+    }
+    o->inj_vmp = cs.inj_vmp; o->inj_sig = cs.inj_sig; o->inj_ux = cs.inj_ux; o->inj_uy = cs.inj_uy;
+    o->margin = cs.margin; o->body_bc = cs.body_bc; o->seed = cs.seed; o->cell_volume = P.Vcell;
+    o->ppc_eff = P.ppc_eff;
+    o->capacity = b->capi[i];
+    return DSMC_OK;
+}
+
+int dsmc_batch_init_freestream(dsmc_batch b)
+{
+    if (!b) return fail(DSMC_ERR_ARG, "null batch");
+    int r = set_device(b);
+    if (r) return r;
+    const size_t C = b->ncell;
+    std::vector<double> e(C);
+    std::vector<float> ux(C), uy(C), sg(C);
+    for (int i = 0; i < b->ninst; ++i)
+        for (int c = b->cbase[i]; c < b->cbase[i + 1]; ++c) {
+            const int lc = c - b->cbase[i];
+            e[c] = b->P[i].ppc_eff * b->T[i].frac[lc];
+            ux[c] = (float)b->P[i].in.V_jet;
+            uy[c] = (float)b->P[i].in.V_y;
+            sg[c] = (float)b->P[i].inj_sig;
+        }
+    return fill(b, e, ux, uy, sg);
+}
+
+int dsmc_batch_init_fields(dsmc_batch b, const double *n, const double *ux, const double *uy, const double *T)
+{
+    if (!b || !n || !ux || !uy || !T) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(b);
+    if (r) return r;
+    const size_t C = b->ncell;
+    std::vector<double> e(C);
+    std::vector<float> fx(C), fy(C), sg(C);
+    for (int i = 0; i < b->ninst; ++i) {
+        const Param &P = b->P[i];
+        for (int c = b->cbase[i]; c < b->cbase[i + 1]; ++c) {
+            if (!(n[c] >= 0.0) || !(T[c] > 0.0) || !std::isfinite(ux[c]) || !std::isfinite(uy[c]))
+                return fail(DSMC_ERR_ARG, "init_fields: n must be >= 0, T > 0, velocities finite");
+            e[c] = P.ppc_eff * b->T[i].frac[c - b->cbase[i]] * (n[c] / P.in.n_inf);
+            fx[c] = (float)ux[c];
+            fy[c] = (float)uy[c];
+            sg[c] = (float)std::sqrt(P.in.boltz * T[c] / P.in.mass);
+        }
+    }
+    return fill(b, e, fx, fy, sg);
+}
+
+int dsmc_batch_run(dsmc_batch b, int nsteps, int graph)
+{
+    if (!b || nsteps < 0) return fail(DSMC_ERR_ARG, "bad argument");
+    int r = set_device(b);
+    if (r) return r;
+    if (!b->timing) {
+        int done = 0;
+        if (graph) {
+            if (!b->gexec && (r = build_graph(b))) return r;
+            constexpr int kFlight = 32;   // at most 2*kFlight replays queued ahead
+            for (; done + b->graph_steps <= nsteps; done += b->graph_steps) {
+                HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+                if (++b->flight_k % kFlight == 0) {
+                    hipEvent_t ev = b->ev_flight[(b->flight_k / kFlight) & 1];
+                    HIPCHK(hipEventSynchronize(ev));
+                    HIPCHK(hipEventRecord(ev, b->stream));
+                }
+            }
+        }
+        for (; done < nsteps; ++done)
+            if ((r = launch_step(b, 15))) return r;
+        return DSMC_OK;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        Ctrl before, after;
+        if ((r = read_ctrl(b, before))) return r;
+        hipStream_t st = b->stream;
+        HIPCHK(hipEventRecord(b->ev[0], st));
+        for (int q = 0; q < 4; ++q) {
+            if ((r = launch_step(b, 1 << q))) return r;
+            HIPCHK(hipEventRecord(b->ev[q + 1], st));
+        }
+        HIPCHK(hipEventSynchronize(b->ev[4]));
+        if ((r = read_ctrl(b, after))) return r;
+        const double C = b->ncell, nl = before.n_live, nm = after.n_move, nn = after.n_live;
+        double by[4];
+        by[0] = 40.0 * nl + 24.0 * (nm - nl) + 12.0 * nm + 4.0 * C;
+        by[1] = 8.0 * C;
+        by[2] = 12.0 * nm;
+        by[3] = 52.0 * nn + 32.0 * after.last_cand + 24.0 * after.last_coll + 16.0 * C;
+        for (int q = 0; q < 4; ++q) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]));
+            b->t_ms[q] += ms;
+            b->t_bytes[q] = by[q];
+        }
+        b->t_launches += 1;
+    }
+    return DSMC_OK;
+}
+
+int dsmc_batch_sync(dsmc_batch b)
+{
+    if (!b) return fail(DSMC_ERR_ARG, "null batch");
+    int r = set_device(b);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return DSMC_OK;
+}
+
+int dsmc_batch_status(dsmc_batch b, int i, dsmc_status *o)
+{
+    if (!b || !o || i < -1 || i >= b->ninst) return fail(DSMC_ERR_ARG, "bad instance index or null argument");
+    int r = set_device(b);
+    if (r) return r;
+    Ctrl c;
+    std::vector<InstCtrl> ic;
+    if ((r = read_ctrl(b, c, &ic))) return r;
+    std::memset(o, 0, sizeof(*o));
+    o->step = c.step;
+    if (i < 0) {
+        o->n_live = c.n_live; o->next_pid = 0; o->err = c.err;
+        o->particle_steps = c.psteps; o->collisions = c.ncoll; o->last_collisions = c.last_coll;
+        o->n_move = c.n_move;
+        for (const InstCtrl &x : ic) { o->err |= x.err; o->nsamples = std::max<uint64_t>(o->nsamples, x.nsamples); }
+        return DSMC_OK;
+    }
+    const InstCtrl &x = ic[i];
+    o->n_live = x.n_live; o->next_pid = x.next_pid; o->err = x.err;
+    o->particle_steps = x.psteps; o->collisions = x.ncoll; o->nsamples = x.nsamples;
+    o->last_collisions = x.last_coll; o->n_move = x.n_move;
+    return DSMC_OK;
+}
+
+int dsmc_batch_download_particles(dsmc_batch b, int i, int64_t cap, float *x, float *y, float *vx, float *vy,
+                                  float *vz, uint32_t *pid, int64_t *n_out)
+{
+    if (!valid_index(b, i)) return fail(DSMC_ERR_ARG, "bad instance index");
+    int r = set_device(b);
+    if (r) return r;
+    Ctrl c;
+    std::vector<InstCtrl> ic;
+    if ((r = read_ctrl(b, c, &ic))) return r;
+    size_t off = 0;
+    for (int k = 0; k < i; ++k) off += ic[k].n_live;
+    const int64_t n = ic[i].n_live;
+    if (n_out) *n_out = n;
+    if (n > cap) return fail(DSMC_ERR_CAPACITY, "output arrays too small");
+    const SoA &S = (c.step & 1u) ? b->B : b->A;
+    const size_t bytes = sizeof(float) * (size_t)n;
+    if (n > 0) {
+        if (x) HIPCHK(hipMemcpyAsync(x, S.x + off, bytes, hipMemcpyDeviceToHost, b->stream));
+        if (y) HIPCHK(hipMemcpyAsync(y, S.y + off, bytes, hipMemcpyDeviceToHost, b->stream));
+        if (vx) HIPCHK(hipMemcpyAsync(vx, S.vx + off, bytes, hipMemcpyDeviceToHost, b->stream));
+        if (vy) HIPCHK(hipMemcpyAsync(vy, S.vy + off, bytes, hipMemcpyDeviceToHost, b->stream));
+        if (vz) HIPCHK(hipMemcpyAsync(vz, S.vz + off, bytes, hipMemcpyDeviceToHost, b->stream));
+        if (pid) HIPCHK(hipMemcpyAsync(pid, S.pid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return DSMC_OK;
+}
+
+int dsmc_batch_cell_state(dsmc_batch b, int i, float *sigcmax, float *mrem, int *ncoll)
+{
+    if (!valid_index(b, i)) return fail(DSMC_ERR_ARG, "bad instance index");
+    int r = set_device(b);
+    if (r) return r;
+    const int cb = b->cbase[i];
+    const size_t Ci = b->cbase[i + 1] - cb;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (sigcmax) HIPCHK(hipMemcpy(sigcmax, b->sigcmax + cb, sizeof(float) * Ci, hipMemcpyDeviceToHost));
+    if (mrem) HIPCHK(hipMemcpy(mrem, b->mrem + cb, sizeof(float) * Ci, hipMemcpyDeviceToHost));
+    if (ncoll) HIPCHK(hipMemcpy(ncoll, b->ncoll_cell + cb, sizeof(int) * Ci, hipMemcpyDeviceToHost));
+    return DSMC_OK;
+}
+
+int dsmc_batch_reset_samples(dsmc_batch b)
+{
+    if (!b) return fail(DSMC_ERR_ARG, "null batch");
+    int r = set_device(b);
+    if (r) return r;
+    Ctrl c;
+    std::vector<InstCtrl> ic;
+    if ((r = read_ctrl(b, c, &ic))) return r;
+    for (InstCtrl &x : ic) x.nsamples = 0;
+    HIPCHK(hipMemsetAsync(b->acc, 0, sizeof(double) * 6 * (size_t)b->ncell, b->stream));
+    HIPCHK(hipMemcpyAsync(b->ictl, ic.data(), sizeof(InstCtrl) * b->ninst, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return DSMC_OK;
+}
+
+int dsmc_batch_download_samples(dsmc_batch b, int i, double *acc, uint64_t *nsamples)
+{
+    if (!valid_index(b, i)) return fail(DSMC_ERR_ARG, "bad instance index");
+    int r = set_device(b);
+    if (r) return r;
+    Ctrl c;
+    std::vector<InstCtrl> ic;
+    if ((r = read_ctrl(b, c, &ic))) return r;
+    if (nsamples) *nsamples = ic[i].nsamples;
+    if (acc) {
+        const int cb = b->cbase[i];
+        const size_t Ci = b->cbase[i + 1] - cb, C = b->ncell;
+        for (int k = 0; k < 6; ++k)
+            HIPCHK(hipMemcpy(acc + k * Ci, b->acc + k * C + cb, sizeof(double) * Ci, hipMemcpyDeviceToHost));
+    }
+    return DSMC_OK;
+}
+
+int dsmc_batch_download_fields(dsmc_batch b, int i, double *n, double *ux, double *uy, double *uz, double *T,
+                               double *p)
+{
+    if (!valid_index(b, i)) return fail(DSMC_ERR_ARG, "bad instance index");
+    const size_t Ci = b->cbase[i + 1] - b->cbase[i];
+    std::vector<double> a(6 * Ci);
+    uint64_t ns = 0;
+    int r = dsmc_batch_download_samples(b, i, a.data(), &ns);
+    if (r) return r;
+    fields_from_sums(b->P[i], b->T[i], a.data(), Ci, ns, n, ux, uy, uz, T, p);
+    return DSMC_OK;
+}
+
+int dsmc_batch_set_timing(dsmc_batch b, int on)
+{
+    if (!b) return fail(DSMC_ERR_ARG, "null batch");
+    b->timing = on ? 1 : 0;
+    for (int q = 0; q < 4; ++q) b->t_ms[q] = b->t_bytes[q] = 0.0;
+    b->t_launches = 0;
+    return DSMC_OK;
+}
+
+int dsmc_batch_kernel_times(dsmc_batch b, double ms[4], double bytes[4], int64_t *launches)
+{
+    if (!b) return fail(DSMC_ERR_ARG, "null batch");
+    for (int q = 0; q < 4; ++q) {
+        if (ms) ms[q] = b->t_launches ? b->t_ms[q] / (double)b->t_launches : 0.0;
+        if (bytes) bytes[q] = b->t_bytes[q];
+    }
+    if (launches) *launches = b->t_launches;
+    return DSMC_OK;
+}
+
+}  // extern "C"

@@ -20,6 +20,8 @@
 //      GPUCell.cu:336-402, as time-averaged accumulators);
 //   5. velocities and cell state written back coalesced.
 // The last block to finish advances the device step control.
+#include <type_traits>
+
 #include "dsmc_kernels.h"
 
 namespace dsmc {
@@ -39,6 +41,7 @@ struct CellShared {
     int nc[CELL_MAXC], M[CELL_MAXC];
     int big[CELL_MAXC], heavy[CELL_MAXC];
     int rstart[CELL_WAVES][64], rdst[CELL_WAVES][64];   // per-wave run tables
+    int inst[CELL_MAXC];             // batched runs: instance of each staged cell
     int nbig, nheavy;
     uint32_t coll, cand;
 };
@@ -215,11 +218,19 @@ __device__ __forceinline__ void wave_small_sort(int *S, int len, int *rtab)
     wave_sync<GLOBAL>();
 }
 
+// Collision constants of one cell: VHS sigma*c_r = K1 (c_r^2)^e_exp, and the
+// Philox key (instance-local cell id, instance seed).
+struct CK {
+    float K1, e_exp;
+    uint32_t seed;
+    int lc;
+};
+
 // Number of NTC candidates of a cell with remainder carry:
 // M = (int)(np(np-1) Fn dt/2 (sigma c)max / V + r),  r' = that - M.
-__device__ __forceinline__ int ntc_count(const DevCase &cs, int np, float vol, float smax, float &mrem)
+__device__ __forceinline__ int ntc_count(float ntc_coef, int np, float vol, float smax, float &mrem)
 {
-    float e = (float)((long long)np * (long long)(np - 1)) * cs.ntc_coef;
+    float e = (float)((long long)np * (long long)(np - 1)) * ntc_coef;
     e = e * smax;
     e = e / vol;
     const float tot = e + mrem;
@@ -230,13 +241,13 @@ __device__ __forceinline__ int ntc_count(const DevCase &cs, int np, float vol, f
 
 // Post-collision velocities: isotropic (VHS) scattering of the relative
 // velocity about the centre of mass (Particle.cpp:84-100, GPUCell.cu:278-310).
-__device__ __forceinline__ void scatter_pair(const DevCase &cs, int c, uint32_t step, int m, uint32_t w3,
+__device__ __forceinline__ void scatter_pair(const CK &k, uint32_t step, int m, uint32_t w3,
                                              float g2, float &v1x, float &v1y, float &v1z, float &v2x,
                                              float &v2y, float &v2z)
 {
     float cosr = 2.0f * u01(w3) - 1.0f;
     float sinr = sqrtf(1.0f - cosr * cosr);
-    uint4 w2 = philox((uint32_t)m, step, SUB_COLLIDE, 1u, (uint32_t)c, cs.seed);
+    uint4 w2 = philox((uint32_t)m, step, SUB_COLLIDE, 1u, (uint32_t)k.lc, k.seed);
     float sp, cp;
     dm_sincos2pi(u01(w2.x), sp, cp);
     float g = sqrtf(g2);
@@ -247,17 +258,17 @@ __device__ __forceinline__ void scatter_pair(const DevCase &cs, int c, uint32_t 
     v1z = mz + 0.5f * rz; v2z = mz - 0.5f * rz;
 }
 
-__device__ __forceinline__ float vhs_sigc(const DevCase &cs, float g2)
+__device__ __forceinline__ float vhs_sigc(const CK &k, float g2)
 {
-    return g2 > 0.0f ? cs.K1 * dm_expf(cs.e_exp * dm_logf(g2)) : 0.0f;
+    return g2 > 0.0f ? k.K1 * dm_expf(k.e_exp * dm_logf(g2)) : 0.0f;
 }
 
 // The serial candidate loop of many cells at once: a wave takes 64/G cells,
 // G lanes per cell, and runs conflict-free rounds of up to G candidates in
 // each cell (the same rule as ntc_rounds, within G-lane segments).  list[]
 // holds the chunk-local cell numbers; results go to sh.sm / sh.nc.
-template <int G, bool GLOBAL>
-__device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *list, int nlist, CellShared &sh,
+template <int G, bool GLOBAL, class KF>
+__device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list, int nlist, CellShared &sh,
                            float *Vx, float *Vy, float *Vz, int wave, int nwaves)
 {
     const int lane = lane_id(), g = lane / G, sub = lane % G;
@@ -270,7 +281,7 @@ __device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *
         const int o = has ? sh.st[i] : 0;
         const int np = has ? sh.st[i + 1] - o : 2;
         const int M = has ? sh.M[i] : 0;
-        const int c = c0 + i;
+        const CK k = kf.key(i, c0 + i);
         float *vx = Vx + o, *vy = Vy + o, *vz = Vz + o;
         float smax = has ? sh.sm[i] : 0.0f;
         int nc = 0, mb = 0;
@@ -281,7 +292,7 @@ __device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *
             int i1 = -1, i2 = -2;
             uint4 w = make_uint4(0u, 0u, 0u, 0u);
             if (act) {
-                w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+                w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)k.lc, k.seed);
                 i1 = mulhi_idx(w.x, np);
                 i2 = mulhi_idx(w.y, np - 1);
                 if (i2 >= i1) i2 = i2 + 1;
@@ -302,7 +313,7 @@ __device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *
                 v2x = vx[i2]; v2y = vy[i2]; v2z = vz[i2];
                 const float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
                 g2 = gx * gx + gy * gy + gz * gz;
-                sc = vhs_sigc(cs, g2);
+                sc = vhs_sigc(k, g2);
             }
             float pm = sc;
 #pragma unroll
@@ -313,7 +324,7 @@ __device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *
             const float sml = fmaxf(smax, pm);
             const bool accd = proc && (u01(w.z) < sc / sml);
             if (accd) {
-                scatter_pair(cs, c, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+                scatter_pair(k, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
                 vx[i1] = v1x; vy[i1] = v1y; vz[i1] = v1z;
                 vx[i2] = v2x; vy[i2] = v2y; vz[i2] = v2z;
             }
@@ -331,7 +342,7 @@ __device__ void ntc_groups(const DevCase &cs, uint32_t step, int c0, const int *
 // next <= 64 candidates sharing no particle with an earlier candidate of the
 // round; (sigma c)max becomes a prefix max.  Identical results.
 template <bool GLOBAL>
-__device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M, float *Vx, float *Vy,
+__device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, float *Vy,
                           float *Vz, float &smax, int *mark)
 {
     const int lane = lane_id();
@@ -349,7 +360,7 @@ __device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M
         int i1 = -1, i2 = -2;
         uint4 w = make_uint4(0u, 0u, 0u, 0u);
         if (act) {
-            w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+            w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)k.lc, k.seed);
             i1 = mulhi_idx(w.x, np);
             i2 = mulhi_idx(w.y, np - 1);
             if (i2 >= i1) i2 = i2 + 1;
@@ -379,7 +390,7 @@ __device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M
             v2x = Vx[i2]; v2y = Vy[i2]; v2z = Vz[i2];
             float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
             g2 = gx * gx + gy * gy + gz * gz;
-            sc = vhs_sigc(cs, g2);
+            sc = vhs_sigc(k, g2);
         }
         float pm = sc;
 #pragma unroll
@@ -390,7 +401,7 @@ __device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M
         const float sml = fmaxf(smax, pm);
         const bool accd = proc && (u01(w.z) < sc / sml);
         if (accd) {
-            scatter_pair(cs, c, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+            scatter_pair(k, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
             Vx[i1] = v1x; Vy[i1] = v1y; Vz[i1] = v1z;
             Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
         }
@@ -405,7 +416,7 @@ __device__ int ntc_rounds(const DevCase &cs, int c, uint32_t step, int np, int M
 }
 
 struct CellArgs {
-    DevCase cs;
+    DevCase cs;              // batched runs: ncell = all cells, vol_ntc = all cells
     SoA A, B;
     const int *start;
     int *perm;
@@ -416,12 +427,76 @@ struct CellArgs {
     Ctrl *ctrl;
     int sample_start;
     int keep_perm;
+    // batched runs only
+    const CellConst *cc;     // ninst
+    const int *cbase;        // ninst + 1: first global cell of each instance
+    int ninst;
+    InstCtrl *ictl;
+};
+
+// Largest i with tab[i] <= c (tab[0] = 0, nondecreasing).
+__device__ __forceinline__ int find_cell_inst(const int *tab, int ninst, int c)
+{
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid] <= c) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Per-cell constants of one case: uniform across the launch.
+struct KSingle {
+    const DevCase &cs;
+    bool sampling;
+    __device__ void prepare(CellShared &, int, int) const {}
+    __device__ CK key(int, int c) const { return CK{ cs.K1, cs.e_exp, cs.seed, c }; }
+    __device__ float ntc_coef(int) const { return cs.ntc_coef; }
+    __device__ bool sample(int) const { return sampling; }
+    __device__ void add(int, uint32_t, uint32_t) const {}
+};
+
+// Batched instances: LDS copies of the instance table and of the first cell
+// of every instance; sums of collisions and candidates per instance.
+struct BatchShared {
+    int cbase[MAX_INST + 1];
+    CellConst cc[MAX_INST];
+    uint32_t icoll[MAX_INST], icand[MAX_INST];
+    int last;
+};
+
+struct KBatch {
+    BatchShared &b;
+    const int *inst;     // instance of each staged cell (CellShared::inst)
+    int ninst;
+    uint32_t step;
+    __device__ void prepare(CellShared &sh, int c0, int nc) const
+    {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) sh.inst[i] = find_cell_inst(b.cbase, ninst, c0 + i);
+    }
+    __device__ CK key(int i, int c) const
+    {
+        const int u = inst[i];
+        return CK{ b.cc[u].K1, b.cc[u].e_exp, b.cc[u].seed, c - b.cbase[u] };
+    }
+    __device__ float ntc_coef(int i) const { return b.cc[inst[i]].ntc_coef; }
+    __device__ bool sample(int i) const
+    {
+        const int s0 = b.cc[inst[i]].sample_start;
+        return s0 >= 0 && step >= (uint32_t)s0;
+    }
+    __device__ void add(int i, uint32_t coll, uint32_t cand) const
+    {
+        if (coll) atomicAdd(&b.icoll[inst[i]], coll);
+        if (cand) atomicAdd(&b.icand[inst[i]], cand);
+    }
 };
 
 // One tile: cells [ca, cb), particles [p0, p0+n).  LDS selects where the
 // permutation and velocities are staged (block LDS, or HBM in place).
-template <bool LDS>
-__device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uint32_t step, bool sampling,
+template <bool LDS, class KF>
+__device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const SoA &nxt, uint32_t step,
                           int ca, int cb, int p0, int n, int *Sp, float *Vx, float *Vy, float *Vz,
                           CellShared &sh, uint32_t &tcoll, uint32_t &tcand)
 {
@@ -507,6 +582,7 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             sh.mr[i] = a.mrem[c0 + i];
             sh.vol[i] = cs.vol_ntc[c0 + i];
         }
+        kf.prepare(sh, c0, nc);
         if (tid == 0) { sh.nheavy = 0; sh.nbig = 0; }
         __syncthreads();
         // candidate counts; cells go to the grouped rounds or, with many
@@ -517,7 +593,7 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             const float vol = sh.vol[i];
             if (k >= 2 && vol > 0.0f) {
                 float mr = sh.mr[i];
-                M = ntc_count(cs, k, vol, sh.sm[i], mr);
+                M = ntc_count(kf.ntc_coef(i), k, vol, sh.sm[i], mr);
                 sh.mr[i] = mr;
                 if (M > NTC_GROUP_MAX) sh.heavy[atomicAdd(&sh.nheavy, 1)] = i;
                 else if (M > 0) sh.big[atomicAdd(&sh.nbig, 1)] = i;
@@ -525,9 +601,10 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             sh.nc[i] = 0;
             sh.M[i] = M;
             tcand += (uint32_t)M;
+            kf.add(i, 0u, (uint32_t)M);
         }
         __syncthreads();
-        ntc_groups<NTC_G, !LDS>(cs, step, c0, sh.big, sh.nbig, sh, Vx, Vy, Vz, wave, nwaves);
+        ntc_groups<NTC_G, !LDS>(kf, step, c0, sh.big, sh.nbig, sh, Vx, Vy, Vz, wave, nwaves);
         if (!LDS) __threadfence();
         __syncthreads();
         const int nh = sh.nheavy;
@@ -536,7 +613,8 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             const int o = sh.st[i], k = sh.st[i + 1] - o;
             float smax = sh.sm[i];
             // the cell's permutation segment is free after the gather: marks
-            const int ncoll = ntc_rounds<!LDS>(cs, c0 + i, step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax, Sp + o);
+            const int ncoll = ntc_rounds<!LDS>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o,
+                                               smax, Sp + o);
             if (lane == 0) { sh.sm[i] = smax; sh.nc[i] = ncoll; }
         }
         if (!LDS) __threadfence();
@@ -548,7 +626,8 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
             a.mrem[c] = sh.mr[i];
             a.ncoll_cell[c] = sh.nc[i];
             tcoll += (uint32_t)sh.nc[i];
-            if (sampling && k > 0) {
+            kf.add(i, (uint32_t)sh.nc[i], 0u);
+            if (k > 0 && kf.sample(i)) {
                 double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
                 for (int q = 0; q < k; ++q) {
                     const double u = (double)Vx[o + q], v = (double)Vy[o + q], w = (double)Vz[o + q];
@@ -577,12 +656,17 @@ __device__ void cell_tile(const CellArgs &a, const SoA &cur, const SoA &nxt, uin
     __syncthreads();
 }
 
+struct NoBatch {};
+
+template <bool BATCH>
 __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
 {
     __shared__ int s_perm[CELL_CAP];
     __shared__ float s_vx[CELL_CAP], s_vy[CELL_CAP], s_vz[CELL_CAP];
     __shared__ CellShared sh;
+    __shared__ typename std::conditional<BATCH, BatchShared, NoBatch>::type bsh;
     Ctrl *ctrl = a.ctrl;
+    const int tid = threadIdx.x;
     const uint32_t step = ctrl->step;
     const SoA cur = (step & 1u) ? a.B : a.A;
     const SoA nxt = (step & 1u) ? a.A : a.B;
@@ -590,11 +674,19 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
     const int ntiles = (n_next + CELL_P - 1) / CELL_P;
     const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
     __shared__ int s_tile;
-    if (threadIdx.x == 0) { sh.coll = 0u; sh.cand = 0u; }
+    if (tid == 0) { sh.coll = 0u; sh.cand = 0u; }
+    if constexpr (BATCH) {
+        for (int t = tid; t <= a.ninst; t += blockDim.x) bsh.cbase[t] = a.cbase[t];
+        for (int t = tid; t < a.ninst; t += blockDim.x) {
+            bsh.cc[t] = a.cc[t];
+            bsh.icoll[t] = 0u;
+            bsh.icand[t] = 0u;
+        }
+    }
     uint32_t tcoll = 0, tcand = 0;
     for (;;) {
         // dynamic tile queue: dense tiles cost more, blocks take the next free one
-        if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u);
+        if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u);
         __syncthreads();
         const int t = s_tile;
         __syncthreads();
@@ -603,40 +695,79 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
         const int cb = (t + 1 < ntiles) ? a.tile_first[t + 1] : a.cs.ncell;
         if (ca >= cb) continue;
         const int p0 = a.start[ca], n = a.start[cb] - p0;
-        if (n <= CELL_CAP)
-            cell_tile<true>(a, cur, nxt, step, sampling, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll,
-                            tcand);
-        else
-            cell_tile<false>(a, cur, nxt, step, sampling, ca, cb, p0, n, a.perm + p0, nxt.vx + p0,
-                             nxt.vy + p0, nxt.vz + p0, sh, tcoll, tcand);
+        auto run = [&](const auto &kf) {
+            if (n <= CELL_CAP)
+                cell_tile<true>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll, tcand);
+            else
+                cell_tile<false>(a, kf, cur, nxt, step, ca, cb, p0, n, a.perm + p0, nxt.vx + p0, nxt.vy + p0,
+                                 nxt.vz + p0, sh, tcoll, tcand);
+        };
+        if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step });
+        else run(KSingle{ a.cs, sampling });
     }
     if (tcoll) atomicAdd(&sh.coll, tcoll);
     if (tcand) atomicAdd(&sh.cand, tcand);
+    if constexpr (BATCH) {
+        __syncthreads();
+        for (int t = tid; t < a.ninst; t += blockDim.x) {
+            if (bsh.icoll[t]) atomicAdd(&a.ictl[t].step_coll, bsh.icoll[t]);
+            if (bsh.icand[t]) atomicAdd(&a.ictl[t].step_cand, bsh.icand[t]);
+        }
+        __threadfence();
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    bool last = false;
+    if (tid == 0) {
         if (sh.coll) atomicAdd(&ctrl->step_coll, sh.coll);
         if (sh.cand) atomicAdd(&ctrl->step_cand, sh.cand);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (atomicAdd(&ctrl->cell_done, 1u) == gridDim.x - 1) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const uint32_t n_live = ctrl->n_live;
-            const uint32_t sc = __hip_atomic_load(&ctrl->step_coll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t sm = __hip_atomic_load(&ctrl->step_cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ctrl->psteps += n_live;
-            ctrl->next_pid += ctrl->n_move - n_live;
-            ctrl->n_live = (uint32_t)n_next;
-            ctrl->last_coll = sc;
-            ctrl->ncoll += sc;
-            ctrl->last_cand = sm;
-            ctrl->ncand += sm;
-            ctrl->step_coll = 0u;
-            ctrl->step_cand = 0u;
-            if (sampling) ctrl->nsamples += 1ull;
-            ctrl->step = step + 1u;
-            ctrl->tile_next = 0u;
-            ctrl->cell_done = 0u;
-        }
+        last = atomicAdd(&ctrl->cell_done, 1u) == gridDim.x - 1;
+        if constexpr (BATCH) bsh.last = last ? 1 : 0;
     }
+    if constexpr (BATCH) {
+        // the last block closes every instance's step
+        __syncthreads();
+        if (!bsh.last) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        for (int t = tid; t < a.ninst; t += blockDim.x) {
+            InstCtrl &ic = a.ictl[t];
+            const uint32_t nl = ic.n_live;
+            const uint32_t nn = (uint32_t)(a.start[bsh.cbase[t + 1]] - a.start[bsh.cbase[t]]);
+            const uint32_t sc = __hip_atomic_load(&ic.step_coll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t sm = __hip_atomic_load(&ic.step_cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int s0 = bsh.cc[t].sample_start;
+            ic.psteps += nl;
+            ic.next_pid += ic.n_move - nl;
+            ic.n_live = nn;
+            ic.last_coll = sc;
+            ic.ncoll += sc;
+            ic.last_cand = sm;
+            ic.ncand += sm;
+            ic.step_coll = 0u;
+            ic.step_cand = 0u;
+            if (s0 >= 0 && step >= (uint32_t)s0) ic.nsamples += 1ull;
+        }
+        if (tid != 0) return;
+    } else {
+        if (!last) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    const uint32_t n_live = ctrl->n_live;
+    const uint32_t sc = __hip_atomic_load(&ctrl->step_coll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t sm = __hip_atomic_load(&ctrl->step_cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ctrl->psteps += n_live;
+    ctrl->next_pid += ctrl->n_move - n_live;
+    ctrl->n_live = (uint32_t)n_next;
+    ctrl->last_coll = sc;
+    ctrl->ncoll += sc;
+    ctrl->last_cand = sm;
+    ctrl->ncand += sm;
+    ctrl->step_coll = 0u;
+    ctrl->step_cand = 0u;
+    if (sampling) ctrl->nsamples += 1ull;
+    ctrl->step = step + 1u;
+    ctrl->tile_next = 0u;
+    ctrl->cell_done = 0u;
 }
 
 }  // namespace
@@ -647,15 +778,37 @@ hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int 
                        hipStream_t st)
 {
     CellArgs a{ cs, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
-                keep_perm };
-    hipLaunchKernelGGL(k_cell, dim3(grid), dim3(256), 0, st, a);
+                keep_perm, nullptr, nullptr, 0, nullptr };
+    hipLaunchKernelGGL(k_cell<false>, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
+                             InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
+                             const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
+                             Ctrl *ctrl, int grid, hipStream_t st)
+{
+    if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
+    CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, 0, cc, cbase, ninst,
+                ictl };
+    hipLaunchKernelGGL(k_cell<true>, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 int occupancy_cell()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell, 256, 0) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell<false>, 256, 0) != hipSuccess ||
+        nb < 1)
+        nb = 1;
+    return nb;
+}
+
+int occupancy_cell_batch()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell<true>, 256, 0) != hipSuccess ||
+        nb < 1)
         nb = 1;
     return nb;
 }

@@ -57,6 +57,23 @@ struct Ctrl {
     unsigned long long psteps, ncoll, nsamples, ncand;
 };
 
+// Batched runs: every instance keeps its own counters (the global Ctrl holds
+// the totals that drive the shared scan, scatter and tile queue).
+constexpr int MAX_INST = 256;
+struct InstCtrl {
+    uint32_t n_live, n_move, next_pid, err;
+    uint32_t step_coll, step_cand, last_coll, last_cand;
+    unsigned long long psteps, ncoll, nsamples, ncand;
+};
+
+// What k_cell needs of an instance, per cell.
+struct CellConst {
+    float ntc_coef, K1, e_exp;
+    uint32_t seed;
+    int sample_start;   // < 0: never
+    int pad;
+};
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11), counter-based: key = (id, seed),
 // counter = (draw, step, substream, extra).

@@ -15,6 +15,10 @@ constexpr int CELL_WAVES = 4;                // waves per k_cell block
 
 hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, int2 *cellslot, int *count,
                        Ctrl *ctrl, int grid, hipStream_t st);
+// Many instances in one launch: cases[ninst] and cbase[ninst+1] in device
+// memory, per-instance counters in ictl[ninst], totals in ctrl.
+hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, const SoA &A, const SoA &B,
+                             int2 *cellslot, int *count, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st);
 hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell, Ctrl *ctrl,
                        hipStream_t st);
 hipError_t launch_scatter(const int2 *cellslot, const int *lstart, const int *toff, int *start,
@@ -24,6 +28,10 @@ hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int 
                        const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
                        double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int grid,
                        hipStream_t st);
+hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
+                             InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
+                             const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
+                             Ctrl *ctrl, int grid, hipStream_t st);
 hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, const uint8_t *cut,
                        const float *ux, const float *uy, const float *sig, int *count, int *lstart,
                        int *tsum, int *toff, Ctrl *ctrl, int grid, hipStream_t st);
@@ -31,7 +39,9 @@ hipError_t launch_ctrl_reset(Ctrl *ctrl, uint32_t n, uint32_t next_pid, hipStrea
 
 // Resident blocks per CU of the persistent kernels (occupancy query).
 int occupancy_move();
+int occupancy_move_batch();
 int occupancy_cell();
+int occupancy_cell_batch();
 int occupancy_scatter();
 
 }  // namespace dsmc

@@ -330,6 +330,100 @@ __global__ void k_ctrl_reset(Ctrl *ctrl, int use_n_next, uint32_t n, uint32_t ne
 }
 
 // ---------------------------------------------------------------------------
+// k_move_batch: k_move for many independent instances in one launch.  Live
+// particles are ordered by instance (instance i at [P_i, P_i + n_i) with P
+// the prefix of the live counts); inflow particles follow all live ones in
+// instance order (instance i from N_l + Q_i, Q the prefix of the clamped
+// inflow counts).  Every block rebuilds both offset tables in LDS; a particle
+// finds its instance by binary search and is moved with that instance's
+// constants, one instance at a time per wave (wave-uniform loads).  Within
+// an instance the work is exactly that of k_move, so each instance evolves
+// bit-identically to a run of its own.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_inst(const uint32_t *tab, int ninst, uint32_t key)
+{
+    int lo = 0, hi = ninst - 1;   // largest i with tab[i] <= key
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid] <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ cases, int ninst,
+                                                    const int *__restrict__ cbase, SoA A, SoA B,
+                                                    int2 *__restrict__ cellslot, int *__restrict__ count,
+                                                    Ctrl *__restrict__ ctrl, InstCtrl *__restrict__ ictl)
+{
+    __shared__ uint32_t s_p[MAX_INST + 1], s_q[MAX_INST + 1];
+    __shared__ int s_w[4];
+    const int t = threadIdx.x;
+    const uint32_t step = ctrl->step;
+    const SoA S = (step & 1u) ? B : A;
+    uint32_t nl = 0, jt = 0;
+    bool clamped = false;
+    if (t < ninst) {
+        nl = ictl[t].n_live;
+        int J[4];
+        inject_counts(cases[t], step, J);
+        jt = (uint32_t)(J[0] + J[1] + J[2] + J[3]);
+        const uint32_t cap = cases[t].cap;
+        if (nl + jt > cap) { jt = cap > nl ? cap - nl : 0u; clamped = true; }
+    }
+    int ex_p = 0, ex_q = 0;
+    const int tot_p = block_excl_scan((int)nl, ex_p, s_w);
+    const int tot_q = block_excl_scan((int)jt, ex_q, s_w);
+    if (t < ninst) { s_p[t] = (uint32_t)ex_p; s_q[t] = (uint32_t)ex_q; }
+    if (t == 0) { s_p[ninst] = (uint32_t)tot_p; s_q[ninst] = (uint32_t)tot_q; }
+    if (blockIdx.x == 0) {
+        if (t < ninst) {
+            ictl[t].n_move = nl + jt;
+            if (clamped) atomicOr(&ictl[t].err, (uint32_t)ERR_CAPACITY);
+        }
+        if (t == 0) ctrl->n_move = (uint32_t)(tot_p + tot_q);
+    }
+    __syncthreads();
+    const uint32_t n_live = (uint32_t)tot_p, ntot = n_live + (uint32_t)tot_q;
+    uint32_t err = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < ntot; i += stride) {
+        const bool inflow = i >= n_live;
+        const uint32_t key = inflow ? i - n_live : i;
+        const int inst = find_inst(inflow ? s_q : s_p, ninst, key);
+        int c = -1;
+        for (bool todo = true; todo;) {
+            const int u = __builtin_amdgcn_readfirstlane(inst);
+            if (inst != u) continue;
+            todo = false;
+            const DevCase &cs = cases[u];
+            float x, y, vx, vy, vz;
+            bool dead;
+            if (!inflow) {
+                x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
+                bool vchg = false;
+                dead = move_one(cs, x, y, vx, vy, vz, S.pid + i, 0u, false, step, cs.dt, err, vchg);
+                S.x[i] = x; S.y[i] = y;
+                if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
+            } else {
+                const uint32_t q = key - s_q[u];
+                int J[4];
+                inject_counts(cs, step, J);
+                int f = 0;
+                uint32_t acc = (uint32_t)J[0];
+                while (f < 3 && q >= acc) { ++f; acc += (uint32_t)J[f]; }
+                const uint32_t pid = ictl[u].next_pid + q;
+                dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, err);
+                S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
+            }
+            c = dead ? -1 : cbase[u] + cell_of(cs, x, y);
+            if (err) { atomicOr(&ictl[u].err, err); err = 0; }
+        }
+        cellslot[i] = make_int2(c, agg_slot(count, c));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, int2 *cellslot, int *count,
@@ -373,6 +467,15 @@ hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, 
     return hipGetLastError();
 }
 
+hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, const SoA &A, const SoA &B,
+                             int2 *cellslot, int *count, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st)
+{
+    if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_move_batch, dim3(grid), dim3(256), 0, st, cases, ninst, cbase, A, B, cellslot, count,
+                       ctrl, ictl);
+    return hipGetLastError();
+}
+
 hipError_t launch_ctrl_reset(Ctrl *ctrl, uint32_t n, uint32_t next_pid, hipStream_t st)
 {
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, ctrl, 0, n, next_pid);
@@ -390,6 +493,7 @@ static int occ(const void *fn)
     return nb;
 }
 int occupancy_move() { return occ((const void *)k_move); }
+int occupancy_move_batch() { return occ((const void *)k_move_batch); }
 int occupancy_scatter() { return occ((const void *)k_scatter); }
 
 }  // namespace dsmc

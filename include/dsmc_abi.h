@@ -202,6 +202,39 @@ int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,
 int dsmc_set_timing(dsmc_handle h, int on);
 int dsmc_kernel_times(dsmc_handle h, double ms[4], double bytes[4], int64_t *launches);
 
+/* ---- Batched instances ------------------------------------------------
+ * No counterpart in the reference, which runs one case per process.  n
+ * independent cases (own grid, body, gas state, walls, seed, sampling start;
+ * uneven particle and cell counts) advance together: every phase of a step
+ * is one launch over all instances, driven by per-instance offset tables,
+ * and dsmc_batch_run replays the whole step as a hipGraph.  Each instance
+ * evolves bit-identically to a dsmc_handle of its own.
+ *   bodies: n entries (n_points = 0 for none), or NULL for no bodies;
+ *   init_fields: per-cell n, ux, uy, T of all instances concatenated in
+ *     instance order (sum of ncell values);
+ *   status / downloads take the instance index i; dsmc_batch_status with
+ *     i = -1 gives the totals (particle_steps, collisions, n_live). */
+typedef struct dsmc_batch_s *dsmc_batch;
+int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n, dsmc_batch *out);
+int dsmc_batch_destroy(dsmc_batch b);
+int dsmc_batch_case_info(dsmc_batch b, int i, dsmc_case_info *out);
+int dsmc_batch_init_freestream(dsmc_batch b);
+int dsmc_batch_init_fields(dsmc_batch b, const double *n, const double *ux, const double *uy,
+                           const double *T);
+int dsmc_batch_run(dsmc_batch b, int nsteps, int graph);
+int dsmc_batch_sync(dsmc_batch b);
+int dsmc_batch_status(dsmc_batch b, int i, dsmc_status *out);
+int dsmc_batch_download_particles(dsmc_batch b, int i, int64_t cap, float *x, float *y, float *vx,
+                                  float *vy, float *vz, uint32_t *pid, int64_t *n_out);
+int dsmc_batch_cell_state(dsmc_batch b, int i, float *sigcmax, float *mrem, int *ncoll);
+int dsmc_batch_reset_samples(dsmc_batch b);
+int dsmc_batch_download_samples(dsmc_batch b, int i, double *acc, uint64_t *nsamples);
+int dsmc_batch_download_fields(dsmc_batch b, int i, double *n, double *ux, double *uy, double *uz,
+                               double *T, double *p);
+/* Per-kernel timing of batched steps (eager), as dsmc_set_timing. */
+int dsmc_batch_set_timing(dsmc_batch b, int on);
+int dsmc_batch_kernel_times(dsmc_batch b, double ms[4], double bytes[4], int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif
