@@ -71,11 +71,18 @@ def main():
     ap.add_argument("--eager", action="store_true", help="launch steps eagerly instead of hipGraph replay")
     ap.add_argument("--save-state", default=None,
                     help="write a checkpoint (.npz) of the state the timing window starts from")
+    ap.add_argument("--workload", default="cylinder_mach10_400x200",
+                    choices=["cylinder_mach10_400x200", "sweep64"],
+                    help="sweep64: the 64-instance batched sweep (1000 timed steps unless --steps)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the launch-settings autotuner in the untimed warm-up (always on for sweep64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
               "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sweep64":
+        return bench_sweep(args)
 
     from cudadsmc_amd import cases
     from cudadsmc_amd.run import Run
@@ -89,6 +96,11 @@ def main():
     graph = not args.eager
     run.solver(args.warmup, graph=graph)
     run.sync()
+    tuned = None
+    if args.tune:
+        from cudadsmc_amd import autotune
+        tuned, _ = autotune.tune(autotune.RunTarget(run), autotune.device_cus(), steps=10,
+                                 log=lambda m: print(m, file=sys.stderr, flush=True))
     s0 = run.status()
     t0 = time.perf_counter()
     done = 0
@@ -155,6 +167,8 @@ def main():
             "steps_total": args.warmup + args.steps,
             "sample_from_step": cfg["sample_from"],
             "launch": "eager" if args.eager else "hipgraph",
+            "launch_settings": run.get_launch(),
+            "autotuned": tuned is not None,
             "collisions": s1.collisions - s0.collisions,
             "seed": int(p.seed),
         },
@@ -163,6 +177,82 @@ def main():
     }
     print(json.dumps(out))
     run.close()
+
+
+def bench_sweep(args):
+    """The batched sweep: 64 uneven instances (cylinders and wedges, Mach
+    3-12, Kn 0.005-0.1, T_w/T_inf 1-5, 64x32 ... 256x128 cells, about 110M
+    particles) advanced together; launch settings autotuned in the warm-up;
+    value = particle-steps of all instances per second of the timed steps."""
+    from cudadsmc_amd import autotune, cases
+    from cudadsmc_amd.batch import Batch
+
+    steps = args.steps if args.steps != 19990 else 1000
+    sw = cases.sweep(64)
+    b = Batch([c["params"] for c in sw], [c["body"] for c in sw])
+    b.initialize()
+    n_init = b.status(-1).n_live
+    print(f"[bench] sweep64: {len(sw)} instances, {n_init} particles, {int(b.cbase[-1])} cells",
+          file=sys.stderr, flush=True)
+    tuned, _ = autotune.tune(autotune.BatchTarget(b, lambda s: s.initialize(), warm_steps=5),
+                             autotune.device_cus(), steps=6, graph_steps_measure=32,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    b.initialize()
+    b.solver(args.warmup, graph=not args.eager)
+    b.sync()
+    s0 = b.status(-1)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(250, steps - done)
+        b.solver(k, graph=not args.eager)
+        done += k
+        print(f"[bench] {done}/{steps} steps queued", file=sys.stderr, flush=True)
+    b.sync()
+    elapsed = time.perf_counter() - t0
+    s1 = b.status(-1)
+    if s1.err:
+        print(f"warning: device error bits {s1.err:#x}", file=sys.stderr)
+    b.set_timing(True)
+    b.solver(args.timing_steps // 2 or 1, graph=False)
+    kt, _ = b.kernel_times()
+    b.set_timing(False)
+    dom = max(kt, key=lambda k: kt[k]["ms"])
+    achieved = kt[dom]["bytes"] / (kt[dom]["ms"] * 1e-3) / 1e9
+    psteps = s1.particle_steps - s0.particle_steps
+    out = {
+        "metric": METRIC,
+        "value": psteps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "sweep64_batched",
+            "instances": len(sw),
+            "cells": int(b.cbase[-1]),
+            "particles_initial": n_init,
+            "particles_final": s1.n_live,
+            "launch": "eager" if args.eager else "hipgraph",
+            "launch_settings": tuned,
+            "autotuned": True,
+            "seed": 20251203,
+        },
+        "roofline": dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=achieved / HBM_PEAK_GBS, traffic=None,
+                         per_kernel={k: dict(ms=v["ms"], alg_bytes=v["bytes"],
+                                             gbs=v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0)
+                                     for k, v in kt.items()}),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    b.close()
 
 
 if __name__ == "__main__":

@@ -63,6 +63,14 @@ class dsmc_status(C.Structure):
     ]
 
 
+class dsmc_launch(C.Structure):
+    _fields_ = [("move_blocks", C.c_int), ("scatter_blocks", C.c_int), ("cell_blocks", C.c_int),
+                ("cell_tile", C.c_int), ("graph_steps", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = C.POINTER
 _f32p = _P(C.c_float)
 _f64p = _P(C.c_double)
@@ -74,6 +82,7 @@ _h = C.c_void_p
 # name -> argtypes; every function returns int status except dsmc_last_error
 SIGNATURES = {
     "dsmc_device_count": [_i32p],
+    "dsmc_device_cus": [_i32p],
     "dsmc_create": [_P(dsmc_params), _P(dsmc_body), _P(C.c_void_p)],
     "dsmc_destroy": [_h],
     "dsmc_case_info_get": [_h, _P(dsmc_case_info)],
@@ -118,6 +127,10 @@ SIGNATURES = {
     "dsmc_batch_download_fields": [_h, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],
     "dsmc_batch_set_timing": [_h, C.c_int],
     "dsmc_batch_kernel_times": [_h, _f64p, _f64p, _P(C.c_int64)],
+    "dsmc_get_launch": [_h, _P(dsmc_launch)],
+    "dsmc_set_launch": [_h, _P(dsmc_launch)],
+    "dsmc_batch_get_launch": [_h, _P(dsmc_launch)],
+    "dsmc_batch_set_launch": [_h, _P(dsmc_launch)],
     "dsmc_set_timing": [_h, C.c_int],
     "dsmc_kernel_times": [_h, _f64p, _f64p, _P(C.c_int64)],
 }

@@ -119,6 +119,21 @@ class Batch:
         shape = (self.params[i].N1, self.params[i].N2)
         return {k: v.reshape(shape) for k, v in zip(("n", "ux", "uy", "uz", "T", "p"), out)}
 
+    # -- launch settings (results never depend on them) --
+    def get_launch(self):
+        L = abi.dsmc_launch()
+        check(self.lib.dsmc_batch_get_launch(self.h, C.byref(L)))
+        return L.as_dict()
+
+    def set_launch(self, **kw):
+        L = abi.dsmc_launch()
+        check(self.lib.dsmc_batch_get_launch(self.h, C.byref(L)))
+        for k, v in kw.items():
+            if not hasattr(L, k):
+                raise KeyError(k)
+            setattr(L, k, int(v))
+        check(self.lib.dsmc_batch_set_launch(self.h, C.byref(L)))
+
     # -- per-kernel timing --
     def set_timing(self, on=True):
         check(self.lib.dsmc_batch_set_timing(self.h, 1 if on else 0))

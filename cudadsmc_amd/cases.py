@@ -138,11 +138,14 @@ def flow_case(geometry="cylinder", mach=10.0, kn=0.01, tw_ratio=2.5, nx=400, ny=
 SWEEP_GRIDS = [(64, 32), (96, 48), (128, 64), (160, 80), (192, 96), (224, 112), (256, 128)]
 
 
-def sweep(n=64, seed=20251203, particles=(2e5, 6e6), steps=20000, sample_from=10000, n_points=128):
+def sweep(n=64, seed=20251203, particles=(2e5, 6e6), ppc=(30.0, 200.0), steps=20000, sample_from=10000,
+          n_points=128):
     """The batched parameter sweep, every value drawn from `seed`: geometry
     cylinder or wedge (half-angle U[10, 30] deg), Mach U[3, 12], Kn
-    log-U[0.005, 0.1], T_w/T_inf U[1, 5], grid 64x32 ... 256x128, particle
-    count log-U over `particles` (uneven instances, about 1.7M on average)."""
+    log-U[0.005, 0.1], T_w/T_inf U[1, 5], particle count log-U over
+    `particles` (uneven instances, about 1.7M on average), and the grid of
+    64x32 ... 256x128 closest to a particles-per-cell target log-U over
+    `ppc`, so that cells stay DSMC-sized whatever the particle count."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -151,8 +154,9 @@ def sweep(n=64, seed=20251203, particles=(2e5, 6e6), steps=20000, sample_from=10
         mach = float(rng.uniform(3.0, 12.0))
         kn = float(math.exp(rng.uniform(math.log(0.005), math.log(0.1))))
         tw = float(rng.uniform(1.0, 5.0))
-        nx, ny = SWEEP_GRIDS[int(rng.integers(len(SWEEP_GRIDS)))]
         npart = float(math.exp(rng.uniform(math.log(particles[0]), math.log(particles[1]))))
+        target = npart / float(math.exp(rng.uniform(math.log(ppc[0]), math.log(ppc[1]))))
+        nx, ny = min(SWEEP_GRIDS, key=lambda g: abs(math.log(g[0] * g[1] / target)))
         solid = 0.3 * 0.3 * math.tan(math.radians(half)) if geom == "wedge" else math.pi * 0.01
         fluid_cells = nx * ny * (1.0 - solid / 0.18)
         cfg = flow_case(geom, mach, kn, tw, nx, ny, ppc=npart / fluid_cells, half_angle_deg=half,

@@ -51,6 +51,7 @@ struct dsmc_handle_s {
     float *d_fbox = nullptr;
     double *d_expected = nullptr;
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
+    int tile_p = CELL_P, graph_len = 8;
     int keep_perm = 0, timing = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -86,11 +87,12 @@ int launch_step(dsmc_handle h, int phase_mask)
     if (phase_mask & 2) {
         HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
         HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
-                              h->ctrl, h->grid_scatter, st));
+                              h->tile_p, h->ctrl, h->grid_scatter, st));
     }
     if (phase_mask & 4)
         HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->tile_first, h->sigcmax, h->mrem,
-                           h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->keep_perm, h->grid_cell, st));
+                           h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->keep_perm, h->tile_p,
+                           h->grid_cell, st));
     return DSMC_OK;
 }
 
@@ -189,6 +191,16 @@ int dsmc_device_count(int *n)
     return DSMC_OK;
 }
 
+int dsmc_device_cus(int *n)
+{
+    if (!n) return fail(DSMC_ERR_ARG, "null argument");
+    int dev = 0, cu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    *n = cu;
+    return DSMC_OK;
+}
+
 int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
 {
     if (!p || !out) return fail(DSMC_ERR_ARG, "null argument");
@@ -227,7 +239,7 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
             (e = h->alloc(&s->vy, N)) || (e = h->alloc(&s->vz, N)) || (e = h->alloc(&s->pid, N)))
             return bad(e, "particle buffers");
     }
-    const int ntiles_cell = (int)((N + CELL_P - 1) / CELL_P) + 1;
+    const int ntiles_cell = (int)((N + CELL_P_MIN - 1) / CELL_P_MIN) + 1;
     if ((e = h->alloc(&h->cellslot, N)) || (e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
         (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||
         (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, C + 1)) ||
@@ -445,21 +457,19 @@ int dsmc_collide(dsmc_handle h)
 
 namespace {
 
-constexpr int kGraphSteps = 8;
-
 int build_graph(dsmc_handle h)
 {
     drop_graph(h);
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int r = DSMC_OK;
-    for (int s = 0; s < kGraphSteps && r == DSMC_OK; ++s) r = launch_step(h, 7);
+    for (int s = 0; s < h->graph_len && r == DSMC_OK; ++s) r = launch_step(h, 7);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (r) { if (g) (void)hipGraphDestroy(g); return r; }
     if (e != hipSuccess) return fail(DSMC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     h->graph = g;
     HIPCHK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
-    h->graph_steps = kGraphSteps;
+    h->graph_steps = h->graph_len;
     return DSMC_OK;
 }
 
@@ -521,7 +531,7 @@ int dsmc_run(dsmc_handle h, int nsteps, int graph)
         HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
         HIPCHK(hipEventRecord(h->ev[2], st));
         HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
-                              h->ctrl, h->grid_scatter, st));
+                              h->tile_p, h->ctrl, h->grid_scatter, st));
         HIPCHK(hipEventRecord(h->ev[3], st));
         if ((r = launch_step(h, 4))) return r;
         HIPCHK(hipEventRecord(h->ev[4], st));
@@ -697,6 +707,31 @@ int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy, doubl
     int r = dsmc_download_samples(h, a.data(), &ns);
     if (r) return r;
     fields_from_sums(h->P, h->T, a.data(), C, ns, n, ux, uy, uz, T, p);
+    return DSMC_OK;
+}
+
+int dsmc_get_launch(dsmc_handle h, dsmc_launch *o)
+{
+    if (!h || !o) return fail(DSMC_ERR_ARG, "null argument");
+    o->move_blocks = h->grid_move;
+    o->scatter_blocks = h->grid_scatter;
+    o->cell_blocks = h->grid_cell;
+    o->cell_tile = h->tile_p;
+    o->graph_steps = h->graph_len;
+    return DSMC_OK;
+}
+
+int dsmc_set_launch(dsmc_handle h, const dsmc_launch *c)
+{
+    if (!h || !c) return fail(DSMC_ERR_ARG, "null argument");
+    int r = check_launch(c);
+    if (r) return r;
+    h->grid_move = c->move_blocks;
+    h->grid_scatter = c->scatter_blocks;
+    h->grid_cell = c->cell_blocks;
+    h->tile_p = c->cell_tile;
+    h->graph_len = c->graph_steps;
+    drop_graph(h);
     return DSMC_OK;
 }
 

@@ -49,6 +49,7 @@ struct dsmc_batch_s {
     double *d_expected = nullptr;
     DevCase all{};                     // what k_cell reads of "the case": all cells
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
+    int tile_p = CELL_P, graph_len = 8;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
@@ -71,8 +72,6 @@ struct dsmc_batch_s {
 
 namespace {
 
-constexpr int kGraphSteps = 8;
-
 int set_device(dsmc_batch b)
 {
     HIPCHK(hipSetDevice(b->device));
@@ -87,12 +86,12 @@ int launch_step(dsmc_batch b, int mask)
                                  b->ictl, b->grid_move, st));
     if (mask & 2) HIPCHK(launch_scan(b->count, b->lstart, b->tsum, b->toff, b->ncell, b->ctrl, st));
     if (mask & 4)
-        HIPCHK(launch_scatter(b->cellslot, b->lstart, b->toff, b->start, b->perm, b->tile_first, b->ncell, b->ctrl,
-                              b->grid_scatter, st));
+        HIPCHK(launch_scatter(b->cellslot, b->lstart, b->toff, b->start, b->perm, b->tile_first, b->ncell,
+                              b->tile_p, b->ctrl, b->grid_scatter, st));
     if (mask & 8)
         HIPCHK(launch_cell_batch(b->all, b->d_cc, b->d_cbase, b->ninst, b->ictl, b->A, b->B, b->start, b->perm,
-                                 b->tile_first, b->sigcmax, b->mrem, b->ncoll_cell, b->acc, b->ctrl, b->grid_cell,
-                                 st));
+                                 b->tile_first, b->sigcmax, b->mrem, b->ncoll_cell, b->acc, b->ctrl, b->tile_p,
+                                 b->grid_cell, st));
     return DSMC_OK;
 }
 
@@ -121,14 +120,14 @@ int build_graph(dsmc_batch b)
     drop_graph(b);
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int r = DSMC_OK;
-    for (int s = 0; s < kGraphSteps && r == DSMC_OK; ++s) r = launch_step(b, 15);
+    for (int s = 0; s < b->graph_len && r == DSMC_OK; ++s) r = launch_step(b, 15);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (r) { if (g) (void)hipGraphDestroy(g); return r; }
     if (e != hipSuccess) return fail(DSMC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     b->graph = g;
     HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-    b->graph_steps = kGraphSteps;
+    b->graph_steps = b->graph_len;
     return DSMC_OK;
 }
 
@@ -268,7 +267,7 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
         if ((e = b->alloc(&s->x, N)) || (e = b->alloc(&s->y, N)) || (e = b->alloc(&s->vx, N)) ||
             (e = b->alloc(&s->vy, N)) || (e = b->alloc(&s->vz, N)) || (e = b->alloc(&s->pid, N)))
             return bad(e, "particle buffers");
-    const int ntiles_cell = (int)((N + CELL_P - 1) / CELL_P) + 1;
+    const int ntiles_cell = (int)((N + CELL_P_MIN - 1) / CELL_P_MIN) + 1;
     if ((e = b->alloc(&b->cellslot, N)) || (e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
         (e = b->alloc(&b->lstart, Cs)) || (e = b->alloc(&b->tsum, b->ntile_scan)) ||
         (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, Cs + 1)) ||
@@ -565,6 +564,31 @@ int dsmc_batch_download_fields(dsmc_batch b, int i, double *n, double *ux, doubl
     int r = dsmc_batch_download_samples(b, i, a.data(), &ns);
     if (r) return r;
     fields_from_sums(b->P[i], b->T[i], a.data(), Ci, ns, n, ux, uy, uz, T, p);
+    return DSMC_OK;
+}
+
+int dsmc_batch_get_launch(dsmc_batch b, dsmc_launch *o)
+{
+    if (!b || !o) return fail(DSMC_ERR_ARG, "null argument");
+    o->move_blocks = b->grid_move;
+    o->scatter_blocks = b->grid_scatter;
+    o->cell_blocks = b->grid_cell;
+    o->cell_tile = b->tile_p;
+    o->graph_steps = b->graph_len;
+    return DSMC_OK;
+}
+
+int dsmc_batch_set_launch(dsmc_batch b, const dsmc_launch *c)
+{
+    if (!b || !c) return fail(DSMC_ERR_ARG, "null argument");
+    int r = check_launch(c);
+    if (r) return r;
+    b->grid_move = c->move_blocks;
+    b->grid_scatter = c->scatter_blocks;
+    b->grid_cell = c->cell_blocks;
+    b->tile_p = c->cell_tile;
+    b->graph_len = c->graph_steps;
+    drop_graph(b);
     return DSMC_OK;
 }
 

@@ -432,6 +432,7 @@ struct CellArgs {
     const int *cbase;        // ninst + 1: first global cell of each instance
     int ninst;
     InstCtrl *ictl;
+    int tile_p;              // particles per tile (the scatter's tile boundaries)
 };
 
 // Largest i with tab[i] <= c (tab[0] = 0, nondecreasing).
@@ -671,7 +672,7 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
     const SoA cur = (step & 1u) ? a.B : a.A;
     const SoA nxt = (step & 1u) ? a.A : a.B;
     const int n_next = (int)ctrl->n_next;
-    const int ntiles = (n_next + CELL_P - 1) / CELL_P;
+    const int ntiles = (n_next + a.tile_p - 1) / a.tile_p;
     const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
     __shared__ int s_tile;
     if (tid == 0) { sh.coll = 0u; sh.cand = 0u; }
@@ -774,11 +775,12 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
 
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
                        const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
-                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int grid,
+                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int tile_p, int grid,
                        hipStream_t st)
 {
+    if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     CellArgs a{ cs, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
-                keep_perm, nullptr, nullptr, 0, nullptr };
+                keep_perm, nullptr, nullptr, 0, nullptr, tile_p };
     hipLaunchKernelGGL(k_cell<false>, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
@@ -786,11 +788,12 @@ hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int 
 hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
                              InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
                              const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
-                             Ctrl *ctrl, int grid, hipStream_t st)
+                             Ctrl *ctrl, int tile_p, int grid, hipStream_t st)
 {
     if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
+    if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, 0, cc, cbase, ninst,
-                ictl };
+                ictl, tile_p };
     hipLaunchKernelGGL(k_cell<true>, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -8,7 +8,8 @@ namespace dsmc {
 
 constexpr int SCAN_IPT = 8;                  // cells per thread in k_scan
 constexpr int SCAN_TILE = 256 * SCAN_IPT;    // cells per k_scan block
-constexpr int CELL_P = 1024;                 // particles per k_cell tile (nominal)
+constexpr int CELL_P = 1024;                 // particles per k_cell tile (default)
+constexpr int CELL_P_MIN = 256;              // smallest tile size the launch settings accept
 constexpr int CELL_CAP = 2048;               // LDS staging capacity of one tile
 constexpr int CELL_BIG = 64;                 // cells with > 64 particles listed per tile
 constexpr int CELL_WAVES = 4;                // waves per k_cell block
@@ -22,16 +23,16 @@ hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, 
 hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell, Ctrl *ctrl,
                        hipStream_t st);
 hipError_t launch_scatter(const int2 *cellslot, const int *lstart, const int *toff, int *start,
-                          int *perm, int *tile_first, int ncell, const Ctrl *ctrl, int grid,
+                          int *perm, int *tile_first, int ncell, int tile_p, const Ctrl *ctrl, int grid,
                           hipStream_t st);
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
                        const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
-                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int grid,
+                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int tile_p, int grid,
                        hipStream_t st);
 hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
                              InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
                              const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
-                             Ctrl *ctrl, int grid, hipStream_t st);
+                             Ctrl *ctrl, int tile_p, int grid, hipStream_t st);
 hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, const uint8_t *cut,
                        const float *ux, const float *uy, const float *sig, int *count, int *lstart,
                        int *tsum, int *toff, Ctrl *ctrl, int grid, hipStream_t st);

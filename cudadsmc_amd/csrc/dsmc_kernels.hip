@@ -207,11 +207,11 @@ __global__ __launch_bounds__(256) void k_scatter(const int2 *__restrict__ cellsl
                                                  const int *__restrict__ lstart,
                                                  const int *__restrict__ toff, int *__restrict__ start,
                                                  int *__restrict__ perm, int *__restrict__ tile_first,
-                                                 int ncell, const Ctrl *__restrict__ ctrl)
+                                                 int ncell, int tile_p, const Ctrl *__restrict__ ctrl)
 {
     const uint32_t n_move = ctrl->n_move;
     const int n_next = (int)ctrl->n_next;
-    const int ntiles = (n_next + CELL_P - 1) / CELL_P;
+    const int ntiles = (n_next + tile_p - 1) / tile_p;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = gid; i < n_move; i += stride) {
@@ -224,9 +224,9 @@ __global__ __launch_bounds__(256) void k_scatter(const int2 *__restrict__ cellsl
         int tlo = 0;
         if (c > 0) {
             int prev = lstart[c - 1] + toff[(c - 1) / SCAN_TILE];
-            tlo = prev / CELL_P + 1;
+            tlo = prev / tile_p + 1;
         }
-        int thi = st / CELL_P;
+        int thi = st / tile_p;
         if (thi > ntiles - 1) thi = ntiles - 1;
         for (int t = tlo; t <= thi; ++t) tile_first[t] = (int)c;
     }
@@ -443,11 +443,12 @@ hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell,
 }
 
 hipError_t launch_scatter(const int2 *cellslot, const int *lstart, const int *toff, int *start,
-                          int *perm, int *tile_first, int ncell, const Ctrl *ctrl, int grid,
+                          int *perm, int *tile_first, int ncell, int tile_p, const Ctrl *ctrl, int grid,
                           hipStream_t st)
 {
+    if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, cellslot, lstart, toff, start, perm,
-                       tile_first, ncell, ctrl);
+                       tile_first, ncell, tile_p, ctrl);
     return hipGetLastError();
 }
 

@@ -35,6 +35,20 @@ hipError_t dalloc(T **p, size_t n)
     return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1));
 }
 
+// Validates launch settings (dsmc_set_launch): grids of 1..65536 blocks,
+// k_cell tiles of CELL_P_MIN..CELL_CAP particles, 1..64 steps per graph.
+inline int check_launch(const dsmc_launch *c)
+{
+    auto ok = [](int v, int lo, int hi) { return v >= lo && v <= hi; };
+    if (!ok(c->move_blocks, 1, 65536) || !ok(c->scatter_blocks, 1, 65536) || !ok(c->cell_blocks, 1, 65536))
+        return fail(DSMC_ERR_ARG, "launch: block counts must be 1..65536");
+    if (!ok(c->cell_tile, CELL_P_MIN, CELL_CAP))
+        return fail(DSMC_ERR_ARG, "launch: cell_tile must be " + std::to_string(CELL_P_MIN) + ".." +
+                                      std::to_string(CELL_CAP));
+    if (!ok(c->graph_steps, 1, 64)) return fail(DSMC_ERR_ARG, "launch: graph_steps must be 1..64");
+    return DSMC_OK;
+}
+
 // Particle capacity: the requested one, else 4x the fill estimate.
 inline int64_t default_capacity(const Param &P, const CutTables &T)
 {

@@ -216,6 +216,21 @@ class Run:
         names = ("n", "ux", "uy", "uz", "T", "p")
         return {k: v.reshape(shape) for k, v in zip(names, out)}
 
+    # -- launch settings (results never depend on them) --
+    def get_launch(self):
+        L = abi.dsmc_launch()
+        check(self.lib.dsmc_get_launch(self.h, C.byref(L)))
+        return L.as_dict()
+
+    def set_launch(self, **kw):
+        L = abi.dsmc_launch()
+        check(self.lib.dsmc_get_launch(self.h, C.byref(L)))
+        for k, v in kw.items():
+            if not hasattr(L, k):
+                raise KeyError(k)
+            setattr(L, k, int(v))
+        check(self.lib.dsmc_set_launch(self.h, C.byref(L)))
+
     # -- per-kernel timing --
     def set_timing(self, on=True):
         check(self.lib.dsmc_set_timing(self.h, 1 if on else 0))

@@ -108,6 +108,8 @@ typedef struct dsmc_handle_s *dsmc_handle;
 
 const char *dsmc_last_error(void);
 int dsmc_device_count(int *n);
+/* Compute units of the current device (sizes the launch settings). */
+int dsmc_device_cus(int *n);
 
 /* Lifecycle.  Replaces GPUParticles(N) / GPUCells(C) construction and
  * GPUCells::UploadFromHost's cut-cell tables (Particle.cuh:11, Cell.cuh:10-12;
@@ -196,6 +198,20 @@ int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples);
 int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,
                          double *uz, double *T, double *p);
 
+/* Launch settings (no reference counterpart; the reference hard-codes
+ * 128-thread blocks, GPUParticle.cu:60-74).  Results never depend on them --
+ * only the time does -- which is what lets the autotuner change them.
+ * Changing them drops the captured graph. */
+typedef struct dsmc_launch {
+    int move_blocks;     /* k_move grid, blocks of 256 threads */
+    int scatter_blocks;  /* k_scatter grid */
+    int cell_blocks;     /* k_cell grid (persistent blocks pulling tiles) */
+    int cell_tile;       /* particles per k_cell tile, 256..2048 */
+    int graph_steps;     /* steps per captured hipGraph, 1..64 */
+} dsmc_launch;
+int dsmc_get_launch(dsmc_handle h, dsmc_launch *out);
+int dsmc_set_launch(dsmc_handle h, const dsmc_launch *cfg);
+
 /* Per-kernel timing with HIP events on the handle's stream (eager steps
  * only).  ms[k] = mean duration of kernel k over timed steps, k = 0 move,
  * 1 scan, 2 scatter, 3 cell; bytes[k] = algorithmic bytes of the last launch. */
@@ -231,6 +247,8 @@ int dsmc_batch_reset_samples(dsmc_batch b);
 int dsmc_batch_download_samples(dsmc_batch b, int i, double *acc, uint64_t *nsamples);
 int dsmc_batch_download_fields(dsmc_batch b, int i, double *n, double *ux, double *uy, double *uz,
                                double *T, double *p);
+int dsmc_batch_get_launch(dsmc_batch b, dsmc_launch *out);
+int dsmc_batch_set_launch(dsmc_batch b, const dsmc_launch *cfg);
 /* Per-kernel timing of batched steps (eager), as dsmc_set_timing. */
 int dsmc_batch_set_timing(dsmc_batch b, int on);
 int dsmc_batch_kernel_times(dsmc_batch b, double ms[4], double bytes[4], int64_t *launches);
