@@ -77,6 +77,7 @@ _f64p = _P(C.c_double)
 _i32p = _P(C.c_int)
 _u32p = _P(C.c_uint32)
 _u8p = _P(C.c_uint8)
+_u64p = _P(C.c_uint64)
 _h = C.c_void_p
 
 # name -> argtypes; every function returns int status except dsmc_last_error
@@ -86,9 +87,9 @@ SIGNATURES = {
     "dsmc_create": [_P(dsmc_params), _P(dsmc_body), _P(C.c_void_p)],
     "dsmc_destroy": [_h],
     "dsmc_case_info_get": [_h, _P(dsmc_case_info)],
-    "dsmc_case_tables": [_h, _f32p, _i32p, _i32p, _f64p, _f32p, _u8p, _f32p],
+    "dsmc_case_tables": [_h, _f32p, _i32p, _i32p, _f64p, _f32p, _u8p, _f32p, _u64p],
     "dsmc_case_describe": [_P(dsmc_params), _P(dsmc_body), _P(dsmc_case_info), _f32p, _i32p, _i32p,
-                           _f64p, _f32p, _u8p, _f64p, _f32p],
+                           _f64p, _f32p, _u8p, _f64p, _f32p, _u64p],
     "dsmc_init_freestream": [_h],
     "dsmc_init_fields": [_h, _f64p, _f64p, _f64p, _f64p],
     "dsmc_upload_particles": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p],

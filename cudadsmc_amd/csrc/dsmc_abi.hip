@@ -49,6 +49,7 @@ struct dsmc_handle_s {
     int *d_cseg_start = nullptr, *d_cseg_idx = nullptr;
     uint8_t *d_cut = nullptr;
     float *d_fbox = nullptr;
+    uint64_t *d_fmask = nullptr;
     double *d_expected = nullptr;
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
     int tile_p = CELL_P, graph_len = 8;
@@ -143,8 +144,9 @@ void make_info(const Param &P, const CutTables &T, const DevCase &cs, int64_t ca
 }
 
 void copy_tables(const CutTables &T, float *seg, int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc,
-                 uint8_t *cut, double *frac, float *fbox)
+                 uint8_t *cut, double *frac, float *fbox, uint64_t *fmask)
 {
+    if (fmask) std::copy(T.fmask.begin(), T.fmask.end(), fmask);
     if (seg) std::copy(T.seg.begin(), T.seg.end(), seg);
     if (cseg_start) std::copy(T.cseg_start.begin(), T.cseg_start.end(), cseg_start);
     if (cseg_idx) std::copy(T.cseg_idx.begin(), T.cseg_idx.end(), cseg_idx);
@@ -247,7 +249,7 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
         (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, 6 * C)) ||
         (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
         (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||
-        (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) || (e = h->alloc(&h->d_fbox, 4 * C)) ||
+        (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) || (e = h->alloc(&h->d_fbox, 4 * C)) || (e = h->alloc(&h->d_fmask, C)) ||
         (e = h->alloc(&h->d_expected, C)) || (e = h->alloc(&h->d_ux, C)) || (e = h->alloc(&h->d_uy, C)) ||
         (e = h->alloc(&h->d_sig, C)))
         return bad(e, "cell buffers");
@@ -259,6 +261,7 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
         (e = up(h->d_cseg_idx, T.cseg_idx.data(), sizeof(int) * T.cseg_idx.size())) ||
         (e = up(h->d_vol_ntc, T.vol_ntc.data(), sizeof(float) * C)) ||
         (e = up(h->d_cut, T.cut.data(), C)) || (e = up(h->d_fbox, T.fbox.data(), sizeof(float) * 4 * C)) ||
+        (e = up(h->d_fmask, T.fmask.data(), sizeof(uint64_t) * C)) ||
         (e = hipMemset(h->ctrl, 0, sizeof(Ctrl))))
         return bad(e, "table upload");
 
@@ -266,6 +269,7 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
     cs = make_devcase(P, T, h->cap);
     cs.seg = h->d_seg; cs.cseg_start = h->d_cseg_start; cs.cseg_idx = h->d_cseg_idx; cs.vol_ntc = h->d_vol_ntc;
     cs.fbox = h->d_fbox;
+    cs.fmask = h->d_fmask;
 
     int cu = 0;
     if ((e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device)) != hipSuccess)
@@ -304,16 +308,16 @@ int dsmc_case_info_get(dsmc_handle h, dsmc_case_info *o)
 }
 
 int dsmc_case_tables(dsmc_handle h, float *seg, int *cseg_start, int *cseg_idx, double *vol,
-                     float *vol_ntc, uint8_t *cut, float *fbox)
+                     float *vol_ntc, uint8_t *cut, float *fbox, uint64_t *fmask)
 {
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
-    copy_tables(h->T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, nullptr, fbox);
+    copy_tables(h->T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, nullptr, fbox, fmask);
     return DSMC_OK;
 }
 
 int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_info *info, float *seg,
                        int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc, uint8_t *cut,
-                       double *frac, float *fbox)
+                       double *frac, float *fbox, uint64_t *fmask)
 {
     if (!p || !info) return fail(DSMC_ERR_ARG, "null argument");
     Param P;
@@ -325,7 +329,7 @@ int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_in
     int64_t cap = default_capacity(P, T);
     DevCase cs = make_devcase(P, T, cap);
     make_info(P, T, cs, cap, info);
-    copy_tables(T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, frac, fbox);
+    copy_tables(T, seg, cseg_start, cseg_idx, vol, vol_ntc, cut, frac, fbox, fmask);
     return DSMC_OK;
 }
 

@@ -43,6 +43,7 @@ struct dsmc_batch_s {
     CellConst *d_cc = nullptr;
     int *d_cbase = nullptr;
     float *d_seg = nullptr, *d_vol_ntc = nullptr, *d_fbox = nullptr;
+    uint64_t *d_fmask = nullptr;
     float *d_ux = nullptr, *d_uy = nullptr, *d_sig = nullptr;
     int *d_cseg_start = nullptr, *d_cseg_idx = nullptr;
     uint8_t *d_cut = nullptr;
@@ -277,7 +278,7 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
         (e = b->alloc(&b->d_cases, n)) || (e = b->alloc(&b->d_cc, n)) || (e = b->alloc(&b->d_cbase, n + 1)) ||
         (e = b->alloc(&b->d_seg, segs)) || (e = b->alloc(&b->d_cseg_start, Cs + n)) ||
         (e = b->alloc(&b->d_cseg_idx, idx)) || (e = b->alloc(&b->d_vol_ntc, Cs)) || (e = b->alloc(&b->d_cut, Cs)) ||
-        (e = b->alloc(&b->d_fbox, 4 * Cs)) || (e = b->alloc(&b->d_expected, Cs)) || (e = b->alloc(&b->d_ux, Cs)) ||
+        (e = b->alloc(&b->d_fbox, 4 * Cs)) || (e = b->alloc(&b->d_fmask, Cs)) || (e = b->alloc(&b->d_expected, Cs)) || (e = b->alloc(&b->d_ux, Cs)) ||
         (e = b->alloc(&b->d_uy, Cs)) || (e = b->alloc(&b->d_sig, Cs)))
         return bad(e, "cell buffers");
 
@@ -296,7 +297,8 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
             (e = up(b->d_cseg_idx + io, Ti.cseg_idx.data(), sizeof(int) * Ti.cseg_idx.size())) ||
             (e = up(b->d_vol_ntc + cb, Ti.vol_ntc.data(), sizeof(float) * Ci)) ||
             (e = up(b->d_cut + cb, Ti.cut.data(), Ci)) ||
-            (e = up(b->d_fbox + 4 * (size_t)cb, Ti.fbox.data(), sizeof(float) * 4 * Ci)))
+            (e = up(b->d_fbox + 4 * (size_t)cb, Ti.fbox.data(), sizeof(float) * 4 * Ci)) ||
+            (e = up(b->d_fmask + cb, Ti.fmask.data(), sizeof(uint64_t) * Ci)))
             return bad(e, "table upload");
         DevCase &c = b->cs[i];
         c = make_devcase(b->P[i], Ti, b->capi[i]);
@@ -305,6 +307,7 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
         c.cseg_idx = b->d_cseg_idx + io;
         c.vol_ntc = b->d_vol_ntc + cb;
         c.fbox = b->d_fbox + 4 * (size_t)cb;
+        c.fmask = b->d_fmask + cb;
         cc[i] = CellConst{ c.ntc_coef, c.K1, c.e_exp, c.seed, b->P[i].in.sample_start, 0 };
         so += (int64_t)Ti.seg.size();
         io += (int64_t)Ti.cseg_idx.size();

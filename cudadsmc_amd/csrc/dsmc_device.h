@@ -17,7 +17,7 @@ enum { SUB_MOVE = 1, SUB_INJECT = 2, SUB_COLLIDE = 3, SUB_INIT = 4 };
 enum { ERR_BOUNCE = 1, ERR_INFLOW = 2, ERR_CAPACITY = 4, ERR_FILL = 8 };
 constexpr int MAX_BOUNCE = 64;
 constexpr int INFLOW_TRIES = 256;
-constexpr int FILL_TRIES = 256;
+constexpr int FILL_TRIES = 1024;
 constexpr float SEG_TOL = 1e-5f;   // relative overlap at body segment ends
 constexpr float SEG_BAND = 1e-4f;  // wall band (fraction of a cell) caught as an immediate hit
 
@@ -44,6 +44,7 @@ struct DevCase {
     const int *cseg_start, *cseg_idx;
     const float *vol_ntc;
     const float *fbox;      // 4 per cell: fluid-part bounding box of cut cells
+    const uint64_t *fmask;  // per cell: 8x8 sub-boxes of fbox that touch fluid
 };
 
 // Device-resident step control: everything a step needs that changes

@@ -260,6 +260,17 @@ __device__ __forceinline__ bool inside_body(const DevCase &cs, float px, float p
     return in;
 }
 
+// Index of the r-th set bit of m (r < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r)
+{
+    for (int b = 0; b < 64; ++b)
+        if ((m >> b) & 1ull) {
+            if (r == 0) return b;
+            --r;
+        }
+    return 63;
+}
+
 // One wave per cell; particles of cell c go to slots start(c) + k, pid = slot.
 __global__ __launch_bounds__(256) void k_fill(DevCase cs, SoA A, const int *__restrict__ lstart,
                                               const int *__restrict__ toff, const uint8_t *__restrict__ cut,
@@ -280,17 +291,26 @@ __global__ __launch_bounds__(256) void k_fill(DevCase cs, SoA A, const int *__re
         const float y0 = ct ? cs.fbox[4 * c + 1] : (float)cj * cs.dy;
         const float wx = ct ? cs.fbox[4 * c + 2] - cs.fbox[4 * c] : cs.dx;
         const float wy = ct ? cs.fbox[4 * c + 3] - cs.fbox[4 * c + 1] : cs.dy;
+        const uint64_t fm = ct ? cs.fmask[c] : ~0ull;
+        const int nm = __popcll(fm);
+        const float sw = wx * 0.125f, sh = wy * 0.125f;
         for (int k = lane; k < en - st; k += 64) {
             float px = x0, py = y0;
             bool ok = false;
-            for (int tr = 0; tr < FILL_TRIES && !ok; tr += 2) {
-                uint4 w = philox((uint32_t)k, 0u, SUB_INIT, 1u + (uint32_t)(tr / 2), (uint32_t)c, cs.seed);
+            if (!ct) {
+                const uint4 w = philox((uint32_t)k, 0u, SUB_INIT, 1u, (uint32_t)c, cs.seed);
                 px = x0 + wx * u01(w.x);
                 py = y0 + wy * u01(w.y);
-                if (!ct || !inside_body(cs, px, py)) { ok = true; break; }
-                px = x0 + wx * u01(w.z);
-                py = y0 + wy * u01(w.w);
-                if (!ct || !inside_body(cs, px, py)) { ok = true; break; }
+                ok = true;
+            }
+            // cut cells: a sub-box of the fluid bounding box that touches
+            // fluid, a point in it, rejected if inside the body
+            for (int tr = 0; tr < FILL_TRIES && !ok; ++tr) {
+                const uint4 w = philox((uint32_t)k, 0u, SUB_INIT, 1u + (uint32_t)tr, (uint32_t)c, cs.seed);
+                const int sb = nth_set_bit(fm, mulhi_idx(w.x, nm));
+                px = x0 + sw * ((float)(sb & 7) + u01(w.y));
+                py = y0 + sh * ((float)(sb >> 3) + u01(w.z));
+                ok = !inside_body(cs, px, py);
             }
             if (!ok) err |= ERR_FILL;
             uint4 w = philox((uint32_t)k, 0u, SUB_INIT, 0u, (uint32_t)c, cs.seed);

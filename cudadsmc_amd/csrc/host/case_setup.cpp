@@ -199,6 +199,23 @@ void fluid_bbox(const std::vector<V2> &poly, double x0, double y0, double x1, do
     out[3] = (float)std::min(y1, by1 + gy);
 }
 
+// Which sub-boxes of the 8x8 split of a fluid bounding box touch fluid
+// (are not entirely inside the body).  The fill draws a marked sub-box, then
+// a point in it, then rejects points inside the body: uniform over the fluid,
+// with an acceptance rate that does not collapse for thin fluid slivers.
+uint64_t fluid_mask(const std::vector<V2> &poly, const float *fb)
+{
+    const double x0 = fb[0], y0 = fb[1], w = ((double)fb[2] - x0) / 8.0, h = ((double)fb[3] - y0) / 8.0;
+    uint64_t m = 0;
+    for (int sy = 0; sy < 8; ++sy)
+        for (int sx = 0; sx < 8; ++sx) {
+            const double a0 = x0 + sx * w, b0 = y0 + sy * h;
+            const double solid = clipped_area(poly, a0, b0, a0 + w, b0 + h);
+            if (solid < w * h * (1.0 - 1e-9)) m |= 1ull << (sx + 8 * sy);
+        }
+    return m ? m : ~0ull;
+}
+
 }  // namespace
 
 std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T)
@@ -213,6 +230,7 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
     T.vol_ntc.assign(C, (float)P.Vcell);
     T.cut.assign(C, 0);
     T.fbox.assign(4 * (size_t)C, 0.0f);
+    T.fmask.assign(C, ~0ull);
     T.cseg_start.assign(C + 1, 0);
     if (!body || body->n_points == 0) return "";
     if (body->n_points < 3 || !body->xy) return "body polygon needs >= 3 points";
@@ -287,6 +305,7 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
             if (fr > 0.0) {
                 ++T.ncut;
                 fluid_bbox(poly, i * P.dx, j * P.dy, (i + 1) * P.dx, (j + 1) * P.dy, &T.fbox[4 * (size_t)c]);
+                T.fmask[c] = fluid_mask(poly, &T.fbox[4 * (size_t)c]);
             } else {
                 T.vol_ntc[c] = 0.0f;
             }

@@ -47,6 +47,8 @@ struct CutTables {
     std::vector<float> vol_ntc;      // volume used by NTC (floored at 0.1 cell)
     std::vector<uint8_t> cut;        // 0 < frac < 1
     std::vector<float> fbox;         // 4 per cell: bounding box of the fluid part of a cut cell
+    std::vector<uint64_t> fmask;     // per cut cell: bit sx + 8 sy set when sub-box (sx, sy) of the
+                                     // 8x8 split of fbox touches fluid (all bits for uncut cells)
     float margin = 0;
 };
 

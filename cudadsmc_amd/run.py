@@ -27,17 +27,18 @@ def describe(params, body=None):
     b, _xy = _body_struct(body)
     bptr = C.byref(b) if b is not None else None
     info = abi.dsmc_case_info()
-    check(lib.dsmc_case_describe(C.byref(params), bptr, C.byref(info), None, None, None, None, None, None, None, None))
+    check(lib.dsmc_case_describe(C.byref(params), bptr, C.byref(info), None, None, None, None, None, None, None, None,
+                                 None))
     nc = info.ncell
     t = dict(seg=np.empty(6 * info.nseg, np.float32), cseg_start=np.empty(nc + 1, np.int32),
              cseg_idx=np.empty(max(info.cseg_len, 1), np.int32), vol=np.empty(nc, np.float64),
              vol_ntc=np.empty(nc, np.float32), cut=np.empty(nc, np.uint8), frac=np.empty(nc, np.float64),
-             fbox=np.empty(4 * nc, np.float32))
+             fbox=np.empty(4 * nc, np.float32), fmask=np.empty(nc, np.uint64))
     check(lib.dsmc_case_describe(C.byref(params), bptr, C.byref(info), ptr(t["seg"], C.c_float),
                                  ptr(t["cseg_start"], C.c_int), ptr(t["cseg_idx"], C.c_int),
                                  ptr(t["vol"], C.c_double), ptr(t["vol_ntc"], C.c_float),
                                  ptr(t["cut"], C.c_uint8), ptr(t["frac"], C.c_double),
-                                 ptr(t["fbox"], C.c_float)))
+                                 ptr(t["fbox"], C.c_float), ptr(t["fmask"], C.c_uint64)))
     return info, t
 
 
@@ -132,10 +133,11 @@ class Run:
         vn = np.empty(i.ncell, np.float32)
         cut = np.empty(i.ncell, np.uint8)
         fb = np.empty(4 * i.ncell, np.float32)
+        fm = np.empty(i.ncell, np.uint64)
         check(self.lib.dsmc_case_tables(self.h, ptr(seg, C.c_float), ptr(cs, C.c_int), ptr(ci, C.c_int),
                                         ptr(vol, C.c_double), ptr(vn, C.c_float), ptr(cut, C.c_uint8),
-                                        ptr(fb, C.c_float)))
-        return dict(seg=seg, cseg_start=cs, cseg_idx=ci, vol=vol, vol_ntc=vn, cut=cut, fbox=fb)
+                                        ptr(fb, C.c_float), ptr(fm, C.c_uint64)))
+        return dict(seg=seg, cseg_start=cs, cseg_idx=ci, vol=vol, vol_ntc=vn, cut=cut, fbox=fb, fmask=fm)
 
     def bins(self):
         s = self.status()

@@ -122,13 +122,13 @@ int dsmc_case_info_get(dsmc_handle h, dsmc_case_info *out);
  * vol_ntc[ncell] NTC volume, cut[ncell] partial-cell flag, fbox[4*ncell]
  * bounding box (x0 y0 x1 y1) of the fluid part of cut cells.  NULL skips. */
 int dsmc_case_tables(dsmc_handle h, float *seg, int *cseg_start, int *cseg_idx,
-                     double *vol, float *vol_ntc, uint8_t *cut, float *fbox);
+                     double *vol, float *vol_ntc, uint8_t *cut, float *fbox, uint64_t *fmask);
 /* Host only (no device needed): the same derived constants and tables for a
  * parameter set, plus frac[ncell] fluid fraction.  Call with NULL tables to
  * get the sizes in info first. */
 int dsmc_case_describe(const dsmc_params *p, const dsmc_body *body, dsmc_case_info *info,
                        float *seg, int *cseg_start, int *cseg_idx, double *vol,
-                       float *vol_ntc, uint8_t *cut, double *frac, float *fbox);
+                       float *vol_ntc, uint8_t *cut, double *frac, float *fbox, uint64_t *fmask);
 
 /* Initial state.  Replaces Run::assignParticle + TransferParticlesFromHostToDevice
  * (Run.cpp:108-191).  init_freestream fills every fluid cell at n_inf, T,

@@ -43,6 +43,16 @@ static inline float u01(uint32_t w)  { return (float)(w >> 8) * 0x1p-24f; }
 static inline float u01o(uint32_t w) { return (float)((w >> 8) + 1u) * 0x1p-24f; }
 /* floor(w * n / 2^32): unbiased-enough integer in [0,n) without FP */
 static inline int mulhi_idx(uint32_t w, int n) { return (int)(((uint64_t)w * (uint64_t)(uint32_t)n) >> 32); }
+/* index of the r-th set bit of m (r < popcount(m)) */
+static inline int nth_set_bit(uint64_t m, int r)
+{
+    for (int b = 0; b < 64; ++b)
+        if ((m >> b) & 1u) {
+            if (r == 0) return b;
+            --r;
+        }
+    return 63;
+}
 
 /* ------------------------------------------------------------------------- */
 /* Elementary functions from IEEE basic operations only.                      */
@@ -545,18 +555,29 @@ int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
         float y0 = cut[c] ? fb[1] : (float)cj * cs->dy;
         float wx = cut[c] ? fb[2] - fb[0] : cs->dx;
         float wy = cut[c] ? fb[3] - fb[1] : cs->dy;
+        const uint64_t fm = cut[c] ? cs->fmask[c] : ~0ull;
+        const int nm = __builtin_popcountll(fm);
+        const float sw = wx * 0.125f, sh = wy * 0.125f;
         for (int k = 0; k < count[c]; ++k, ++i) {
             uint32_t w[4];
             float px = x0, py = y0;
             int ok = 0;
-            for (int tr = 0; tr < OR_FILL_TRIES && !ok; tr += 2) {
-                uint32_t ctr[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 1u + (uint32_t)(tr / 2) };
+            if (!cut[c]) {
+                uint32_t ctr[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 1u };
                 or_philox(ctr, (uint32_t)c, cs->seed, w);
-                for (int h = 0; h < 2; ++h) {
-                    px = x0 + wx * u01(w[2 * h]);
-                    py = y0 + wy * u01(w[2 * h + 1]);
-                    if (!cut[c] || !inside_body(cs, px, py)) { ok = 1; break; }
-                }
+                px = x0 + wx * u01(w[0]);
+                py = y0 + wy * u01(w[1]);
+                ok = 1;
+            }
+            /* cut cells: a sub-box of the fluid bounding box that touches
+             * fluid (host table fmask), a point in it, rejected inside the body */
+            for (int tr = 0; tr < OR_FILL_TRIES && !ok; ++tr) {
+                uint32_t ctr[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 1u + (uint32_t)tr };
+                or_philox(ctr, (uint32_t)c, cs->seed, w);
+                int sb = nth_set_bit(fm, mulhi_idx(w[0], nm));
+                px = x0 + sw * ((float)(sb & 7) + u01(w[1]));
+                py = y0 + sh * ((float)(sb >> 3) + u01(w[2]));
+                ok = !inside_body(cs, px, py);
             }
             if (!ok) e |= OR_ERR_FILL;
             uint32_t cv[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 0u };

@@ -39,7 +39,7 @@ enum { OR_ERR_BOUNCE = 1, OR_ERR_INFLOW = 2, OR_ERR_CAPACITY = 4, OR_ERR_FILL = 
 
 #define OR_MAX_BOUNCE 64
 #define OR_INFLOW_TRIES 256
-#define OR_FILL_TRIES 256
+#define OR_FILL_TRIES 1024
 /* Segment ends overlap by this fraction so that a ray through a vertex shared
  * by two body segments cannot pass between the two rounded tests. */
 #define OR_SEG_TOL 1e-5f
@@ -72,6 +72,7 @@ typedef struct or_case {
     const int   *cseg_idx;
     const float *vol_ntc;              /* per-cell volume used by NTC */
     const float *fbox;                 /* 4 per cell: fluid-part bounding box of cut cells */
+    const uint64_t *fmask;             /* per cell: 8x8 sub-boxes of fbox that touch fluid */
     uint32_t seed;
 } or_case;
 

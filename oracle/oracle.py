@@ -37,6 +37,7 @@ class or_case(C.Structure):
         ("cseg_idx", C.POINTER(C.c_int)),
         ("vol_ntc", C.POINTER(C.c_float)),
         ("fbox", C.POINTER(C.c_float)),
+        ("fmask", C.POINTER(C.c_uint64)),
         ("seed", C.c_uint32),
     ]
 
@@ -167,6 +168,7 @@ class OracleCase:
         c.cseg_idx = _p(self.t["cseg_idx"], C.c_int)
         c.vol_ntc = _p(self.t["vol_ntc"], C.c_float)
         c.fbox = _p(self.t["fbox"], C.c_float)
+        c.fmask = _p(self.t["fmask"], C.c_uint64)
         self.c = c
         self.ptr = C.byref(c)
 

@@ -40,6 +40,7 @@ def test_null_arguments_are_errors():
     assert lib.dsmc_run(None, 1, 0) == abi.DSMC_ERR_ARG
     assert b"handle" in lib.dsmc_last_error() or b"argument" in lib.dsmc_last_error()
     assert lib.dsmc_destroy(None) == abi.DSMC_OK        # like free(NULL)
-    assert lib.dsmc_case_describe(None, None, None, None, None, None, None, None, None, None, None) != abi.DSMC_OK
+    assert lib.dsmc_case_describe(None, None, None, None, None, None, None, None, None, None, None,
+                                  None) != abi.DSMC_OK
     with pytest.raises(abi.DSMCError):
         abi.check(lib.dsmc_sync(None))

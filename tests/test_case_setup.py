@@ -142,6 +142,11 @@ def test_cut_volumes_match_monte_carlo_per_cell():
         out = ~_point_in_poly(fx, fy, body)
         assert fx[out].min() >= fb[0] - 1e-6 and fx[out].max() <= fb[2] + 1e-6
         assert fy[out].min() >= fb[1] - 1e-6 and fy[out].max() <= fb[3] + 1e-6
+        # ... and every fluid point lies in a sub-box the fill mask marks
+        sx = np.clip(((fx[out] - fb[0]) / (fb[2] - fb[0]) * 8).astype(int), 0, 7)
+        sy = np.clip(((fy[out] - fb[1]) / (fb[3] - fb[1]) * 8).astype(int), 0, 7)
+        bits = (np.uint64(t["fmask"][c]) >> (sx + 8 * sy).astype(np.uint64)) & np.uint64(1)
+        assert np.all(bits == 1), c
 
 
 def _seg_rect_dist(ax, ay, bx, by, x0, y0, x1, y1):

@@ -361,3 +361,22 @@ def test_no_particle_enters_the_body():
     q = st.particles()
     assert np.all(_outside(q["x"].astype(np.float64), q["y"].astype(np.float64), body))
     assert np.all((q["x"] >= 0) & (q["x"] <= info.L1) & (q["y"] >= 0) & (q["y"] <= info.L2))
+
+
+@pytest.mark.parametrize("index", [3, 54])
+def test_fill_of_thin_fluid_slivers(index):
+    """Wedge instances of the batched sweep whose back-face cells keep 1-3 %
+    of their fluid bounding box as an L-shaped sliver: the sub-box masked
+    sampler places every particle in the fluid (plain rejection in the
+    bounding box ran out of tries here)."""
+    c = cases.sweep(64)[index]
+    oc, info, tables = O.case_from_params(c["params"], c["body"])
+    cut = tables["cut"] == 1
+    e = np.where(cut, info.ppc_eff * tables["frac"], 0.0)
+    ux = np.full(info.ncell, info.inj_ux, np.float32)
+    uy = np.full(info.ncell, info.inj_uy, np.float32)
+    sg = np.full(info.ncell, info.inj_sig, np.float32)
+    parts, count, err = oc.init_fill(e, tables["cut"], ux, uy, sg, int(e.sum() * 2 + 1000))
+    assert err == 0
+    assert np.all(_outside(parts["x"].astype(np.float64), parts["y"].astype(np.float64), c["body"]))
+
