@@ -1,0 +1,91 @@
+"""GPU: the warm-start entry (dsmc_init_fields / Batch.initialize_fields)
+places local Maxwellians with the requested moments; the residual check
+falls back to freestream for a deliberately wrong prediction and keeps a
+sound one; the steady-state runner reports every instance."""
+import math
+
+import numpy as np
+import pytest
+
+from cudadsmc_amd import cases, surrogate as S
+from cudadsmc_amd.batch import Batch
+from cudadsmc_amd.run import Run, describe
+
+pytestmark = pytest.mark.gpu
+
+
+def test_init_fields_moments(gpu_lib):
+    """Left half of the box at 2 n0, 600 K, drift +300 m/s; right half at
+    n0, 300 K, at rest: counts, mean velocity and temperature of each half
+    match the request within 3 sigma."""
+    cfg = cases.box_equilibrium(ppc=20, n_cells=32, seed=5)
+    p = cfg["params"]
+    r = Run(p, None)
+    C = p.N1 * p.N2
+    i = np.arange(C) // p.N2
+    left = i < p.N1 // 2
+    n = np.where(left, 2.0 * p.n_inf, p.n_inf)
+    T = np.where(left, 600.0, 300.0)
+    ux = np.where(left, 300.0, 0.0)
+    r.initialize_fields(n, ux, np.zeros(C), T)
+    q = r.download()
+    L = p.L1
+    k, m = cases.BOLTZ, cases.ARGON["mass"]
+    for sel, nn, TT, uu in ((q["x"] < L / 2, 2.0, 600.0, 300.0), (q["x"] >= L / 2, 1.0, 300.0, 0.0)):
+        N = int(sel.sum())
+        expect = 20 * nn * C / 2
+        assert abs(N - expect) < 3 * math.sqrt(expect) + 2, (N, expect)
+        sig = math.sqrt(k * TT / m)
+        vx = q["vx"][sel].astype(np.float64)
+        assert abs(vx.mean() - uu) < 3 * sig / math.sqrt(N)
+        var = np.var(q["vz"][sel].astype(np.float64))
+        assert abs(var - sig * sig) < 3 * sig * sig * math.sqrt(2.0 / N)
+    r.close()
+
+
+class _RunPredictor:
+    """Stands in for a trained surrogate: 'predicts' the fields a plain run
+    of the same case reached after settling (a sound prediction)."""
+
+    def __init__(self, fields):
+        self.fields = fields
+
+    def fields_for(self, pt, cfg):
+        f = self.fields[pt["seed"]]
+        p = cfg["params"]
+        sampled = f["n"].ravel() > 0
+        return (np.where(sampled, f["n"].ravel(), p.n_inf), np.where(sampled, f["ux"].ravel(), p.V_jet),
+                np.where(sampled, f["uy"].ravel(), 0.0), np.where(sampled & (f["T"].ravel() > 0), f["T"].ravel(), p.T))
+
+
+def test_residual_check_falls_back_on_a_bad_prediction(gpu_lib):
+    """A sound prediction passes the check; the same prediction with the
+    flow reversed (forced from the host) falls back to freestream."""
+    pts = S.draw_points(2, seed=5, grids=[(64, 32)], ppc=(20.0, 20.0))
+    cfgs = [S.case_of(pt) for pt in pts]
+    tables = [describe(c["params"], c["body"])[1] for c in cfgs]
+    plain = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+    plain.initialize()
+    plain.solver(1500)
+    f = S.averaged_fields(plain, 1000)
+    plain.close()
+    pred = _RunPredictor({pt["seed"]: f[i] for i, pt in enumerate(pts)})
+    sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+    fb, res = S.start(sim, pts, cfgs, tables, pred, check_steps=60, force_bad=[1], log=None)
+    print(f"[warm] residuals {res}", flush=True)
+    assert list(fb) == [False, True]
+    assert res[0] < 0.35 and res[1] > 1.0
+    assert sim.status(-1).err == 0
+    sim.close()
+
+
+def test_run_to_steady_reports_every_instance(gpu_lib):
+    pts = S.draw_points(3, seed=8, grids=[(64, 32)], ppc=(10.0, 10.0))
+    cfgs = [S.case_of(pt) for pt in pts]
+    tables = [describe(c["params"], c["body"])[1] for c in cfgs]
+    sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+    sim.initialize()
+    steps_at, wall = S.run_to_steady(sim, tables, window=200, max_steps=20000, log=None)
+    print(f"[warm] steps to steady {steps_at}", flush=True)
+    assert np.all(steps_at > 0)
+    sim.close()
