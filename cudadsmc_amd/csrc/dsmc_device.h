@@ -234,8 +234,13 @@ __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, 
                 float s = (wx * ey - wy * ex) / den;
                 if (!(s < best)) continue;
                 if (!(s > 0.0f)) {
-                    // on or just behind the wall line (a previous hit point
-                    // rounded to the solid side) and moving in: hit now
+                    // on or just behind the wall line (a previous step's hit
+                    // point rounded to the solid side) and moving in: hit now,
+                    // at the start of a flight only.  After a reflection the
+                    // particle leaves the face it hit, which for a convex body
+                    // is leaving the body; a band hit on the neighbouring face
+                    // of an acute vertex would trap it there.
+                    if (b > 0) continue;
                     float d = -(wx * snx + wy * sny);
                     if (d < -(cs.margin * SEG_BAND)) continue;
                     s = 0.0f;

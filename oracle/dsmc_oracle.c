@@ -217,7 +217,13 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
                 float s = (wx * ey - wy * ex) / den;
                 if (!(s < best)) continue;
                 if (!(s > 0.0f)) {
-                    /* on or just behind the wall line and moving in: hit now */
+                    /* on or just behind the wall line and moving in: hit now --
+                     * only at the start of a flight (a previous step's hit point
+                     * rounded to the solid side).  After a reflection the particle
+                     * leaves the face it hit, which for a convex body is leaving
+                     * the body; a band hit on the neighbouring face of an acute
+                     * vertex would trap it there. */
+                    if (b > 0) continue;
                     float d = -(wx * S[4] + wy * S[5]);
                     if (d < -(cs->margin * OR_SEG_BAND)) continue;
                     s = 0.0f;

@@ -380,3 +380,22 @@ def test_fill_of_thin_fluid_slivers(index):
     assert err == 0
     assert np.all(_outside(parts["x"].astype(np.float64), parts["y"].astype(np.float64), c["body"]))
 
+
+
+def test_acute_vertex_does_not_trap_a_particle():
+    """A particle that hits a 17 deg wedge 0.6 um from the apex, where the
+    body is thinner than the wall band: the re-emitted particle leaves with
+    one wall event instead of bouncing between the two faces' bands until
+    the 64-event limit (found in a batched sweep run)."""
+    from cudadsmc_amd import surrogate as S
+    pt = {"geometry": "wedge", "half_angle_deg": 17.054940866729332, "mach": 5.867278090943784,
+          "kn": 0.02070981818448305, "tw_ratio": 3.3452777555517583, "grid": (128, 64),
+          "ppc": 30.000000000000004, "seed": 3175139750848533173}
+    cfg = S.case_of(pt, sample_from=4000)
+    oc, info, _ = O.case_from_params(cfg["params"], cfg["body"])
+    s = dict(x=np.array([0.14987538754940033], np.float32), y=np.array([0.15022362768650055], np.float32),
+             vx=np.array([790.2767944335938], np.float32), vy=np.array([-1410.759033203125], np.float32),
+             vz=np.array([80.26766967773438], np.float32), pid=np.array([422860], np.uint32))
+    dead, err = oc.move(s, 667)
+    assert err == 0 and not dead[0]
+    assert np.all(_outside(s["x"].astype(np.float64), s["y"].astype(np.float64), cfg["body"]))
