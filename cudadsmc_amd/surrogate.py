@@ -26,6 +26,8 @@ from .steady import SteadyDetector, density
 
 WIDTHS = [7, 256, 256, 256, 256, 4]
 GRID = (128, 64)
+# held-out evaluation grids: the Mach-10 cylinder bench resolution and below
+EVAL_GRIDS = {"coarse": cases.SWEEP_GRIDS[:5], "fine": [(256, 128), (320, 160), (400, 200)]}
 
 
 # ----------------------------------------------------------------- features
@@ -159,7 +161,7 @@ class Surrogate:
             lr_k = lr * 0.5 * (1 + math.cos(math.pi * k / max(steps, 1)))
             losses.append(self.net.train(n, batch=batch, lr=lr_k, seed=seed + k))
             if log:
-                log(f"[surrogate] fit step {k + n}/{steps}: loss {losses[-1][-1]:.5f}")
+                log(f"[surrogate] fit step {k + n}/{steps}: mean loss {float(np.mean(losses[-1])):.5f}")
         return np.concatenate(losses)
 
     def predict(self, X):
@@ -281,7 +283,7 @@ def field_errors(fw, fp, frac):
     return dict(n=e_n, u=e_u, T=e_T)
 
 
-def evaluate(surrogate, pts, window=200, average_steps=3000, settle_steps=4000, check_steps=60, log=print):
+def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000, check_steps=60, log=print):
     """Warm start vs plain on held-out points (one batch each): steps and
     wall-clock to steady; field error of the warm run (averaged over
     average_steps right after it is steady) against the converged plain run
@@ -331,11 +333,14 @@ def evaluate(surrogate, pts, window=200, average_steps=3000, settle_steps=4000, 
                 max_field_error=float(max(max(e.values()) for e in errs)),
                 plain_field_error=errs_plain,
                 max_plain_field_error=float(max(max(e.values()) for e in errs_plain)),
+                within_2pct=int(sum(max(e.values()) <= 0.02 for e in errs)),
+                within_2pct_plain=int(sum(max(e.values()) <= 0.02 for e in errs_plain)),
+                window=window, average_steps=average_steps, settle_steps=settle_steps,
                 device_err=[out["plain"]["err"], out["warm"]["err"]])
 
 
 def pipeline(outdir, runs=512, n_eval=32, fit_steps=3000, steady_steps=4000, sample_steps=2000,
-             average_steps=3000, settle_steps=4000, log=print):
+             average_steps=12000, settle_steps=6000, grids="fine", log=print):
     os.makedirs(outdir, exist_ok=True)
     t0 = time.perf_counter()
     train_pts = draw_points(runs, seed=11)
@@ -348,27 +353,52 @@ def pipeline(outdir, runs=512, n_eval=32, fit_steps=3000, steady_steps=4000, sam
     t_fit = time.perf_counter() - t1
     ms, flop = sur.net.train_stats()
     sur.save(os.path.join(outdir, "surrogate.npz"))
-    eval_pts = draw_points(n_eval, seed=97, grids=cases.SWEEP_GRIDS[:5])
+    eval_pts = draw_points(n_eval, seed=97, grids=EVAL_GRIDS[grids])
     res = evaluate(sur, eval_pts, average_steps=average_steps, settle_steps=settle_steps, log=log)
+    res["grids"] = grids
     res.update(train_samples=int(X.shape[0]), train_runs=runs, data_s=t_data, fit_s=t_fit,
                fit_steps=fit_steps, fit_loss_first=float(losses[0]), fit_loss_last=float(losses[-1]),
                train_ms_per_step=ms, train_tflops=flop / ms / 1e9 if ms > 0 else 0.0)
     with open(os.path.join(outdir, "warmstart.json"), "w") as f:
         json.dump(res, f, indent=1)
     log(f"[surrogate] speedup (wall) {res['speedup_wall']:.2f}x, median steps {res['median_steps_plain']:.0f} -> "
-        f"{res['median_steps_warm']:.0f}, max field error {res['max_field_error']:.4f}, "
-        f"fallback {res['fallback_rate']:.2f}")
+        f"{res['median_steps_warm']:.0f}, max field error {res['max_field_error']:.4f} "
+        f"(plain {res['max_plain_field_error']:.4f}), within 2%: warm {res['within_2pct']}/{n_eval}, "
+        f"plain {res['within_2pct_plain']}/{n_eval}, fallback {res['fallback_rate']:.2f}")
+    return res
+
+
+def evaluate_saved(model, outdir, n_eval=32, window=100, grids="fine", log=print):
+    """The held-out evaluation of a saved surrogate (no data generation)."""
+    os.makedirs(outdir, exist_ok=True)
+    sur = Surrogate(seed=3)
+    sur.load(model)
+    eval_pts = draw_points(n_eval, seed=97, grids=EVAL_GRIDS[grids])
+    res = evaluate(sur, eval_pts, window=window, log=log)
+    res["grids"] = grids
+    with open(os.path.join(outdir, "warmstart.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    log(f"[surrogate] window {window}: speedup (wall) {res['speedup_wall']:.2f}x, median steps "
+        f"{res['median_steps_plain']:.0f} -> {res['median_steps_warm']:.0f}, max field error "
+        f"{res['max_field_error']:.4f} (plain {res['max_plain_field_error']:.4f}), within 2%: warm "
+        f"{res['within_2pct']}/{n_eval}, plain {res['within_2pct_plain']}/{n_eval}, fallback {res['fallback_rate']:.2f}")
     return res
 
 
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["pipeline"])
+    ap.add_argument("cmd", choices=["pipeline", "evaluate"])
     ap.add_argument("out")
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--eval", type=int, default=32)
     ap.add_argument("--fit-steps", type=int, default=3000)
+    ap.add_argument("--model", default=None, help="evaluate: a surrogate saved by pipeline")
+    ap.add_argument("--window", type=int, default=100)
+    ap.add_argument("--grids", default="fine", choices=sorted(EVAL_GRIDS))
     a = ap.parse_args()
-    pipeline(a.out, runs=a.runs, n_eval=a.eval, fit_steps=a.fit_steps,
-             log=lambda m: print(m, flush=True))
+    say = lambda m: print(m, flush=True)
+    if a.cmd == "pipeline":
+        pipeline(a.out, runs=a.runs, n_eval=a.eval, fit_steps=a.fit_steps, grids=a.grids, log=say)
+    else:
+        evaluate_saved(a.model, a.out, n_eval=a.eval, window=a.window, grids=a.grids, log=say)
