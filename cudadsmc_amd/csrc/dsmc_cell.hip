@@ -32,6 +32,7 @@ constexpr int CELL_MAXC = 128;       // cells whose metadata is staged at once
 constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
 constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
 constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
+constexpr int SAMPLE_G = 8;          // lanes per cell in the sampling sums
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
@@ -628,20 +629,43 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             a.ncoll_cell[c] = sh.nc[i];
             tcoll += (uint32_t)sh.nc[i];
             kf.add(i, (uint32_t)sh.nc[i], 0u);
-            if (k > 0 && kf.sample(i)) {
+        }
+        // FP64 sampling sums: SAMPLE_G lanes per cell, each over every
+        // SAMPLE_G-th particle, then a fixed butterfly -- an order that
+        // depends on the cell's particle count only (not on tiles or
+        // launch settings), so every run of a case sums alike
+        {
+            const int g = lane / SAMPLE_G, sub = lane % SAMPLE_G;
+            constexpr int CPW = 64 / SAMPLE_G;
+            const size_t C = (size_t)cs.ncell;
+            for (int base = wave * CPW; base < nc; base += nwaves * CPW) {
+                const int i = base + g;
+                const int o = i < nc ? sh.st[i] : 0;
+                const int k = i < nc ? sh.st[i + 1] - o : 0;
+                const bool on = k > 0 && kf.sample(i);
                 double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
-                for (int q = 0; q < k; ++q) {
-                    const double u = (double)Vx[o + q], v = (double)Vy[o + q], w = (double)Vz[o + q];
-                    sx += u; sy += v; sz += w;
-                    s2 += u * u + v * v + w * w;
+                if (on)
+                    for (int q = sub; q < k; q += SAMPLE_G) {
+                        const double u = (double)Vx[o + q], v = (double)Vy[o + q], w = (double)Vz[o + q];
+                        sx += u; sy += v; sz += w;
+                        s2 += u * u + v * v + w * w;
+                    }
+#pragma unroll
+                for (int d = SAMPLE_G / 2; d > 0; d >>= 1) {
+                    sx += __shfl_xor(sx, d, SAMPLE_G);
+                    sy += __shfl_xor(sy, d, SAMPLE_G);
+                    sz += __shfl_xor(sz, d, SAMPLE_G);
+                    s2 += __shfl_xor(s2, d, SAMPLE_G);
                 }
-                const size_t C = (size_t)cs.ncell;
-                a.acc[c] += (double)k;
-                a.acc[C + c] += sx;
-                a.acc[2 * C + c] += sy;
-                a.acc[3 * C + c] += sz;
-                a.acc[4 * C + c] += s2;
-                a.acc[5 * C + c] += (double)sh.nc[i];
+                if (on && sub == 0) {
+                    const int c = c0 + i;
+                    a.acc[c] += (double)k;
+                    a.acc[C + c] += sx;
+                    a.acc[2 * C + c] += sy;
+                    a.acc[3 * C + c] += sz;
+                    a.acc[4 * C + c] += s2;
+                    a.acc[5 * C + c] += (double)sh.nc[i];
+                }
             }
         }
         __syncthreads();
