@@ -16,8 +16,8 @@
 //      one lane per cell; cells with up to 64 candidates run in groups of 8
 //      lanes, heavier cells by a whole wave, both in conflict-free rounds
 //      whose results equal the serial candidate loop bit for bit;
-//   4. FP64 sampling sums, one lane per cell, serial order (samplingInCells,
-//      GPUCell.cu:336-402, as time-averaged accumulators);
+//   4. FP64 sampling sums, 8 lanes per cell joined by a fixed butterfly
+//      (samplingInCells, GPUCell.cu:336-402, as time-averaged accumulators);
 //   5. velocities and cell state written back coalesced.
 // The last block to finish advances the device step control.
 #include <type_traits>
