@@ -4,11 +4,12 @@
 // occupy [p0, p0+n) of the cell-sorted order; tiles of about CELL_P particles
 // are handed out by a device-wide queue (n <= CELL_CAP: staged in LDS;
 // larger tiles stage in the destination arrays in HBM).  Per tile:
-//   1. stable order inside each cell: a cell's permutation segment is made
-//      ascending by one wave -- the scatter leaves it as a concatenation of
-//      ascending runs, one per k_move wave, so short segments are ranked run
-//      against run, long ones sorted by run key, and a block bitonic sort
-//      takes the rest; the result equals the serial reference order;
+//   1. stable order inside each cell -- the scatter leaves a cell's
+//      permutation segment as a concatenation of ascending runs, one per
+//      k_move wave, so the segment is sorted by ranking its runs: 16 lanes
+//      per cell for segments of up to 64 with up to 16 runs, one wave per
+//      cell for the rest (longer segments sorted by run key), and a block
+//      bitonic sort above 64 runs; the result equals the serial order;
 //   2. gather of the state into cell order -- the outflow compaction of
 //      GPUParticles::DeleteInvalid (cudaDSMC GPUParticle.cu:76-140);
 //   3. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
@@ -29,6 +30,7 @@ namespace dsmc {
 namespace {
 
 constexpr int CELL_MAXC = 128;       // cells whose metadata is staged at once
+constexpr int SORT_G = 16;           // lanes per cell in the grouped in-cell sort
 constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
 constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
 constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
@@ -217,6 +219,69 @@ __device__ __forceinline__ void wave_small_sort(int *S, int len, int *rtab)
     wave_sync<GLOBAL>();
     if (in) S[mydst] = v;
     wave_sync<GLOBAL>();
+}
+
+// The same ranking for many cells at once: a wave takes 64/SORT_G cells,
+// SORT_G lanes per cell, lane `sub` holding elements sub, sub + SORT_G, ...
+// of a segment of 2..64 elements.  Runs are numbered by ballots in element
+// order; run r's destination is the total length of the runs with smaller
+// key.  Cells with longer segments or more than SORT_G runs are appended to
+// list[] (count *nlist) for the one-wave sorts; rt is the wave's 64-int table.
+template <bool GLOBAL>
+__device__ void group_sort(int *S, const int *st, int nc, int *rt, int *list, int *nlist, int wave, int nwaves)
+{
+    constexpr int G = SORT_G, E = 64 / SORT_G;
+    static_assert(G <= 16 && (64 % G) == 0, "group width");
+    const int lane = lane_id(), g = lane / G, sub = lane % G, gsh = g * G;
+    const unsigned upto = (2u << sub) - 1u;
+    for (int base = wave * E; base < nc; base += nwaves * E) {
+        const int i = base + g;
+        const int o = i < nc ? st[i] : 0;
+        const int len = i < nc ? st[i + 1] - o : 0;
+        const bool take = len >= 2 && len <= 64;
+        int v[E], run[E];
+        int nr = 0;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int e = j * G + sub;
+            const bool in = take && e < len;
+            v[j] = in ? S[o + e] : 0;
+            const int pv = (in && e > 0) ? S[o + e - 1] : 0;
+            const bool s = in && (e == 0 || (v[j] >> 6) != (pv >> 6));
+            const unsigned gb = (unsigned)(__ballot(s) >> gsh) & ((1u << G) - 1u);
+            run[j] = nr + __popc(gb & upto) - 1;
+            if (s && run[j] < G) rt[gsh + run[j]] = e;
+            nr += __popc(gb);
+        }
+        if (sub == 0 && (len > 64 || (take && nr > G))) list[atomicAdd(nlist, 1)] = i;
+        const bool work = take && nr > 1 && nr <= G;
+        if (!__ballot(work)) continue;
+        wave_sync<GLOBAL>();
+        const bool hr = work && sub < nr;
+        const int rs = hr ? rt[gsh + sub] : len;
+        int re = __shfl_down(rs, 1, G);
+        if (sub + 1 >= nr) re = len;
+        const int key = hr ? (S[o + rs] >> 6) : 0x7fffffff;
+        const int rl = hr ? re - rs : 0;
+        int dst = 0;
+        const int nrw = work ? nr : 0;
+        for (int r = 0; __ballot(r < nrw); ++r) {
+            const int k2 = __shfl(key, r, G), l2 = __shfl(rl, r, G);
+            if (k2 < key) dst += l2;
+        }
+        int d[E];
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int rr = min(max(run[j], 0), G - 1);
+            d[j] = __shfl(dst, rr, G) + j * G + sub - __shfl(rs, rr, G);
+        }
+        wave_sync<GLOBAL>();
+        if (work)
+#pragma unroll
+            for (int j = 0; j < E; ++j)
+                if (j * G + sub < len) S[o + d[j]] = v[j];
+        wave_sync<GLOBAL>();
+    }
 }
 
 // Collision constants of one cell: VHS sigma*c_r = K1 (c_r^2)^e_exp, and the
@@ -517,9 +582,14 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
         const int nc = min(cb - c0, CELL_MAXC);
         for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
-        if (tid == 0) sh.nbig = 0;
+        if (tid == 0) { sh.nbig = 0; sh.nheavy = 0; }
         __syncthreads();
-        for (int i = wave; i < nc; i += nwaves) {
+        group_sort<!LDS>(Sp, sh.st, nc, sh.rstart[wave], sh.heavy, &sh.nheavy, wave, nwaves);
+        if (!LDS) __threadfence();
+        __syncthreads();
+        const int nl = sh.nheavy;
+        for (int h = wave; h < nl; h += nwaves) {
+            const int i = sh.heavy[h];
             const int o = sh.st[i], len = sh.st[i + 1] - o;
             if (len <= 1) continue;
             if (len <= 64) {
@@ -710,12 +780,16 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
     }
     uint32_t tcoll = 0, tcand = 0;
     for (;;) {
-        // dynamic tile queue: dense tiles cost more, blocks take the next free one
+        // dynamic tile queue: dense tiles cost more, blocks take the next free
+        // one, from the last cell backwards -- with the inflow at x = 0 the
+        // cheap, uniform freestream tiles are then the ones left to fill the
+        // end of the launch
         if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u);
         __syncthreads();
-        const int t = s_tile;
+        const int q = s_tile;
         __syncthreads();
-        if (t >= ntiles) break;
+        if (q >= ntiles) break;
+        const int t = ntiles - 1 - q;
         const int ca = a.tile_first[t];
         const int cb = (t + 1 < ntiles) ? a.tile_first[t + 1] : a.cs.ncell;
         if (ca >= cb) continue;
