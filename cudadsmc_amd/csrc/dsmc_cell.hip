@@ -570,9 +570,13 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
     const int nwaves = bs >> 6;
     const DevCase &cs = a.cs;
+    // a tile of one chunk loads its permutation, cell starts and collision
+    // state in one round trip (the metadata stays staged until the
+    // collisions); larger tiles stage each chunk's starts for the ordering
+    // and everything again for the collisions
+    const bool one = cb - ca <= CELL_MAXC;
     if (LDS)
         for (int k = tid; k < n; k += bs) Sp[k] = a.perm[p0 + k];
-    __syncthreads();
 
     // 1. stable order within each cell, one wave per cell: up to 64 particles
     //    sorted in registers, longer segments by their k_move runs; more than
@@ -582,6 +586,12 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
         const int nc = min(cb - c0, CELL_MAXC);
         for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
+        if (one)
+            for (int i = tid; i < nc; i += bs) {
+                sh.sm[i] = a.sigcmax[c0 + i];
+                sh.mr[i] = a.mrem[c0 + i];
+                sh.vol[i] = cs.vol_ntc[c0 + i];
+            }
         if (tid == 0) { sh.nbig = 0; sh.nheavy = 0; }
         __syncthreads();
         group_sort<!LDS>(Sp, sh.st, nc, sh.rstart[wave], sh.heavy, &sh.nheavy, wave, nwaves);
@@ -648,11 +658,13 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     // 3-4. collisions and sampling, chunk by chunk of cells
     for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
         const int nc = min(cb - c0, CELL_MAXC);
-        for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
-        for (int i = tid; i < nc; i += bs) {
-            sh.sm[i] = a.sigcmax[c0 + i];
-            sh.mr[i] = a.mrem[c0 + i];
-            sh.vol[i] = cs.vol_ntc[c0 + i];
+        if (!one) {
+            for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
+            for (int i = tid; i < nc; i += bs) {
+                sh.sm[i] = a.sigcmax[c0 + i];
+                sh.mr[i] = a.mrem[c0 + i];
+                sh.vol[i] = cs.vol_ntc[c0 + i];
+            }
         }
         kf.prepare(sh, c0, nc);
         if (tid == 0) { sh.nheavy = 0; sh.nbig = 0; }
