@@ -739,14 +739,17 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                     sz += __shfl_xor(sz, d, SAMPLE_G);
                     s2 += __shfl_xor(s2, d, SAMPLE_G);
                 }
+                // one lane per cell adds into the accumulators: the FP64
+                // adds are done at the memory side (the same IEEE sum, one
+                // writer per cell per step), so nothing waits for a load
                 if (on && sub == 0) {
                     const int c = c0 + i;
-                    a.acc[c] += (double)k;
-                    a.acc[C + c] += sx;
-                    a.acc[2 * C + c] += sy;
-                    a.acc[3 * C + c] += sz;
-                    a.acc[4 * C + c] += s2;
-                    a.acc[5 * C + c] += (double)sh.nc[i];
+                    unsafeAtomicAdd(&a.acc[c], (double)k);
+                    unsafeAtomicAdd(&a.acc[C + c], sx);
+                    unsafeAtomicAdd(&a.acc[2 * C + c], sy);
+                    unsafeAtomicAdd(&a.acc[3 * C + c], sz);
+                    unsafeAtomicAdd(&a.acc[4 * C + c], s2);
+                    unsafeAtomicAdd(&a.acc[5 * C + c], (double)sh.nc[i]);
                 }
             }
         }
