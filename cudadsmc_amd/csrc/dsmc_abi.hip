@@ -245,7 +245,7 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
     if ((e = h->alloc(&h->cellslot, N)) || (e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
         (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||
         (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, C + 1)) ||
-        (e = h->alloc(&h->tile_first, ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
+        (e = h->alloc(&h->tile_first, 2 * (size_t)ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
         (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, 6 * C)) ||
         (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
         (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||

@@ -272,7 +272,7 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
     if ((e = b->alloc(&b->cellslot, N)) || (e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
         (e = b->alloc(&b->lstart, Cs)) || (e = b->alloc(&b->tsum, b->ntile_scan)) ||
         (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, Cs + 1)) ||
-        (e = b->alloc(&b->tile_first, ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
+        (e = b->alloc(&b->tile_first, 2 * (size_t)ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
         (e = b->alloc(&b->sigcmax, Cs)) || (e = b->alloc(&b->mrem, Cs)) || (e = b->alloc(&b->acc, 6 * Cs)) ||
         (e = b->alloc(&b->ctrl, 1)) || (e = b->alloc(&b->fill_ctrl, 1)) || (e = b->alloc(&b->ictl, n)) ||
         (e = b->alloc(&b->d_cases, n)) || (e = b->alloc(&b->d_cc, n)) || (e = b->alloc(&b->d_cbase, n + 1)) ||

@@ -486,7 +486,7 @@ struct CellArgs {
     SoA A, B;
     const int *start;
     int *perm;
-    const int *tile_first;
+    const int *tile_first;   // int2 pairs (first cell, its start) per tile
     float *sigcmax, *mrem;
     int *ncoll_cell;
     double *acc;
@@ -794,21 +794,29 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
         }
     }
     uint32_t tcoll = 0, tcand = 0;
-    for (;;) {
-        // dynamic tile queue: dense tiles cost more, blocks take the next free
-        // one, from the last cell backwards -- with the inflow at x = 0 the
+    const int2 *tiles = reinterpret_cast<const int2 *>(a.tile_first);
+    for (int round = 0;; ++round) {
+        // dynamic tile queue: dense tiles cost more, blocks take the next
+        // free one (each block's first tile is its own index, no atomic),
+        // from the last cell backwards -- with the inflow at x = 0 the
         // cheap, uniform freestream tiles are then the ones left to fill the
         // end of the launch
-        if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u);
-        __syncthreads();
-        const int q = s_tile;
-        __syncthreads();
+        int q = (int)blockIdx.x;
+        if (round > 0) {
+            if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u) + (int)gridDim.x;
+            __syncthreads();
+            q = s_tile;
+            __syncthreads();
+        }
         if (q >= ntiles) break;
         const int t = ntiles - 1 - q;
-        const int ca = a.tile_first[t];
-        const int cb = (t + 1 < ntiles) ? a.tile_first[t + 1] : a.cs.ncell;
+        // tile t: cells [ca, cb), particles [p0, p0 + n), from the scatter's
+        // table of (first cell, its start) pairs
+        const int2 tf = tiles[t];
+        const int2 tn = (t + 1 < ntiles) ? tiles[t + 1] : make_int2(a.cs.ncell, n_next);
+        const int ca = tf.x, cb = tn.x;
         if (ca >= cb) continue;
-        const int p0 = a.start[ca], n = a.start[cb] - p0;
+        const int p0 = tf.y, n = tn.y - p0;
         auto run = [&](const auto &kf) {
             if (n <= CELL_CAP)
                 cell_tile<true>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll, tcand);

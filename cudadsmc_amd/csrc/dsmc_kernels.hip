@@ -201,7 +201,8 @@ __global__ __launch_bounds__(256) void k_scan(int *__restrict__ count, int *__re
 
 // ---------------------------------------------------------------------------
 // k_scatter: perm[start(cell)+slot] = i, materialise start[0..C], and record
-// for every tile of CELL_P particles the first cell that starts in it.
+// for every tile of tile_p particles the first cell that starts in it and that
+// cell's start (tile_first holds int2 pairs).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_scatter(const int2 *__restrict__ cellslot,
                                                  const int *__restrict__ lstart,
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(256) void k_scatter(const int2 *__restrict__ cellsl
         }
         int thi = st / tile_p;
         if (thi > ntiles - 1) thi = ntiles - 1;
-        for (int t = tlo; t <= thi; ++t) tile_first[t] = (int)c;
+        for (int t = tlo; t <= thi; ++t) reinterpret_cast<int2 *>(tile_first)[t] = make_int2((int)c, st);
     }
 }
 
