@@ -407,13 +407,15 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
 // The same loop by a whole wave: each round takes the longest prefix of the
 // next <= 64 candidates sharing no particle with an earlier candidate of the
 // round; (sigma c)max becomes a prefix max.  Identical results.
-template <bool GLOBAL>
+// GLOBAL: velocities in HBM; LMARK: conflict marks in LDS (else found by
+// shuffles).
+template <bool GLOBAL, bool LMARK = !GLOBAL>
 __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, float *Vy,
                           float *Vz, float &smax, int *mark)
 {
     const int lane = lane_id();
     int nc = 0;
-    if (!GLOBAL) {
+    if (LMARK) {
         // mark[q] = lowest lane of the round that uses particle q (64 = none)
         for (int q = lane; q < np; q += 64) mark[q] = 64;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -432,7 +434,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
             if (i2 >= i1) i2 = i2 + 1;
         }
         bool conf = false;
-        if (!GLOBAL) {
+        if (LMARK) {
             if (act) { atomicMin(&mark[i1], lane); atomicMin(&mark[i2], lane); }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -565,7 +567,7 @@ struct KBatch {
 template <bool LDS, class KF>
 __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const SoA &nxt, uint32_t step,
                           int ca, int cb, int p0, int n, int *Sp, float *Vx, float *Vy, float *Vz,
-                          CellShared &sh, uint32_t &tcoll, uint32_t &tcand)
+                          CellShared &sh, uint32_t &tcoll, uint32_t &tcand, int *lmarks, int lmark_cap)
 {
     const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
     const int nwaves = bs >> 6;
@@ -696,9 +698,18 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             const int i = sh.heavy[h];
             const int o = sh.st[i], k = sh.st[i + 1] - o;
             float smax = sh.sm[i];
-            // the cell's permutation segment is free after the gather: marks
-            const int ncoll = ntc_rounds<!LDS>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o,
-                                               smax, Sp + o);
+            // conflict marks: the cell's permutation segment, free after the
+            // gather (LDS tiles); for a single cell staged in HBM, the block's
+            // idle LDS staging area when the cell fits, else shuffles
+            int ncoll;
+            if (LDS)
+                ncoll = ntc_rounds<false>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax, Sp + o);
+            else if (lmarks && k <= lmark_cap)
+                ncoll = ntc_rounds<true, true>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax,
+                                               lmarks);
+            else
+                ncoll = ntc_rounds<true, false>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o,
+                                                smax, Sp + o);
             if (lane == 0) { sh.sm[i] = smax; sh.nc[i] = ncoll; }
         }
         if (!LDS) __threadfence();
@@ -771,8 +782,11 @@ struct NoBatch {};
 template <bool BATCH>
 __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
 {
-    __shared__ int s_perm[CELL_CAP];
-    __shared__ float s_vx[CELL_CAP], s_vy[CELL_CAP], s_vz[CELL_CAP];
+    // LDS staging of a tile: permutation and velocities; for a single cell
+    // staged in HBM it holds the collision conflict marks instead
+    __shared__ union { int i[4 * CELL_CAP]; float f[4 * CELL_CAP]; } stage;
+    int *const s_perm = stage.i;
+    float *const s_vx = stage.f + CELL_CAP, *const s_vy = stage.f + 2 * CELL_CAP, *const s_vz = stage.f + 3 * CELL_CAP;
     __shared__ CellShared sh;
     __shared__ typename std::conditional<BATCH, BatchShared, NoBatch>::type bsh;
     Ctrl *ctrl = a.ctrl;
@@ -795,34 +809,57 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
     }
     uint32_t tcoll = 0, tcand = 0;
     const int2 *tiles = reinterpret_cast<const int2 *>(a.tile_first);
+    int rest_c = 0, rest_cb = 0;      // the part of a cut tile still to do
     for (int round = 0;; ++round) {
-        // dynamic tile queue: dense tiles cost more, blocks take the next
-        // free one (each block's first tile is its own index, no atomic),
-        // from the last cell backwards -- with the inflow at x = 0 the
-        // cheap, uniform freestream tiles are then the ones left to fill the
-        // end of the launch
-        int q = (int)blockIdx.x;
-        if (round > 0) {
-            if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u) + (int)gridDim.x;
-            __syncthreads();
-            q = s_tile;
-            __syncthreads();
+        int ca, cb, p0, n;
+        if (rest_c < rest_cb) {
+            ca = rest_c; cb = rest_cb;
+            p0 = a.start[ca]; n = a.start[cb] - p0;
+        } else {
+            // dynamic tile queue: dense tiles cost more, blocks take the
+            // next free one (each block's first tile is its own index, no
+            // atomic), from the last cell backwards -- with the inflow at
+            // x = 0 the cheap, uniform freestream tiles are then the ones
+            // left to fill the end of the launch
+            int q = (int)blockIdx.x;
+            if (round > 0) {
+                if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u) + (int)gridDim.x;
+                __syncthreads();
+                q = s_tile;
+                __syncthreads();
+            }
+            if (q >= ntiles) break;
+            const int t = ntiles - 1 - q;
+            // tile t: cells [ca, cb), particles [p0, p0 + n), from the
+            // scatter's table of (first cell, its start) pairs
+            const int2 tf = tiles[t];
+            const int2 tn = (t + 1 < ntiles) ? tiles[t + 1] : make_int2(a.cs.ncell, n_next);
+            ca = tf.x; cb = tn.x;
+            p0 = tf.y; n = tn.y - p0;
         }
-        if (q >= ntiles) break;
-        const int t = ntiles - 1 - q;
-        // tile t: cells [ca, cb), particles [p0, p0 + n), from the scatter's
-        // table of (first cell, its start) pairs
-        const int2 tf = tiles[t];
-        const int2 tn = (t + 1 < ntiles) ? tiles[t + 1] : make_int2(a.cs.ncell, n_next);
-        const int ca = tf.x, cb = tn.x;
+        rest_c = rest_cb = 0;
         if (ca >= cb) continue;
-        const int p0 = tf.y, n = tn.y - p0;
+        if (n > CELL_CAP) {
+            // beyond the LDS capacity: this round takes the longest run of
+            // whole cells that fits (at least one cell; a single cell above
+            // CELL_CAP stages in HBM), the rest comes next round
+            const int m = min(cb - ca, CELL_MAXC);
+            for (int i = tid; i <= m; i += blockDim.x) sh.st[i] = a.start[ca + i];
+            __syncthreads();
+            const int fit = __syncthreads_count(tid >= 1 && tid <= m && sh.st[tid] - p0 <= CELL_CAP);
+            const int c2 = ca + (fit > 0 ? fit : 1);
+            n = sh.st[c2 - ca] - p0;
+            __syncthreads();
+            rest_c = c2; rest_cb = cb;
+            cb = c2;
+        }
         auto run = [&](const auto &kf) {
             if (n <= CELL_CAP)
-                cell_tile<true>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll, tcand);
-            else
+                cell_tile<true>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll, tcand,
+                                nullptr, 0);
+            else    // a single cell: its collision rounds alone use the idle staging LDS for marks
                 cell_tile<false>(a, kf, cur, nxt, step, ca, cb, p0, n, a.perm + p0, nxt.vx + p0, nxt.vy + p0,
-                                 nxt.vz + p0, sh, tcoll, tcand);
+                                 nxt.vz + p0, sh, tcoll, tcand, stage.i, 4 * CELL_CAP);
         };
         if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step });
         else run(KSingle{ a.cs, sampling });

@@ -171,6 +171,45 @@ def test_heavy_cell_and_empty(gpu_lib):
     r.close()
 
 
+def test_tile_cut_into_lds_runs(gpu_lib):
+    """A tile of several cells that together exceed the LDS capacity is cut
+    into runs of whole cells that fit (cells of 700, 700 | 700 particles),
+    next to a single 2,500-particle cell that stages in HBM and a small
+    tile: order, particles and collision state bit-exact against the oracle."""
+    p, body = T.small_box(8, 1, seed=3)
+    p.n_inf = 1.35e16
+    oc, info, tables = O.case_from_params(p, body)
+    rng = np.random.default_rng(9)
+    dx = np.float32(info.dx)
+    xs, ys = [], []
+    for j, cnt in ((2, 700), (3, 700), (4, 700), (5, 2500), (6, 300)):
+        xs.append((rng.random(cnt, dtype=np.float32) * dx * 0.99).astype(np.float32) + 3 * dx)
+        ys.append((rng.random(cnt, dtype=np.float32) * dx * 0.99).astype(np.float32) + j * dx)
+    n = sum(len(v) for v in xs)
+    # in-plane speeds tiny so that every particle stays in its cell
+    parts = dict(x=np.concatenate(xs), y=np.concatenate(ys),
+                 vx=rng.normal(0, 1e-3, n).astype(np.float32), vy=rng.normal(0, 1e-3, n).astype(np.float32),
+                 vz=rng.normal(0, 300, n).astype(np.float32), pid=np.arange(n, dtype=np.uint32))
+    r = Run(p, body)
+    r.set_launch(cell_tile=2048)
+    r.upload(*[parts[k] for k in ("x", "y", "vx", "vy", "vz")], parts["pid"])
+    r.set_keep_perm(True)
+    sm, mr, _ = r.cell_state()
+    ref = T.oracle_phases(oc, parts, 0, n, sm, mr, int(info.capacity))
+    r.particlemove()
+    r.ressignParticle()
+    r.collision()
+    np.testing.assert_array_equal(r.perm(), ref["perm"])
+    T.assert_same_particles(r.download(), ref["new"], what="cut tile ")
+    sm_g, mr_g, nc_g = r.cell_state()
+    np.testing.assert_array_equal(sm_g, ref["sigcmax"])
+    np.testing.assert_array_equal(nc_g, ref["ncoll"])
+    counts = np.diff(ref["start"])
+    assert sorted(counts[counts > 0].tolist()) == [300, 700, 700, 700, 2500]
+    assert ref["ncoll"].max() > 10
+    r.close()
+
+
 def test_box_full_rate_and_conservation(gpu_lib):
     """Closed box at equilibrium, full size (64x64, 81,920 particles, 1000
     steps): collision rate against VHS theory, energy and particle
