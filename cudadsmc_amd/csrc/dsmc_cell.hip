@@ -7,9 +7,10 @@
 //   1. stable order inside each cell -- the scatter leaves a cell's
 //      permutation segment as a concatenation of ascending runs, one per
 //      k_move wave, so the segment is sorted by ranking its runs: 16 lanes
-//      per cell for segments of up to 64 with up to 16 runs, one wave per
-//      cell for the rest (longer segments sorted by run key), and a block
-//      bitonic sort above 64 runs; the result equals the serial order;
+//      per cell for segments of up to 64 with up to 16 runs, 32 lanes for
+//      up to 128 with up to 32 runs, one wave per cell for the rest (longer
+//      segments sorted by run key), and a block bitonic sort above 64 runs;
+//      the result equals the serial order;
 //   2. gather of the state into cell order -- the outflow compaction of
 //      GPUParticles::DeleteInvalid (cudaDSMC GPUParticle.cu:76-140);
 //   3. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
@@ -30,7 +31,6 @@ namespace dsmc {
 namespace {
 
 constexpr int CELL_MAXC = 128;       // cells whose metadata is staged at once
-constexpr int SORT_G = 16;           // lanes per cell in the grouped in-cell sort
 constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
 constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
 constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
@@ -45,7 +45,7 @@ struct CellShared {
     int big[CELL_MAXC], heavy[CELL_MAXC];
     int rstart[CELL_WAVES][64], rdst[CELL_WAVES][64];   // per-wave run tables
     int inst[CELL_MAXC];             // batched runs: instance of each staged cell
-    int nbig, nheavy;
+    int nbig, nheavy, nrest;
     uint32_t coll, cand;
 };
 
@@ -221,24 +221,28 @@ __device__ __forceinline__ void wave_small_sort(int *S, int len, int *rtab)
     wave_sync<GLOBAL>();
 }
 
-// The same ranking for many cells at once: a wave takes 64/SORT_G cells,
-// SORT_G lanes per cell, lane `sub` holding elements sub, sub + SORT_G, ...
-// of a segment of 2..64 elements.  Runs are numbered by ballots in element
-// order; run r's destination is the total length of the runs with smaller
-// key.  Cells with longer segments or more than SORT_G runs are appended to
-// list[] (count *nlist) for the one-wave sorts; rt is the wave's 64-int table.
-template <bool GLOBAL>
-__device__ void group_sort(int *S, const int *st, int nc, int *rt, int *list, int *nlist, int wave, int nwaves)
+// The same ranking for many cells at once: a wave takes 64/G cells, G lanes
+// per cell, lane `sub` holding elements sub, sub + G, ... of a segment of up
+// to E*G elements.  Runs are numbered by ballots in element order; run r's
+// destination is the total length of the runs with smaller key.  The cells
+// are [0, count) or, FROMLIST, cells[0 .. count).  Cells with longer segments
+// or more than G runs are appended to out[] (count *nout) for the next pass;
+// rt is the wave's 64-int table.
+template <int G, int E, bool FROMLIST, bool GLOBAL>
+__device__ void group_sort(int *S, const int *st, const int *cells, int count, int *rt, int *out, int *nout,
+                           int wave, int nwaves)
 {
-    constexpr int G = SORT_G, E = 64 / SORT_G;
-    static_assert(G <= 16 && (64 % G) == 0, "group width");
+    static_assert(G <= 32 && (64 % G) == 0, "group width");
+    constexpr int CPW = 64 / G, CAP = E * G;
     const int lane = lane_id(), g = lane / G, sub = lane % G, gsh = g * G;
-    const unsigned upto = (2u << sub) - 1u;
-    for (int base = wave * E; base < nc; base += nwaves * E) {
-        const int i = base + g;
-        const int o = i < nc ? st[i] : 0;
-        const int len = i < nc ? st[i + 1] - o : 0;
-        const bool take = len >= 2 && len <= 64;
+    const unsigned long long gbits = (1ull << G) - 1ull;
+    const unsigned long long upto = (2ull << sub) - 1ull;
+    for (int base = wave * CPW; base < count; base += nwaves * CPW) {
+        const int k = base + g;
+        const int i = k < count ? (FROMLIST ? cells[k] : k) : 0;
+        const int o = k < count ? st[i] : 0;
+        const int len = k < count ? st[i + 1] - o : 0;
+        const bool take = len >= 2 && len <= CAP;
         int v[E], run[E];
         int nr = 0;
 #pragma unroll
@@ -248,12 +252,12 @@ __device__ void group_sort(int *S, const int *st, int nc, int *rt, int *list, in
             v[j] = in ? S[o + e] : 0;
             const int pv = (in && e > 0) ? S[o + e - 1] : 0;
             const bool s = in && (e == 0 || (v[j] >> 6) != (pv >> 6));
-            const unsigned gb = (unsigned)(__ballot(s) >> gsh) & ((1u << G) - 1u);
-            run[j] = nr + __popc(gb & upto) - 1;
+            const unsigned long long gb = (__ballot(s) >> gsh) & gbits;
+            run[j] = nr + __popcll(gb & upto) - 1;
             if (s && run[j] < G) rt[gsh + run[j]] = e;
-            nr += __popc(gb);
+            nr += __popcll(gb);
         }
-        if (sub == 0 && (len > 64 || (take && nr > G))) list[atomicAdd(nlist, 1)] = i;
+        if (sub == 0 && (len > CAP || (take && nr > G))) out[atomicAdd(nout, 1)] = i;
         const bool work = take && nr > 1 && nr <= G;
         if (!__ballot(work)) continue;
         wave_sync<GLOBAL>();
@@ -594,14 +598,23 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 sh.mr[i] = a.mrem[c0 + i];
                 sh.vol[i] = cs.vol_ntc[c0 + i];
             }
-        if (tid == 0) { sh.nbig = 0; sh.nheavy = 0; }
+        if (tid == 0) { sh.nbig = 0; sh.nheavy = 0; sh.nrest = 0; }
         __syncthreads();
-        group_sort<!LDS>(Sp, sh.st, nc, sh.rstart[wave], sh.heavy, &sh.nheavy, wave, nwaves);
+        // segments up to 64 with up to 16 runs: 16-lane groups; up to 128
+        // with up to 32 runs: 32-lane groups; the rest one wave per cell
+        group_sort<16, 4, false, !LDS>(Sp, sh.st, nullptr, nc, sh.rstart[wave], sh.heavy, &sh.nheavy, wave,
+                                       nwaves);
         if (!LDS) __threadfence();
         __syncthreads();
-        const int nl = sh.nheavy;
+        const int n2 = sh.nheavy;
+        if (n2 > 0) {
+            group_sort<32, 4, true, !LDS>(Sp, sh.st, sh.heavy, n2, sh.rstart[wave], sh.M, &sh.nrest, wave, nwaves);
+            if (!LDS) __threadfence();
+            __syncthreads();
+        }
+        const int nl = sh.nrest;
         for (int h = wave; h < nl; h += nwaves) {
-            const int i = sh.heavy[h];
+            const int i = sh.M[h];
             const int o = sh.st[i], len = sh.st[i + 1] - o;
             if (len <= 1) continue;
             if (len <= 64) {
