@@ -311,21 +311,32 @@ __device__ __forceinline__ int ntc_count(float ntc_coef, int np, float vol, floa
 
 // Post-collision velocities: isotropic (VHS) scattering of the relative
 // velocity about the centre of mass (Particle.cpp:84-100, GPUCell.cu:278-310).
-__device__ __forceinline__ void scatter_pair(const CK &k, uint32_t step, int m, uint32_t w3,
-                                             float g2, float &v1x, float &v1y, float &v1z, float &v2x,
-                                             float &v2y, float &v2z)
+// The scattered relative velocity depends only on the candidate's random
+// numbers and |g|, not on whether it is accepted, so the rounds compute it
+// before the acceptance test (it then overlaps the cross-section chain);
+// scatter_pair applies it to an accepted pair.
+struct Scatter {
+    float rx, ry, rz;
+};
+
+__device__ __forceinline__ Scatter scatter_dir(const CK &k, uint32_t step, int m, uint32_t w3, float g2)
 {
-    float cosr = 2.0f * u01(w3) - 1.0f;
-    float sinr = sqrtf(1.0f - cosr * cosr);
-    uint4 w2 = philox((uint32_t)m, step, SUB_COLLIDE, 1u, (uint32_t)k.lc, k.seed);
+    const float cosr = 2.0f * u01(w3) - 1.0f;
+    const float sinr = sqrtf(1.0f - cosr * cosr);
+    const uint4 w2 = philox((uint32_t)m, step, SUB_COLLIDE, 1u, (uint32_t)k.lc, k.seed);
     float sp, cp;
     dm_sincos2pi(u01(w2.x), sp, cp);
-    float g = sqrtf(g2);
-    float rx = g * cosr, ry = g * (sinr * cp), rz = g * (sinr * sp);
-    float mx = 0.5f * (v1x + v2x), my = 0.5f * (v1y + v2y), mz = 0.5f * (v1z + v2z);
-    v1x = mx + 0.5f * rx; v2x = mx - 0.5f * rx;
-    v1y = my + 0.5f * ry; v2y = my - 0.5f * ry;
-    v1z = mz + 0.5f * rz; v2z = mz - 0.5f * rz;
+    const float g = sqrtf(g2);
+    return Scatter{ g * cosr, g * (sinr * cp), g * (sinr * sp) };
+}
+
+__device__ __forceinline__ void scatter_pair(const Scatter &r, float &v1x, float &v1y, float &v1z, float &v2x,
+                                             float &v2y, float &v2z)
+{
+    const float mx = 0.5f * (v1x + v2x), my = 0.5f * (v1y + v2y), mz = 0.5f * (v1z + v2z);
+    v1x = mx + 0.5f * r.rx; v2x = mx - 0.5f * r.rx;
+    v1y = my + 0.5f * r.ry; v2y = my - 0.5f * r.ry;
+    v1z = mz + 0.5f * r.rz; v2z = mz - 0.5f * r.rz;
 }
 
 __device__ __forceinline__ float vhs_sigc(const CK &k, float g2)
@@ -378,12 +389,14 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
             const bool proc = sub < f;
             float sc = 0.0f, g2 = 0.0f;
             float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
+            Scatter sd{ 0.f, 0.f, 0.f };
             if (proc) {
                 v1x = vx[i1]; v1y = vy[i1]; v1z = vz[i1];
                 v2x = vx[i2]; v2y = vy[i2]; v2z = vz[i2];
                 const float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
                 g2 = gx * gx + gy * gy + gz * gz;
                 sc = vhs_sigc(k, g2);
+                sd = scatter_dir(k, step, m, w.w, g2);
             }
             float pm = sc;
 #pragma unroll
@@ -394,7 +407,7 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
             const float sml = fmaxf(smax, pm);
             const bool accd = proc && (u01(w.z) < sc / sml);
             if (accd) {
-                scatter_pair(k, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+                scatter_pair(sd, v1x, v1y, v1z, v2x, v2y, v2z);
                 vx[i1] = v1x; vy[i1] = v1y; vz[i1] = v1z;
                 vx[i2] = v2x; vy[i2] = v2y; vz[i2] = v2z;
             }
@@ -457,12 +470,14 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
         const bool proc = lane < f;
         float sc = 0.0f, g2 = 0.0f;
         float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
+        Scatter sd{ 0.f, 0.f, 0.f };
         if (proc) {
             v1x = Vx[i1]; v1y = Vy[i1]; v1z = Vz[i1];
             v2x = Vx[i2]; v2y = Vy[i2]; v2z = Vz[i2];
             float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
             g2 = gx * gx + gy * gy + gz * gz;
             sc = vhs_sigc(k, g2);
+            sd = scatter_dir(k, step, m, w.w, g2);
         }
         float pm = sc;
 #pragma unroll
@@ -473,7 +488,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
         const float sml = fmaxf(smax, pm);
         const bool accd = proc && (u01(w.z) < sc / sml);
         if (accd) {
-            scatter_pair(k, step, m, w.w, g2, v1x, v1y, v1z, v2x, v2y, v2z);
+            scatter_pair(sd, v1x, v1y, v1z, v2x, v2y, v2z);
             Vx[i1] = v1x; Vy[i1] = v1y; Vz[i1] = v1z;
             Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
         }
