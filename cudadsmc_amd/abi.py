@@ -126,6 +126,9 @@ SIGNATURES = {
     "dsmc_batch_reset_samples": [_h],
     "dsmc_batch_download_samples": [_h, C.c_int, _f64p, _P(C.c_uint64)],
     "dsmc_batch_download_fields": [_h, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],
+    "dsmc_batch_download_counts": [_h, _f64p, _P(C.c_uint64)],
+    "dsmc_batch_wall_distance": [_h, _f64p],
+    "dsmc_batch_compact": [_h, _i32p, C.c_int],
     "dsmc_batch_set_timing": [_h, C.c_int],
     "dsmc_batch_kernel_times": [_h, _f64p, _f64p, _P(C.c_int64)],
     "dsmc_get_launch": [_h, _P(dsmc_launch)],

@@ -35,14 +35,29 @@ class Batch:
                 B[i] = abi.dsmc_body(xy.shape[0], xy.ctypes.data_as(C.POINTER(C.c_double)))
         self.h = C.c_void_p()
         check(self.lib.dsmc_batch_create(P, B, n, C.byref(self.h)))
-        self.n = n
+        self.origin = list(range(n))     # index each instance had when the batch was made
+        self._describe()
+
+    def _describe(self):
+        self.n = len(self.params)
         self.info = []
-        for i in range(n):
+        for i in range(self.n):
             inf = abi.dsmc_case_info()
             check(self.lib.dsmc_batch_case_info(self.h, i, C.byref(inf)))
             self.info.append(inf)
         self.ncells = [inf.ncell for inf in self.info]
         self.cbase = np.concatenate([[0], np.cumsum(self.ncells)]).astype(np.int64)
+
+    def compact(self, keep):
+        """Keeps the instances listed in keep (current indices, ascending)
+        and frees the rest; each kept one continues bit-identically.
+        self.origin maps the new indices to the ones the batch was made with."""
+        keep = [int(k) for k in keep]
+        arr = np.ascontiguousarray(keep, dtype=np.int32)
+        check(self.lib.dsmc_batch_compact(self.h, ptr(arr, C.c_int), len(keep)))
+        self.params = [self.params[k] for k in keep]
+        self.origin = [self.origin[k] for k in keep]
+        self._describe()
 
     # -- lifecycle (Run::initialize / finalize) --
     def initialize(self):
@@ -111,6 +126,21 @@ class Batch:
         ns = C.c_uint64()
         check(self.lib.dsmc_batch_download_samples(self.h, int(i), ptr(acc, C.c_double), C.byref(ns)))
         return acc.reshape(6, -1), ns.value
+
+    def counts(self):
+        """Particle-count sums of every cell of every instance (instance
+        order, one transfer) and the per-instance sampled-step counts."""
+        out = np.empty(int(self.cbase[-1]), np.float64)
+        ns = np.empty(self.n, np.uint64)
+        check(self.lib.dsmc_batch_download_counts(self.h, ptr(out, C.c_double), ptr(ns, C.c_uint64)))
+        return out, ns
+
+    def wall_distance(self):
+        """Signed distance (m) of every cell centre to its instance's body
+        outline, negative inside (instance order; +inf without a body)."""
+        out = np.empty(int(self.cbase[-1]), np.float64)
+        check(self.lib.dsmc_batch_wall_distance(self.h, ptr(out, C.c_double)))
+        return out
 
     def fields(self, i):
         c = self.ncells[i]

@@ -29,6 +29,8 @@ struct dsmc_batch_s {
     std::vector<DevCase> cs;           // per instance, device table pointers set
     std::vector<int> cbase;            // ninst + 1
     std::vector<int64_t> capi;         // per-instance capacity
+    std::vector<double> poly;          // body outlines of all instances, x y pairs
+    std::vector<int> poly0, npoly;     // per instance: first vertex in poly, vertex count
     int ncell = 0, ntile_scan = 0;
     int64_t cap = 0;
     SoA A{}, B{};
@@ -238,6 +240,9 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
         const dsmc_body *bd = (bodies && bodies[i].n_points > 0) ? &bodies[i] : nullptr;
         msg = build_cut_tables(b->P[i], bd, b->T[i]);
         if (!msg.empty()) { delete b; return fail(DSMC_ERR_GEOMETRY, "instance " + std::to_string(i) + ": " + msg); }
+        b->poly0.push_back((int)(b->poly.size() / 2));
+        b->npoly.push_back(bd ? bd->n_points : 0);
+        if (bd) b->poly.insert(b->poly.end(), bd->xy, bd->xy + 2 * (size_t)bd->n_points);
         b->capi[i] = default_capacity(b->P[i], b->T[i]);
         b->cap += b->capi[i];
         C += b->T[i].ncell;
@@ -554,6 +559,137 @@ int dsmc_batch_download_samples(dsmc_batch b, int i, double *acc, uint64_t *nsam
         for (int k = 0; k < 6; ++k)
             HIPCHK(hipMemcpy(acc + k * Ci, b->acc + k * C + cb, sizeof(double) * Ci, hipMemcpyDeviceToHost));
     }
+    return DSMC_OK;
+}
+
+int dsmc_batch_download_counts(dsmc_batch b, double *counts, uint64_t *nsamples)
+{
+    if (!b || !counts) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(b);
+    if (r) return r;
+    std::vector<InstCtrl> ic(b->ninst);
+    HIPCHK(hipMemcpyAsync(counts, b->acc, sizeof(double) * (size_t)b->ncell, hipMemcpyDeviceToHost, b->stream));
+    if (nsamples)
+        HIPCHK(hipMemcpyAsync(ic.data(), b->ictl, sizeof(InstCtrl) * b->ninst, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (nsamples)
+        for (int i = 0; i < b->ninst; ++i) nsamples[i] = ic[i].nsamples;
+    return DSMC_OK;
+}
+
+int dsmc_batch_wall_distance(dsmc_batch b, double *d)
+{
+    if (!b || !d) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(b);
+    if (r) return r;
+    std::vector<WallInst> wi(b->ninst);
+    int maxc = 1;
+    for (int i = 0; i < b->ninst; ++i) {
+        const dsmc_params &p = b->P[i].in;
+        wi[i] = WallInst{ b->cbase[i], b->T[i].ncell, p.N2, b->poly0[i], b->npoly[i], p.L1 / p.N1, p.L2 / p.N2 };
+        maxc = std::max(maxc, b->T[i].ncell);
+    }
+    WallInst *dw = nullptr;
+    double *dp = nullptr, *dd = nullptr;
+    hipError_t e = dalloc(&dw, wi.size());
+    if (e == hipSuccess) e = dalloc(&dp, b->poly.size());
+    if (e == hipSuccess) e = dalloc(&dd, (size_t)b->ncell);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dw, wi.data(), sizeof(WallInst) * wi.size(), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess && !b->poly.empty())
+        e = hipMemcpyAsync(dp, b->poly.data(), sizeof(double) * b->poly.size(), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = launch_wall_distance(dw, b->ninst, maxc, dp, dd, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d, dd, sizeof(double) * (size_t)b->ncell, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dw);
+    (void)hipFree(dp);
+    (void)hipFree(dd);
+    if (e != hipSuccess) return fail(DSMC_ERR_HIP, std::string("wall distance: ") + hipGetErrorString(e));
+    return DSMC_OK;
+}
+
+// Keeps the instances keep[0..nkeep) (ascending) and drops the rest: a new
+// set of device arrays sized for the kept instances receives their particle
+// ranges, per-cell state (sigcmax, NTC remainders, collision counts, sample
+// sums) and counters, then takes the handle's place.  Random streams are
+// keyed on step, particle id and local cell id, none of which change, so a
+// kept instance continues bit-identically.
+int dsmc_batch_compact(dsmc_batch b, const int *keep, int nkeep)
+{
+    if (!b || !keep || nkeep < 1 || nkeep > b->ninst) return fail(DSMC_ERR_ARG, "compact: 1 <= nkeep <= n");
+    for (int k = 0; k < nkeep; ++k)
+        if (keep[k] < 0 || keep[k] >= b->ninst || (k > 0 && keep[k] <= keep[k - 1]))
+            return fail(DSMC_ERR_ARG, "compact: keep must list instance indices in ascending order");
+    int r = set_device(b);
+    if (r) return r;
+    Ctrl c;
+    std::vector<InstCtrl> ic;
+    if ((r = read_ctrl(b, c, &ic))) return r;
+    if (nkeep == b->ninst) return DSMC_OK;
+    std::vector<dsmc_params> P(nkeep);
+    std::vector<dsmc_body> B(nkeep);
+    for (int k = 0; k < nkeep; ++k) {
+        const int i = keep[k];
+        P[k] = b->P[i].in;
+        B[k] = dsmc_body{ b->npoly[i], b->npoly[i] ? b->poly.data() + 2 * (size_t)b->poly0[i] : nullptr };
+    }
+    dsmc_batch nb = nullptr;
+    if ((r = dsmc_batch_create(P.data(), B.data(), nkeep, &nb))) return r;
+    nb->grid_move = b->grid_move;
+    nb->grid_scatter = b->grid_scatter;
+    nb->grid_cell = b->grid_cell;
+    nb->tile_p = b->tile_p;
+    nb->graph_len = b->graph_len;
+    nb->timing = b->timing;
+
+    const SoA &So = (c.step & 1u) ? b->B : b->A;
+    const SoA &Sn = (c.step & 1u) ? nb->B : nb->A;
+    std::vector<size_t> off(b->ninst + 1, 0);
+    for (int i = 0; i < b->ninst; ++i) off[i + 1] = off[i] + ic[i].n_live;
+    const size_t Co = b->ncell, Cn = nb->ncell;
+    hipStream_t st = nb->stream;
+    Ctrl g = c;
+    g.psteps = g.ncoll = g.ncand = 0ull;
+    g.last_coll = g.last_cand = 0u;
+    std::vector<InstCtrl> nic(nkeep);
+    size_t dst = 0;
+    hipError_t e = hipSuccess;
+    auto cp = [&](void *to, const void *from, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(to, from, bytes, hipMemcpyDeviceToDevice, st);
+    };
+    for (int k = 0; k < nkeep; ++k) {
+        const int i = keep[k];
+        const size_t n = ic[i].n_live, s = off[i];
+        cp(Sn.x + dst, So.x + s, sizeof(float) * n);
+        cp(Sn.y + dst, So.y + s, sizeof(float) * n);
+        cp(Sn.vx + dst, So.vx + s, sizeof(float) * n);
+        cp(Sn.vy + dst, So.vy + s, sizeof(float) * n);
+        cp(Sn.vz + dst, So.vz + s, sizeof(float) * n);
+        cp(Sn.pid + dst, So.pid + s, sizeof(uint32_t) * n);
+        dst += n;
+        const size_t co = b->cbase[i], cn = nb->cbase[k], Ci = b->T[i].ncell;
+        cp(nb->sigcmax + cn, b->sigcmax + co, sizeof(float) * Ci);
+        cp(nb->mrem + cn, b->mrem + co, sizeof(float) * Ci);
+        cp(nb->ncoll_cell + cn, b->ncoll_cell + co, sizeof(int) * Ci);
+        for (int q = 0; q < 6; ++q) cp(nb->acc + q * Cn + cn, b->acc + q * Co + co, sizeof(double) * Ci);
+        nic[k] = ic[i];
+        g.psteps += ic[i].psteps;
+        g.ncoll += ic[i].ncoll;
+        g.ncand += ic[i].ncand;
+        g.last_coll += ic[i].last_coll;
+        g.last_cand += ic[i].last_cand;
+    }
+    g.n_live = g.n_move = g.n_next = (uint32_t)dst;
+    if (e == hipSuccess) e = hipMemcpyAsync(nb->ictl, nic.data(), sizeof(InstCtrl) * nkeep, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(nb->ctrl, &g, sizeof(Ctrl), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        dsmc_batch_destroy(nb);
+        return fail(DSMC_ERR_HIP, std::string("compact: ") + hipGetErrorString(e));
+    }
+    std::swap(*b, *nb);
+    dsmc_batch_destroy(nb);
     return DSMC_OK;
 }
 

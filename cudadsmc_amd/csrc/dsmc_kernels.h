@@ -38,6 +38,16 @@ hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, 
                        int *tsum, int *toff, Ctrl *ctrl, int grid, hipStream_t st);
 hipError_t launch_ctrl_reset(Ctrl *ctrl, uint32_t n, uint32_t next_pid, hipStream_t st);
 
+// One instance of a wall-distance launch: its cells start at cbase in the
+// output; cell c = j + i*ny has its centre at ((i + 0.5) dx, (j + 0.5) dy);
+// its outline is poly[2*poly0 .. 2*(poly0 + npoly)) (x, y pairs).
+struct WallInst {
+    int cbase, ncell, ny, poly0, npoly;
+    double dx, dy;
+};
+hipError_t launch_wall_distance(const WallInst *inst, int ninst, int max_cells, const double *poly, double *out,
+                                hipStream_t st);
+
 // Resident blocks per CU of the persistent kernels (occupancy query).
 int occupancy_move();
 int occupancy_move_batch();

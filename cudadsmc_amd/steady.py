@@ -16,6 +16,10 @@ several steps), which a Poisson count model misses.  For a steady flow
 chi is about 1; when a drift r per step dominates the noise, d ~ k rW and
 s ~ rW/2, so chi approaches 2 sqrt(2) k.  The lag k makes slow drifts --
 smaller than the noise from one window to the next, but adding up -- count.
+Until k windows lie behind a window, it is compared with the first one
+(lag w - 1 at window w), so a run that is steady from its start is
+confirmed after patience + 1 windows rather than k + patience; from window
+k + 1 on every chi uses the full lag.
 The detector declares a run steady at the first window that completes
 `patience` consecutive windows with chi below `threshold`.
 """
@@ -42,8 +46,8 @@ class SteadyDetector:
         m = 0.5 * (a + b)
         s2 = float(np.mean((b[ok] - a[ok]) ** 2)) if ok.any() else 0.0
         self.windows += 1
-        if len(self.past) >= self.lag:
-            pm, ps2, pok = self.past[-self.lag]
+        if self.past:
+            pm, ps2, pok = self.past[-min(self.lag, len(self.past))]
             both = ok & pok
             d2 = float(np.mean((m[both] - pm[both]) ** 2)) if both.any() else float("inf")
             noise = 0.5 * (s2 + ps2)

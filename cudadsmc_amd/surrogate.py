@@ -72,15 +72,17 @@ def _signed_distance(px, py, poly):
     return np.where(inside, -d, d)
 
 
-def cell_features(pt, cfg):
-    """Features of every cell of the case, in cell order c = j + i*ny."""
+def cell_features(pt, cfg, wall=None):
+    """Features of every cell of the case, in cell order c = j + i*ny.
+    wall: the cells' signed distances to the body (m) when already known
+    (Batch.wall_distance computes them on the device); else computed here."""
     p = cfg["params"]
     nx, ny = p.N1, p.N2
     i, j = np.divmod(np.arange(nx * ny), ny)
     x = (i + 0.5) * (p.L1 / nx)
     y = (j + 0.5) * (p.L2 / ny)
     geom = -1.0 if pt["geometry"] == "cylinder" else pt["half_angle_deg"] / 30.0
-    wd = _signed_distance(x, y, cfg["body"]) / p.L2
+    wd = (_signed_distance(x, y, cfg["body"]) if wall is None else np.asarray(wall, np.float64)) / p.L2
     C = nx * ny
     return np.stack([x / p.L1, y / p.L2, np.full(C, pt["mach"]), np.full(C, math.log10(pt["kn"])),
                      np.full(C, pt["tw_ratio"]), np.full(C, geom), wd], 1).astype(np.float32)
@@ -170,6 +172,15 @@ class Surrogate:
     def fields_for(self, pt, cfg):
         return fields_from_targets(cfg, self.predict(cell_features(pt, cfg)))
 
+    def fields_batch(self, sim, pts, cfgs):
+        """Predicted (n, ux, uy, T) of every instance of a Batch: wall
+        distances from the device, one forward pass over all cells."""
+        wall = sim.wall_distance()
+        cb = sim.cbase
+        X = np.concatenate([cell_features(pt, c, wall[cb[i]:cb[i + 1]]) for i, (pt, c) in enumerate(zip(pts, cfgs))])
+        Y = self.predict(X)
+        return [fields_from_targets(c, Y[cb[i]:cb[i + 1]]) for i, c in enumerate(cfgs)]
+
     def save(self, path):
         np.savez(path, params=self.net.params(), xm=self.xm, xs=self.xs, ym=self.ym, ys=self.ys)
 
@@ -215,49 +226,71 @@ def start(sim, pts, cfgs, tables, surrogate=None, check_steps=60, tol=0.5, force
     if surrogate is None:
         sim.initialize()
         return np.zeros(len(pts), bool), np.zeros(len(pts))
-    pred = [list(surrogate.fields_for(pt, c)) for pt, c in zip(pts, cfgs)]
+    t0 = time.perf_counter()
+    pred = [list(f) for f in surrogate.fields_batch(sim, pts, cfgs)]
     for i in (force_bad or []):
         pred[i][1], pred[i][2] = -pred[i][1], -pred[i][2]
+    t1 = time.perf_counter()
     sim.initialize_fields(*[[p[k] for p in pred] for k in range(4)])
     sim.reset_samples()
+    t2 = time.perf_counter()
     sim.solver(check_steps, graph=False)
     res = np.array([residual(sim, i, pred[i], tables[i]["frac"]) for i in range(len(pts))])
     fb = res > tol
     if log:
-        log(f"[surrogate] residual check: {int(fb.sum())} of {len(pts)} fall back (tol {tol})")
+        log(f"[surrogate] residual check: {int(fb.sum())} of {len(pts)} fall back (tol {tol}); prediction "
+            f"{t1 - t0:.3f} s, fill {t2 - t1:.3f} s, check {time.perf_counter() - t2:.3f} s")
     if fb.any():
         f = [freestream_fields(c, C[i]) if fb[i] else pred[i] for i, c in enumerate(cfgs)]
         sim.initialize_fields(*[[x[k] for x in f] for k in range(4)])
     return fb, res
 
 
-def run_to_steady(sim, tables, window=200, max_steps=30000, threshold=1.5, patience=3, lag=4, log=print, tag=""):
+def run_to_steady(sim, tables, window=200, max_steps=30000, threshold=1.5, patience=3, lag=4, log=print, tag="",
+                  retire=False):
     """Runs whole windows (each sampled as two halves) until every instance
     is steady (or max_steps); returns per-instance steps to steady (-1 if
-    never) and the wall time to the last one."""
+    never) and the wall time to the last one.  The instances must be
+    sampling every step; the half-window counts are differences of the
+    running count sums, read for all instances in one transfer per half.
+    retire: instances confirmed steady leave the batch (Batch.compact), so
+    the rest run without them; the batch keeps the last ones to settle.
+    Results are indexed by the instances the batch held at the call."""
     n = sim.n
     det = [SteadyDetector(threshold, patience, lag) for _ in range(n)]
     fluid = [t["frac"] >= 0.5 for t in tables]
+    ids = list(range(n))            # batch index -> index at the call
     steps_at = np.full(n, -1)
     half = window // 2
     t0 = time.perf_counter()
     steps = 0
+    prev, ns0 = sim.counts()
     while steps < max_steps and (steps_at < 0).any():
-        active = [i for i in range(n) if steps_at[i] < 0]
-        sim.reset_samples()
+        cb = sim.cbase
         sim.solver(half)
-        first = {i: sim.samples(i)[0][0].copy() for i in active}
+        mid, _ = sim.counts()
         sim.solver(half)
+        end, ns = sim.counts()
+        if np.any(ns - ns0 != 2 * half):
+            raise RuntimeError("run_to_steady needs every instance sampling every step")
         steps += 2 * half
-        for i in active:
-            total = sim.samples(i)[0][0]
-            second = total - first[i]
-            vol, Fn = tables[i]["vol"], sim.info[i].Fn
-            ok = fluid[i] & (first[i] > 0) & (second > 0)
-            if det[i].update(density(first[i], half, Fn, vol), density(second, half, Fn, vol), ok):
+        for j, i in enumerate(ids):
+            if steps_at[i] >= 0:
+                continue
+            first = mid[cb[j]:cb[j + 1]] - prev[cb[j]:cb[j + 1]]
+            second = end[cb[j]:cb[j + 1]] - mid[cb[j]:cb[j + 1]]
+            vol, Fn = tables[i]["vol"], sim.info[j].Fn
+            ok = fluid[i] & (first > 0) & (second > 0)
+            if det[i].update(density(first, half, Fn, vol), density(second, half, Fn, vol), ok):
                 steps_at[i] = steps
+        keep = [j for j, i in enumerate(ids) if steps_at[i] < 0]
+        if retire and 0 < len(keep) < len(ids):
+            sim.compact(keep)
+            ids = [ids[j] for j in keep]
+            end, ns = sim.counts()
+        prev, ns0 = end, ns
         if log and steps % (10 * window) == 0:
-            log(f"[surrogate] {tag} step {steps}: {int((steps_at >= 0).sum())}/{n} steady")
+            log(f"[surrogate] {tag} step {steps}: {int((steps_at >= 0).sum())}/{n} steady, {len(ids)} in the batch")
     sim.sync()
     return steps_at, time.perf_counter() - t0
 
@@ -289,19 +322,34 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
     average_steps right after it is steady) against the converged plain run
     (continued settle_steps past its steady point, then averaged over
     average_steps); the same error of the plain run at its own steady
-    point; fallback rate."""
+    point; fallback rate.
+
+    Each arm runs twice from the same seeds.  The timed pass retires
+    instances from the batch as they are confirmed steady (what a sweep
+    does), so its wall-clock is the time until all are steady.  The replay
+    keeps every instance in the batch -- it must find the same steady steps,
+    since a retired instance's neighbours evolve bit-identically -- and its
+    state at the last steady step is what the fields are averaged from."""
     from .batch import Batch
     from .run import describe
     cfgs = [case_of(pt) for pt in pts]
     tables = [describe(c["params"], c["body"])[1] for c in cfgs]
     out = {}
     for mode in ("plain", "warm"):
+        sur = surrogate if mode == "warm" else None
         sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
         t0 = time.perf_counter()
-        fb, res = start(sim, pts, cfgs, tables, surrogate if mode == "warm" else None, check_steps=check_steps,
-                        log=log)
+        fb, res = start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=log)
         t_init = time.perf_counter() - t0
-        steps_at, wall = run_to_steady(sim, tables, window=window, log=log, tag=mode)
+        steps_at, wall = run_to_steady(sim, tables, window=window, log=log, tag=mode, retire=True)
+        sim.close()
+        sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+        t0 = time.perf_counter()
+        start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=None)
+        t_init_keep = time.perf_counter() - t0
+        steps_keep, wall_keep = run_to_steady(sim, tables, window=window, log=None, tag=mode)
+        if not np.array_equal(steps_keep, steps_at):
+            raise RuntimeError(f"{mode}: the replay that keeps every instance found other steady steps")
         if mode == "warm":
             steps_at = np.where(steps_at >= 0, steps_at + check_steps, steps_at)   # the check's steps count
         fields = averaged_fields(sim, average_steps)
@@ -311,10 +359,12 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
             ref = averaged_fields(sim, average_steps)
         err = sim.status(-1).err
         out[mode] = dict(steps_to_steady=steps_at.tolist(), wall_to_steady_s=wall + t_init, init_s=t_init,
+                         wall_keep_all_s=wall_keep + t_init_keep,
                          fallback=fb.tolist(), residual=res.tolist(), fields=fields, ref=ref, err=int(err))
         sim.close()
         if log:
-            log(f"[surrogate] {mode}: all steady after {int(steps_at.max())} steps, {wall + t_init:.2f} s")
+            log(f"[surrogate] {mode}: all steady after {int(steps_at.max())} steps, {wall + t_init:.2f} s retiring "
+                f"steady instances, {wall_keep + t_init_keep:.2f} s keeping all")
     errs = [field_errors(fw, fr, t["frac"].reshape(fw["n"].shape))
             for fw, fr, t in zip(out["warm"]["fields"], out["plain"]["ref"], tables)]
     errs_plain = [field_errors(fp, fr, t["frac"].reshape(fp["n"].shape))
@@ -326,6 +376,10 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
                 median_steps_plain=float(np.median(sp)), median_steps_warm=float(np.median(sw)),
                 wall_plain_s=out["plain"]["wall_to_steady_s"], wall_warm_s=out["warm"]["wall_to_steady_s"],
                 speedup_wall=out["plain"]["wall_to_steady_s"] / out["warm"]["wall_to_steady_s"],
+                init_plain_s=out["plain"]["init_s"], init_warm_s=out["warm"]["init_s"],
+                wall_keep_all_plain_s=out["plain"]["wall_keep_all_s"],
+                wall_keep_all_warm_s=out["warm"]["wall_keep_all_s"],
+                speedup_wall_keep_all=out["plain"]["wall_keep_all_s"] / out["warm"]["wall_keep_all_s"],
                 speedup_median_steps=float(np.median(sp) / max(np.median(sw), 1.0)),
                 fallback_rate=float(np.mean(out["warm"]["fallback"])),
                 residual=out["warm"]["residual"],
@@ -361,7 +415,8 @@ def pipeline(outdir, runs=512, n_eval=32, fit_steps=3000, steady_steps=4000, sam
                train_ms_per_step=ms, train_tflops=flop / ms / 1e9 if ms > 0 else 0.0)
     with open(os.path.join(outdir, "warmstart.json"), "w") as f:
         json.dump(res, f, indent=1)
-    log(f"[surrogate] speedup (wall) {res['speedup_wall']:.2f}x, median steps {res['median_steps_plain']:.0f} -> "
+    log(f"[surrogate] speedup (wall) {res['speedup_wall']:.2f}x (keeping all instances "
+        f"{res['speedup_wall_keep_all']:.2f}x), median steps {res['median_steps_plain']:.0f} -> "
         f"{res['median_steps_warm']:.0f}, max field error {res['max_field_error']:.4f} "
         f"(plain {res['max_plain_field_error']:.4f}), within 2%: warm {res['within_2pct']}/{n_eval}, "
         f"plain {res['within_2pct_plain']}/{n_eval}, fallback {res['fallback_rate']:.2f}")
@@ -378,7 +433,8 @@ def evaluate_saved(model, outdir, n_eval=32, window=100, grids="fine", log=print
     res["grids"] = grids
     with open(os.path.join(outdir, "warmstart.json"), "w") as f:
         json.dump(res, f, indent=1)
-    log(f"[surrogate] window {window}: speedup (wall) {res['speedup_wall']:.2f}x, median steps "
+    log(f"[surrogate] window {window}: speedup (wall) {res['speedup_wall']:.2f}x (keeping all instances "
+        f"{res['speedup_wall_keep_all']:.2f}x), median steps "
         f"{res['median_steps_plain']:.0f} -> {res['median_steps_warm']:.0f}, max field error "
         f"{res['max_field_error']:.4f} (plain {res['max_plain_field_error']:.4f}), within 2%: warm "
         f"{res['within_2pct']}/{n_eval}, plain {res['within_2pct_plain']}/{n_eval}, fallback {res['fallback_rate']:.2f}")

@@ -247,6 +247,27 @@ int dsmc_batch_reset_samples(dsmc_batch b);
 int dsmc_batch_download_samples(dsmc_batch b, int i, double *acc, uint64_t *nsamples);
 int dsmc_batch_download_fields(dsmc_batch b, int i, double *n, double *ux, double *uy, double *uz,
                                double *T, double *p);
+/* Steady-state monitoring of a sweep (no counterpart in the reference, whose
+ * Run::solver samples one case and writes its fields at the end): the
+ * particle-count sums of every cell of every instance (counts[total cells],
+ * instance order) and, when nsamples is not NULL, each instance's number of
+ * sampled steps (nsamples[n]), in one transfer.  Sums only grow between
+ * dsmc_batch_reset_samples calls, so differences of two reads give the
+ * counts of the steps in between. */
+int dsmc_batch_download_counts(dsmc_batch b, double *counts, uint64_t *nsamples);
+/* Signed distance (m) from every cell centre to its instance's body outline,
+ * negative inside the body, +inf without a body (d[total cells], instance
+ * order; no counterpart in the reference: an input feature of the learned
+ * warm start). */
+int dsmc_batch_wall_distance(dsmc_batch b, double *d);
+/* Retires instances from a sweep (no counterpart in the reference, which
+ * runs one case per process): keeps instances keep[0..nkeep) -- strictly
+ * ascending indices -- which become instances 0..nkeep-1, and frees the
+ * rest.  Each kept instance continues bit-identically (particles, cell
+ * state, sample sums, counters and random streams carry over); the totals
+ * of dsmc_batch_status(b, -1, ...) become the kept instances' sums.  The
+ * handle stays valid; on failure it is unchanged. */
+int dsmc_batch_compact(dsmc_batch b, const int *keep, int nkeep);
 int dsmc_batch_get_launch(dsmc_batch b, dsmc_launch *out);
 int dsmc_batch_set_launch(dsmc_batch b, const dsmc_launch *cfg);
 /* Per-kernel timing of batched steps (eager), as dsmc_set_timing. */

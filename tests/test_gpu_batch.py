@@ -96,3 +96,60 @@ def test_sweep_batch_runs_clean(gpu_lib):
     assert tot.collisions == sum(s.collisions for s in per)
     assert all(s.nsamples == 20 for s in per)
     b.close()
+
+
+def _compare_batches(a, i, b, j, what):
+    T.assert_same_particles(a.download(i), b.download(j), what=what)
+    for u, v in zip(a.cell_state(i), b.cell_state(j)):
+        np.testing.assert_array_equal(u, v)
+    np.testing.assert_array_equal(a.samples(i)[0], b.samples(j)[0])
+    sa, sb = a.status(i), b.status(j)
+    for k in ("step", "n_live", "next_pid", "particle_steps", "collisions", "nsamples", "err", "last_collisions"):
+        assert getattr(sa, k) == getattr(sb, k), (what, k, getattr(sa, k), getattr(sb, k))
+
+
+def test_compact_continues_bit_identically(gpu_lib):
+    """Retiring instances mid-run (an odd step count, so the state sits in
+    the second particle buffer): the kept ones go on exactly as in a batch
+    that kept everything -- particles, cell state, sample sums, counters --
+    through graph replays, and a second compaction down to one instance."""
+    specs = _mixed()
+    full = Batch([p for p, _ in specs], [bd for _, bd in specs])
+    part = Batch([p for p, _ in specs], [bd for _, bd in specs])
+    for b in (full, part):
+        b.initialize()
+        b.solver(13, graph=False)
+    print("[batch] compact to [0, 2, 3] at step 13", flush=True)
+    part.compact([0, 2, 3])
+    assert part.n == 3 and part.origin == [0, 2, 3]
+    assert part.ncells == [full.ncells[k] for k in (0, 2, 3)]
+    for b in (full, part):
+        b.solver(21)
+    for j, i in enumerate(part.origin):
+        _compare_batches(full, i, part, j, f"kept {i} ")
+    tot = part.status(-1)
+    assert tot.n_live == sum(part.status(j).n_live for j in range(3))
+    assert tot.particle_steps == sum(part.status(j).particle_steps for j in range(3))
+    print("[batch] compact to [1] (instance 2) at step 34", flush=True)
+    part.compact([1])
+    assert part.origin == [2]
+    for b in (full, part):
+        b.solver(9, graph=False)
+    _compare_batches(full, 2, part, 0, "kept 2 alone ")
+    assert full.status(-1).err == 0 and part.status(-1).err == 0
+    full.close()
+    part.close()
+
+
+def test_compact_rejects_bad_lists(gpu_lib):
+    specs = _mixed()[:2]
+    b = Batch([p for p, _ in specs], [bd for _, bd in specs])
+    b.initialize()
+    for keep in ([], [1, 0], [0, 0], [2], [-1]):
+        with pytest.raises(RuntimeError):
+            b.compact(keep)
+    assert b.n == 2 and b.origin == [0, 1]
+    b.compact([0, 1])
+    b.solver(3)
+    assert b.status(-1).err == 0
+    b.close()

@@ -57,6 +57,9 @@ class _RunPredictor:
         return (np.where(sampled, f["n"].ravel(), p.n_inf), np.where(sampled, f["ux"].ravel(), p.V_jet),
                 np.where(sampled, f["uy"].ravel(), 0.0), np.where(sampled & (f["T"].ravel() > 0), f["T"].ravel(), p.T))
 
+    def fields_batch(self, sim, pts, cfgs):
+        return [self.fields_for(pt, c) for pt, c in zip(pts, cfgs)]
+
 
 def test_residual_check_falls_back_on_a_bad_prediction(gpu_lib):
     """A sound prediction passes the check; the same prediction with the
@@ -80,12 +83,64 @@ def test_residual_check_falls_back_on_a_bad_prediction(gpu_lib):
 
 
 def test_run_to_steady_reports_every_instance(gpu_lib):
-    pts = S.draw_points(3, seed=8, grids=[(64, 32)], ppc=(10.0, 10.0))
+    """Every instance is reported steady; retiring steady instances from the
+    batch (Batch.compact) finds the same steps, since the instances left
+    evolve bit-identically."""
+    pts = S.draw_points(4, seed=8, grids=[(64, 32), (48, 24)], ppc=(10.0, 10.0))
     cfgs = [S.case_of(pt) for pt in pts]
     tables = [describe(c["params"], c["body"])[1] for c in cfgs]
+    got = {}
+    for retire in (False, True):
+        sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+        sim.initialize()
+        got[retire], wall = S.run_to_steady(sim, tables, window=200, max_steps=20000, log=None, retire=retire)
+        print(f"[warm] retire={retire}: steps to steady {got[retire]}, {sim.n} left in the batch", flush=True)
+        if retire and len(set(got[retire].tolist())) > 1:
+            assert sim.n < len(pts)
+        sim.close()
+    assert np.all(got[False] > 0)
+    assert np.array_equal(got[False], got[True])
+
+
+def test_wall_distance_matches_the_host_formula(gpu_lib):
+    """dsmc_batch_wall_distance against the host's vectorised formula on a
+    cylinder, a wedge and a body-free box of different grids."""
+    pts = S.draw_points(6, seed=21, grids=[(64, 32), (96, 48), (40, 20)])
+    pts = [next(p for p in pts if p["geometry"] == g) for g in ("cylinder", "wedge")]
+    cfgs = [S.case_of(pt) for pt in pts] + [cases.box_equilibrium(ppc=5, n_cells=8, seed=2)]
+    sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs[:2]] + [None])
+    d = sim.wall_distance()
+    cb = sim.cbase
+    for i, c in enumerate(cfgs[:2]):
+        p = c["params"]
+        ii, jj = np.divmod(np.arange(p.N1 * p.N2), p.N2)
+        ref = S._signed_distance((ii + 0.5) * (p.L1 / p.N1), (jj + 0.5) * (p.L2 / p.N2), c["body"])
+        got = d[cb[i]:cb[i + 1]]
+        print(f"[warm] wall distance {pts[i]['geometry']}: max |diff| {np.abs(got - ref).max():.3e}", flush=True)
+        assert np.array_equal(np.sign(got), np.sign(ref))
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-15 * p.L2)
+        assert (got < 0).any() and (got > 0).any()
+    assert np.all(np.isinf(d[cb[2]:cb[3]])) and np.all(d[cb[2]:cb[3]] > 0)
+    sim.close()
+
+
+def test_batch_counts_match_per_instance_samples(gpu_lib):
+    """dsmc_batch_download_counts is the count row of every instance's
+    sample sums, bit for bit, with each instance's sampled-step count; the
+    half-window differences the steady runner takes are whole counts."""
+    pts = S.draw_points(3, seed=8, grids=[(64, 32), (48, 24)], ppc=(10.0, 10.0))
+    cfgs = [S.case_of(pt) for pt in pts]
     sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
     sim.initialize()
-    steps_at, wall = S.run_to_steady(sim, tables, window=200, max_steps=20000, log=None)
-    print(f"[warm] steps to steady {steps_at}", flush=True)
-    assert np.all(steps_at > 0)
+    sim.solver(37)
+    c0, n0 = sim.counts()
+    sim.solver(25)
+    c1, n1 = sim.counts()
+    assert n0.tolist() == [37] * 3 and n1.tolist() == [62] * 3
+    for i in range(3):
+        acc, ns = sim.samples(i)
+        assert ns == 62
+        assert np.array_equal(c1[sim.cbase[i]:sim.cbase[i + 1]], acc[0])
+    diff = c1 - c0
+    assert np.all(diff >= 0) and np.array_equal(diff, np.round(diff)) and diff.sum() > 0
     sim.close()

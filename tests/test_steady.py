@@ -102,3 +102,29 @@ def test_lag_catches_slow_drift():
         lag4.update(a, b)
     assert lag1.steady_at is not None
     assert lag4.steady_at is None
+
+
+def test_lag_ramps_up_from_the_first_window():
+    """Before lag windows have passed, each window is compared with the
+    first one: pure noise is confirmed at window patience + 1 whatever the
+    lag, and its chi stays about 1 at every lag used."""
+    for lag in (1, 2, 4, 8):
+        det = SteadyDetector(1.5, 3, lag=lag)
+        for a, b in _signal(1.0, 0.0, 0.02, windows=4, seed=lag):
+            det.update(a, b)
+        assert det.steady_at == 4, lag
+        assert all(abs(c - 1.0) < 0.1 for c in det.history)
+
+
+def test_ramp_keeps_a_drift_from_passing_early():
+    """A drift that one window's lag hides (chi about 1.15 at lag 1) shows at
+    lags 2 and 3 of the ramp, so a run drifting from its start is not
+    confirmed in its first windows."""
+    rng = np.random.default_rng(5)
+    base = _base(4000)
+    det = SteadyDetector(1.5, 3, lag=4)
+    for w in range(12):
+        a, b = (base * (1.0 + 0.003 * (2 * w + q)) + 0.01 * rng.normal(0, 1, 4000) for q in (0, 1))
+        det.update(a, b)
+    assert det.history[0] < 1.5 < det.history[1] < det.history[2]
+    assert det.steady_at is None
