@@ -219,6 +219,10 @@ int dsmc_batch_destroy(dsmc_batch b)
     return DSMC_OK;
 }
 
+namespace {
+int setup_device(dsmc_batch b, dsmc_batch *out);
+}
+
 int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n, dsmc_batch *out)
 {
     if (!params || !out || n < 1) return fail(DSMC_ERR_ARG, "batch needs n >= 1 cases");
@@ -230,10 +234,6 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
     b->ninst = n;
     b->P.resize(n);
     b->T.resize(n);
-    b->cs.resize(n);
-    b->capi.resize(n);
-    b->cbase.assign(n + 1, 0);
-    int64_t C = 0, segs = 0, idx = 0;
     for (int i = 0; i < n; ++i) {
         std::string msg = b->P[i].derive(params[i]);
         if (!msg.empty()) { delete b; return fail(DSMC_ERR_ARG, "instance " + std::to_string(i) + ": " + msg); }
@@ -243,6 +243,22 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
         b->poly0.push_back((int)(b->poly.size() / 2));
         b->npoly.push_back(bd ? bd->n_points : 0);
         if (bd) b->poly.insert(b->poly.end(), bd->xy, bd->xy + 2 * (size_t)bd->n_points);
+    }
+    return setup_device(b, out);
+}
+
+namespace {
+
+// Device arrays, uploaded tables and launch sizes of a batch whose host
+// side (P, T, body outlines) is set; takes ownership of b (freed on failure).
+int setup_device(dsmc_batch b, dsmc_batch *out)
+{
+    const int n = b->ninst;
+    b->cs.resize(n);
+    b->capi.resize(n);
+    b->cbase.assign(n + 1, 0);
+    int64_t C = 0, segs = 0, idx = 0;
+    for (int i = 0; i < n; ++i) {
         b->capi[i] = default_capacity(b->P[i], b->T[i]);
         b->cap += b->capi[i];
         C += b->T[i].ncell;
@@ -338,6 +354,8 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
     *out = b;
     return DSMC_OK;
 }
+
+}  // namespace
 
 int dsmc_batch_case_info(dsmc_batch b, int i, dsmc_case_info *o)
 {
@@ -610,7 +628,8 @@ int dsmc_batch_wall_distance(dsmc_batch b, double *d)
 }
 
 // Keeps the instances keep[0..nkeep) (ascending) and drops the rest: a new
-// set of device arrays sized for the kept instances receives their particle
+// set of device arrays sized for the kept instances (their host tables are
+// copied, not rebuilt) receives their particle
 // ranges, per-cell state (sigcmax, NTC remainders, collision counts, sample
 // sums) and counters, then takes the handle's place.  Random streams are
 // keyed on step, particle id and local cell id, none of which change, so a
@@ -627,15 +646,18 @@ int dsmc_batch_compact(dsmc_batch b, const int *keep, int nkeep)
     std::vector<InstCtrl> ic;
     if ((r = read_ctrl(b, c, &ic))) return r;
     if (nkeep == b->ninst) return DSMC_OK;
-    std::vector<dsmc_params> P(nkeep);
-    std::vector<dsmc_body> B(nkeep);
+    dsmc_batch nb = new dsmc_batch_s();
+    nb->ninst = nkeep;
     for (int k = 0; k < nkeep; ++k) {
         const int i = keep[k];
-        P[k] = b->P[i].in;
-        B[k] = dsmc_body{ b->npoly[i], b->npoly[i] ? b->poly.data() + 2 * (size_t)b->poly0[i] : nullptr };
+        nb->P.push_back(b->P[i]);
+        nb->T.push_back(b->T[i]);
+        nb->poly0.push_back((int)(nb->poly.size() / 2));
+        nb->npoly.push_back(b->npoly[i]);
+        const double *v = b->poly.data() + 2 * (size_t)b->poly0[i];
+        nb->poly.insert(nb->poly.end(), v, v + 2 * (size_t)b->npoly[i]);
     }
-    dsmc_batch nb = nullptr;
-    if ((r = dsmc_batch_create(P.data(), B.data(), nkeep, &nb))) return r;
+    if ((r = setup_device(nb, &nb))) return r;
     nb->grid_move = b->grid_move;
     nb->grid_scatter = b->grid_scatter;
     nb->grid_cell = b->grid_cell;

@@ -216,12 +216,16 @@ def residual(sim, i, pred, frac):
     return float(out)
 
 
-def start(sim, pts, cfgs, tables, surrogate=None, check_steps=60, tol=0.5, force_bad=None, log=print):
+def start(sim, pts, cfgs, tables, surrogate=None, check_steps=60, tol=0.5, force_bad=None, log=print,
+          detectors=None):
     """Initialises a batch: freestream, or the surrogate's fields with a
     residual check per instance (instances that fail restart from
     freestream).  force_bad: indices whose prediction is replaced by a
     deliberately wrong one, the predicted flow reversed (tests of the
-    fallback).  Returns the fallback flags and the check's residuals."""
+    fallback).  detectors: one SteadyDetector per instance; the check is then
+    sampled as two halves and, when no instance falls back (a fallback
+    refills the whole batch), fed to them as their first window.  Returns
+    the fallback flags and the check's residuals."""
     C = [t["vol"].size for t in tables]
     if surrogate is None:
         sim.initialize()
@@ -234,7 +238,16 @@ def start(sim, pts, cfgs, tables, surrogate=None, check_steps=60, tol=0.5, force
     sim.initialize_fields(*[[p[k] for p in pred] for k in range(4)])
     sim.reset_samples()
     t2 = time.perf_counter()
-    sim.solver(check_steps, graph=False)
+    if detectors is None:
+        sim.solver(check_steps, graph=False)
+    else:
+        if check_steps % 2:
+            raise ValueError("a check fed to the detectors needs an even number of steps")
+        half = check_steps // 2
+        sim.solver(half)
+        mid, _ = sim.counts()
+        sim.solver(half)
+        end, _ = sim.counts()
     res = np.array([residual(sim, i, pred[i], tables[i]["frac"]) for i in range(len(pts))])
     fb = res > tol
     if log:
@@ -243,11 +256,22 @@ def start(sim, pts, cfgs, tables, surrogate=None, check_steps=60, tol=0.5, force
     if fb.any():
         f = [freestream_fields(c, C[i]) if fb[i] else pred[i] for i, c in enumerate(cfgs)]
         sim.initialize_fields(*[[x[k] for x in f] for k in range(4)])
+    elif detectors is not None:
+        cb = sim.cbase
+        for i, d in enumerate(detectors):
+            a, b = mid[cb[i]:cb[i + 1]], end[cb[i]:cb[i + 1]] - mid[cb[i]:cb[i + 1]]
+            vol, Fn = tables[i]["vol"], sim.info[i].Fn
+            d.update(density(a, half, Fn, vol), density(b, half, Fn, vol),
+                     (tables[i]["frac"] >= 0.5) & (a > 0) & (b > 0))
     return fb, res
 
 
+# steady-state detector settings of every run here (cudadsmc_amd/steady.py)
+STEADY = dict(threshold=1.5, patience=3, lag=4)
+
+
 def run_to_steady(sim, tables, window=200, max_steps=30000, threshold=1.5, patience=3, lag=4, log=print, tag="",
-                  retire=False):
+                  retire=False, detectors=None):
     """Runs whole windows (each sampled as two halves) until every instance
     is steady (or max_steps); returns per-instance steps to steady (-1 if
     never) and the wall time to the last one.  The instances must be
@@ -255,9 +279,11 @@ def run_to_steady(sim, tables, window=200, max_steps=30000, threshold=1.5, patie
     running count sums, read for all instances in one transfer per half.
     retire: instances confirmed steady leave the batch (Batch.compact), so
     the rest run without them; the batch keeps the last ones to settle.
-    Results are indexed by the instances the batch held at the call."""
+    Results are indexed by the instances the batch held at the call.
+    detectors: the instances' detectors when they already hold windows
+    (start's residual check); their windows count toward patience."""
     n = sim.n
-    det = [SteadyDetector(threshold, patience, lag) for _ in range(n)]
+    det = detectors if detectors is not None else [SteadyDetector(threshold, patience, lag) for _ in range(n)]
     fluid = [t["frac"] >= 0.5 for t in tables]
     ids = list(range(n))            # batch index -> index at the call
     steps_at = np.full(n, -1)
@@ -316,7 +342,7 @@ def field_errors(fw, fp, frac):
     return dict(n=e_n, u=e_u, T=e_T)
 
 
-def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000, check_steps=60, log=print):
+def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000, check_steps=None, log=print):
     """Warm start vs plain on held-out points (one batch each): steps and
     wall-clock to steady; field error of the warm run (averaged over
     average_steps right after it is steady) against the converged plain run
@@ -329,25 +355,30 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
     does), so its wall-clock is the time until all are steady.  The replay
     keeps every instance in the batch -- it must find the same steady steps,
     since a retired instance's neighbours evolve bit-identically -- and its
-    state at the last steady step is what the fields are averaged from."""
+    state at the last steady step is what the fields are averaged from.
+    The warm start's residual check (check_steps, one window by default) is
+    the detector's first window; its steps count toward the warm run's."""
     from .batch import Batch
     from .run import describe
+    check_steps = window if check_steps is None else check_steps
     cfgs = [case_of(pt) for pt in pts]
     tables = [describe(c["params"], c["body"])[1] for c in cfgs]
     out = {}
     for mode in ("plain", "warm"):
         sur = surrogate if mode == "warm" else None
         sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+        det = [SteadyDetector(**STEADY) for _ in pts]
         t0 = time.perf_counter()
-        fb, res = start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=log)
+        fb, res = start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=log, detectors=det)
         t_init = time.perf_counter() - t0
-        steps_at, wall = run_to_steady(sim, tables, window=window, log=log, tag=mode, retire=True)
+        steps_at, wall = run_to_steady(sim, tables, window=window, log=log, tag=mode, retire=True, detectors=det)
         sim.close()
         sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+        det = [SteadyDetector(**STEADY) for _ in pts]
         t0 = time.perf_counter()
-        start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=None)
+        start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=None, detectors=det)
         t_init_keep = time.perf_counter() - t0
-        steps_keep, wall_keep = run_to_steady(sim, tables, window=window, log=None, tag=mode)
+        steps_keep, wall_keep = run_to_steady(sim, tables, window=window, log=None, tag=mode, detectors=det)
         if not np.array_equal(steps_keep, steps_at):
             raise RuntimeError(f"{mode}: the replay that keeps every instance found other steady steps")
         if mode == "warm":
@@ -365,6 +396,13 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
         if log:
             log(f"[surrogate] {mode}: all steady after {int(steps_at.max())} steps, {wall + t_init:.2f} s retiring "
                 f"steady instances, {wall_keep + t_init_keep:.2f} s keeping all")
+    alone = run_alone(surrogate, pts, cfgs, tables, window=window, check_steps=check_steps)
+    same = all(np.array_equal(alone[m]["steps"], out[m]["steps_to_steady"]) for m in ("plain", "warm"))
+    ratio = np.array(alone["plain"]["wall"]) / np.array(alone["warm"]["wall"])
+    if log:
+        log(f"[surrogate] each case alone: wall-clock to steady, median ratio {np.median(ratio):.2f}x, "
+            f"total {sum(alone['plain']['wall']):.2f} s -> {sum(alone['warm']['wall']):.2f} s; steps "
+            f"{'the same as' if same else 'DIFFERENT from'} the batch's")
     errs = [field_errors(fw, fr, t["frac"].reshape(fw["n"].shape))
             for fw, fr, t in zip(out["warm"]["fields"], out["plain"]["ref"], tables)]
     errs_plain = [field_errors(fp, fr, t["frac"].reshape(fp["n"].shape))
@@ -390,7 +428,34 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
                 within_2pct=int(sum(max(e.values()) <= 0.02 for e in errs)),
                 within_2pct_plain=int(sum(max(e.values()) <= 0.02 for e in errs_plain)),
                 window=window, average_steps=average_steps, settle_steps=settle_steps,
+                wall_alone_plain_s=alone["plain"]["wall"], wall_alone_warm_s=alone["warm"]["wall"],
+                speedup_alone_median=float(np.median(ratio)), alone_steps_match_batch=bool(same),
+                speedup_alone_total=float(sum(alone["plain"]["wall"]) / sum(alone["warm"]["wall"])),
                 device_err=[out["plain"]["err"], out["warm"]["err"]])
+
+
+def run_alone(surrogate, pts, cfgs, tables, window=100, check_steps=100):
+    """Each case in a batch of its own, plain then warm: wall-clock from the
+    start of initialisation (prediction, fill and residual check for the
+    warm start) until the detector confirms it steady.  The steps must be
+    the ones the same case reached in a shared batch (an instance evolves
+    bit-identically alone or among others)."""
+    from .batch import Batch
+    res = {m: dict(steps=[], wall=[]) for m in ("plain", "warm")}
+    for pt, c, t in zip(pts, cfgs, tables):
+        for mode in ("plain", "warm"):
+            sim = Batch([c["params"]], [c["body"]])
+            det = [SteadyDetector(**STEADY)]
+            t0 = time.perf_counter()
+            start(sim, [pt], [c], [t], surrogate if mode == "warm" else None, check_steps=check_steps, log=None,
+                  detectors=det)
+            steps_at, _ = run_to_steady(sim, [t], window=window, log=None, detectors=det)
+            wall = time.perf_counter() - t0
+            sim.close()
+            k = int(steps_at[0])
+            res[mode]["steps"].append(k + check_steps if (mode == "warm" and k >= 0) else k)
+            res[mode]["wall"].append(wall)
+    return res
 
 
 def pipeline(outdir, runs=512, n_eval=32, fit_steps=3000, steady_steps=4000, sample_steps=2000,

@@ -10,6 +10,7 @@ import pytest
 from cudadsmc_amd import cases, surrogate as S
 from cudadsmc_amd.batch import Batch
 from cudadsmc_amd.run import Run, describe
+from cudadsmc_amd.steady import SteadyDetector
 
 pytestmark = pytest.mark.gpu
 
@@ -74,11 +75,20 @@ def test_residual_check_falls_back_on_a_bad_prediction(gpu_lib):
     plain.close()
     pred = _RunPredictor({pt["seed"]: f[i] for i, pt in enumerate(pts)})
     sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
-    fb, res = S.start(sim, pts, cfgs, tables, pred, check_steps=60, force_bad=[1], log=None)
+    det = [SteadyDetector(**S.STEADY) for _ in pts]
+    fb, res = S.start(sim, pts, cfgs, tables, pred, check_steps=60, force_bad=[1], log=None, detectors=det)
     print(f"[warm] residuals {res}", flush=True)
     assert list(fb) == [False, True]
     assert res[0] < 0.35 and res[1] > 1.0
+    assert [d.windows for d in det] == [0, 0]       # the batch was refilled: the check is no window
     assert sim.status(-1).err == 0
+    sim.close()
+    # without a fallback the check is every detector's first window
+    sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
+    det = [SteadyDetector(**S.STEADY) for _ in pts]
+    fb, res = S.start(sim, pts, cfgs, tables, pred, check_steps=60, log=None, detectors=det)
+    assert not fb.any()
+    assert [d.windows for d in det] == [1, 1]
     sim.close()
 
 
