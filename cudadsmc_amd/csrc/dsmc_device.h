@@ -32,6 +32,8 @@ struct DevCase {
     int nx, ny, ncell, nseg;
     float L1, L2, dx, dy, dt;
     float ntc_coef, K1, e_exp, margin;
+    float rdx, rdy;                // float nearest 1/dx, 1/dy (cell_of's fast path)
+    float bx0, by0, bx1, by1;      // outside this box every cell's segment list is empty
     int body_bc;
     float body_vmp, body_sig;
     int face_bc[4];
@@ -154,12 +156,34 @@ __device__ __forceinline__ void dm_sincos2pi(float u, float &sn, float &cs)
     else              { sn = -cp; cs = sp; }
 }
 
+// (int)(x / d) with the correctly rounded quotient, mostly without the
+// division.  r is within one unit in the last place of 1/d, so q = x*r is
+// within 2.5 units of fl(x/d).  When q lies more than 8 units (1e-6 q) from
+// every integer, both truncate to the same integer; otherwise (and for
+// q <= 0) the division decides.  q - truncf(q) is exact for float q.  The
+// division sits behind a wave-uniform branch that holds an empty volatile
+// asm statement: without it the compiler turns the branch into a select and
+// divides on every lane.
+__device__ __forceinline__ int trunc_quot(float x, float d, float r)
+{
+    const float q = x * r;
+    const float f = q - truncf(q);
+    const float tol = q * 1e-6f;
+    const bool slow = !(f > tol && f < 1.0f - tol);
+    int i = (int)q;
+    if (__ballot(slow)) {
+        asm volatile("");   // a side effect: keeps the branch a branch
+        if (slow) i = (int)(x / d);
+    }
+    return i;
+}
+
 // Cell index (cudaDSMC GPUCell.cu:184-189: i = (int)(x/dx), cell = j + i*ny),
 // clamped into the grid.
 __device__ __forceinline__ int cell_of(const DevCase &cs, float x, float y)
 {
-    int i = (int)(x / cs.dx);
-    int j = (int)(y / cs.dy);
+    int i = trunc_quot(x, cs.dx, cs.rdx);
+    int j = trunc_quot(y, cs.dy, cs.rdy);
     i = i < 0 ? 0 : (i > cs.nx - 1 ? cs.nx - 1 : i);
     j = j < 0 ? 0 : (j > cs.ny - 1 ? cs.ny - 1 : j);
     return j + i * cs.ny;
@@ -195,26 +219,47 @@ __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, 
         if (b == MAX_BOUNCE) { err |= ERR_BOUNCE; break; }
         float best = t;
         int kind = -1;
+        // A face's time s = d / |v| (d the distance to it) is computed only
+        // when it can beat best: for best > 0, d > |v| best (1 + 1e-4)
+        // implies fl(d / |v|) >= best, so skipping changes no outcome.
         if (vx < 0.0f) {
-            float s = (0.0f - x) / vx;
-            if (s < best) { best = s; kind = 0; }
+            const float d = 0.0f - x;
+            if (!(best > 0.0f && -d > -vx * best * 1.0001f)) {
+                float s = d / vx;
+                if (s < best) { best = s; kind = 0; }
+            }
         } else if (vx > 0.0f) {
-            float s = (cs.L1 - x) / vx;
-            if (s < best) { best = s; kind = 1; }
+            const float d = cs.L1 - x;
+            if (!(best > 0.0f && d > vx * best * 1.0001f)) {
+                float s = d / vx;
+                if (s < best) { best = s; kind = 1; }
+            }
         }
         if (vy < 0.0f) {
-            float s = (0.0f - y) / vy;
-            if (s < best) { best = s; kind = 2; }
+            const float d = 0.0f - y;
+            if (!(best > 0.0f && -d > -vy * best * 1.0001f)) {
+                float s = d / vy;
+                if (s < best) { best = s; kind = 2; }
+            }
         } else if (vy > 0.0f) {
-            float s = (cs.L2 - y) / vy;
-            if (s < best) { best = s; kind = 3; }
+            const float d = cs.L2 - y;
+            if (!(best > 0.0f && d > vy * best * 1.0001f)) {
+                float s = d / vy;
+                if (s < best) { best = s; kind = 3; }
+            }
         }
         if (cs.nseg > 0) {
-            float reach = sqrtf(vx * vx + vy * vy) * t;
-            bool all = reach > cs.margin;
+            // reach = sqrt(v^2) t against the margin; the square decides
+            // unless it lies within 0.1 % of margin^2
+            const float v2 = vx * vx + vy * vy;
+            const float r2 = v2 * t * t, m2 = cs.margin * cs.margin;
+            const bool all = r2 > 1.001f * m2 ? true : (r2 < 0.999f * m2 ? false : sqrtf(v2) * t > cs.margin);
             int cnt, c0 = 0;
             if (all) {
                 cnt = cs.nseg;
+            } else if (fminf(fmaxf(x, 0.0f), cs.L1) < cs.bx0 || fminf(fmaxf(x, 0.0f), cs.L1) > cs.bx1 ||
+                       fminf(fmaxf(y, 0.0f), cs.L2) < cs.by0 || fminf(fmaxf(y, 0.0f), cs.L2) > cs.by1) {
+                cnt = 0;   // the clamped position's cell lists no segment
             } else {
                 int c = cell_of(cs, x, y);
                 c0 = cs.cseg_start[c];

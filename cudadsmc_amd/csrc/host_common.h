@@ -78,6 +78,26 @@ inline DevCase make_devcase(const Param &P, const CutTables &T, int64_t cap)
     cs.inj_vmp = (float)P.inj_vmp; cs.inj_sig = (float)P.inj_sig;
     cs.inj_ux = (float)p->V_jet; cs.inj_uy = (float)p->V_y;
     cs.seed = P.seed32; cs.cap = (uint32_t)cap;
+    cs.rdx = (float)(1.0 / (double)cs.dx);
+    cs.rdy = (float)(1.0 / (double)cs.dy);
+    // Box outside which no cell lists a segment: the segments' bounding box
+    // grown by the list margin (case_setup.cpp), two cells for the distance
+    // between a particle and its cell's far side, and a float safety band.
+    cs.bx0 = cs.by0 = 1.0f;
+    cs.bx1 = cs.by1 = -1.0f;
+    if (T.nseg > 0) {
+        double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+        for (int k = 0; k < T.nseg; ++k)
+            for (int e = 0; e < 2; ++e) {
+                const double sx = T.seg[6 * k + 2 * e], sy = T.seg[6 * k + 2 * e + 1];
+                x0 = std::min(x0, sx); x1 = std::max(x1, sx);
+                y0 = std::min(y0, sy); y1 = std::max(y1, sy);
+            }
+        const double L = std::max(p->L1, p->L2);
+        const double g = 2.0 * std::max(P.dx, P.dy) + 1.001 * (double)T.margin + 1e-9 * L + 1e-5 * L;
+        cs.bx0 = (float)(x0 - g); cs.bx1 = (float)(x1 + g);
+        cs.by0 = (float)(y0 - g); cs.by1 = (float)(y1 + g);
+    }
     return cs;
 }
 

@@ -210,6 +210,49 @@ def test_tile_cut_into_lds_runs(gpu_lib):
     r.close()
 
 
+def test_dense_cells_over_steps(gpu_lib):
+    """Cells of 255 to 900 particles next to sparse ones, over four steps
+    (the dense cells stay dense): order, particles and collision state
+    bit-exact against the oracle at every step, for the default tile and
+    for a tile of 256 particles, where the dense cells each exceed a tile."""
+    for tile in (1024, 256):
+        p, body = T.small_box(8, 1, seed=4)
+        p.n_inf = 1.35e16
+        oc, info, tables = O.case_from_params(p, body)
+        rng = np.random.default_rng(11)
+        dx = np.float32(info.dx)
+        xs, ys = [], []
+        for j, cnt in ((0, 40), (1, 300), (2, 260), (3, 20), (4, 900), (5, 255), (6, 256), (7, 3)):
+            xs.append((rng.random(cnt, dtype=np.float32) * dx * 0.99).astype(np.float32) + 2 * dx)
+            ys.append((rng.random(cnt, dtype=np.float32) * dx * 0.99).astype(np.float32) + j * dx)
+        n = sum(len(v) for v in xs)
+        parts = dict(x=np.concatenate(xs), y=np.concatenate(ys),
+                     vx=rng.normal(0, 1e-3, n).astype(np.float32), vy=rng.normal(0, 1e-3, n).astype(np.float32),
+                     vz=rng.normal(0, 300, n).astype(np.float32), pid=np.arange(n, dtype=np.uint32))
+        r = Run(p, body)
+        r.set_launch(cell_tile=tile)
+        r.upload(*[parts[k] for k in ("x", "y", "vx", "vy", "vz")], parts["pid"])
+        r.set_keep_perm(True)
+        for step in range(4):
+            print(f"[parity] dense cells, tile {tile}, step {step}", flush=True)
+            cur = r.download()
+            sm, mr, _ = r.cell_state()
+            st = r.status()
+            ref = T.oracle_phases(oc, cur, step, st.next_pid, sm, mr, int(info.capacity))
+            r.particlemove()
+            r.ressignParticle()
+            r.collision()
+            np.testing.assert_array_equal(r.perm(), ref["perm"])
+            T.assert_same_particles(r.download(), ref["new"], what=f"tile {tile} step {step} ")
+            sm_g, mr_g, nc_g = r.cell_state()
+            np.testing.assert_array_equal(sm_g, ref["sigcmax"])
+            np.testing.assert_array_equal(nc_g, ref["ncoll"])
+            counts = np.diff(ref["start"])
+            assert (counts >= 256).sum() >= 3          # 4 at the start; collisions spread a few
+        assert r.status().err == 0
+        r.close()
+
+
 def test_box_full_rate_and_conservation(gpu_lib):
     """Closed box at equilibrium, full size (64x64, 81,920 particles, 1000
     steps): collision rate against VHS theory, energy and particle
