@@ -1,6 +1,8 @@
 """Builds the native libraries in-tree.
 
 libcudadsmc.so  -- the HIP kernels, the C ABI and the C++ host layer, for gfx950.
+dsmc_run        -- the C++ driver (cudadsmc_amd/cpp: Run, Param, Geom, Output),
+                   linked against libcudadsmc.so through the C ABI only.
 oracle/liboracle.so is built by oracle/Makefile (test infrastructure).
 """
 import os
@@ -51,6 +53,23 @@ def build_library(verbose=False):
     return LIB
 
 
+HOST = os.path.join(PKG, "dsmc_run")
+HOST_SOURCES = [os.path.join(PKG, "cpp", f) for f in ("Param.cpp", "Geom.cpp", "Output.cpp", "Run.cpp", "main.cpp")]
+
+
+def build_host():
+    """The C++ driver: plain g++, the C ABI header and libcudadsmc.so (found
+    next to the executable at run time)."""
+    tmp = HOST + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", tmp] + HOST_SOURCES + [
+        "-L" + PKG, "-lcudadsmc", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, HOST)
+    return HOST
+
+
 def build_oracle():
     res = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")],
                          capture_output=True, text=True)
@@ -60,4 +79,5 @@ def build_oracle():
 
 if __name__ == "__main__":
     print(build_library())
+    print(build_host())
     build_oracle()
