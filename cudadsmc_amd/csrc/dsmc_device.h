@@ -32,7 +32,6 @@ struct DevCase {
     int nx, ny, ncell, nseg;
     float L1, L2, dx, dy, dt;
     float ntc_coef, K1, e_exp, margin;
-    float rdx, rdy;                // float nearest 1/dx, 1/dy (cell_of's fast path)
     float bx0, by0, bx1, by1;      // outside this box every cell's segment list is empty
     int body_bc;
     float body_vmp, body_sig;
@@ -156,34 +155,12 @@ __device__ __forceinline__ void dm_sincos2pi(float u, float &sn, float &cs)
     else              { sn = -cp; cs = sp; }
 }
 
-// (int)(x / d) with the correctly rounded quotient, mostly without the
-// division.  r is within one unit in the last place of 1/d, so q = x*r is
-// within 2.5 units of fl(x/d).  When q lies more than 8 units (1e-6 q) from
-// every integer, both truncate to the same integer; otherwise (and for
-// q <= 0) the division decides.  q - truncf(q) is exact for float q.  The
-// division sits behind a wave-uniform branch that holds an empty volatile
-// asm statement: without it the compiler turns the branch into a select and
-// divides on every lane.
-__device__ __forceinline__ int trunc_quot(float x, float d, float r)
-{
-    const float q = x * r;
-    const float f = q - truncf(q);
-    const float tol = q * 1e-6f;
-    const bool slow = !(f > tol && f < 1.0f - tol);
-    int i = (int)q;
-    if (__ballot(slow)) {
-        asm volatile("");   // a side effect: keeps the branch a branch
-        if (slow) i = (int)(x / d);
-    }
-    return i;
-}
-
 // Cell index (cudaDSMC GPUCell.cu:184-189: i = (int)(x/dx), cell = j + i*ny),
 // clamped into the grid.
 __device__ __forceinline__ int cell_of(const DevCase &cs, float x, float y)
 {
-    int i = trunc_quot(x, cs.dx, cs.rdx);
-    int j = trunc_quot(y, cs.dy, cs.rdy);
+    int i = (int)(x / cs.dx);
+    int j = (int)(y / cs.dy);
     i = i < 0 ? 0 : (i > cs.nx - 1 ? cs.nx - 1 : i);
     j = j < 0 ? 0 : (j > cs.ny - 1 ? cs.ny - 1 : j);
     return j + i * cs.ny;

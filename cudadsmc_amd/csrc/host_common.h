@@ -78,8 +78,6 @@ inline DevCase make_devcase(const Param &P, const CutTables &T, int64_t cap)
     cs.inj_vmp = (float)P.inj_vmp; cs.inj_sig = (float)P.inj_sig;
     cs.inj_ux = (float)p->V_jet; cs.inj_uy = (float)p->V_y;
     cs.seed = P.seed32; cs.cap = (uint32_t)cap;
-    cs.rdx = (float)(1.0 / (double)cs.dx);
-    cs.rdy = (float)(1.0 / (double)cs.dy);
     // Box outside which no cell lists a segment: the segments' bounding box
     // grown by the list margin (case_setup.cpp), two cells for the distance
     // between a particle and its cell's far side, and a float safety band.

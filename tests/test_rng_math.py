@@ -85,32 +85,3 @@ def test_sincos2pi_accuracy():
     np.testing.assert_array_equal(s, [0, 1, 0, -1])
     np.testing.assert_array_equal(c, [1, 0, -1, 0])
 
-
-def test_cell_index_fast_path_equals_division():
-    """cell_of's truncated quotient without dividing (dsmc_device.h
-    trunc_quot): q = x * fl(1/d) decides when it lies more than 1e-6 q from
-    every integer, else x / d does.  Restated in float32 and compared with
-    (int)(x / d) at and around k * d for random spacings d, where a
-    one-unit error in q would flip the cell."""
-    rng = np.random.default_rng(3)
-    f32 = np.float32
-    for d in rng.uniform(1e-4, 0.05, 40).astype(f32):
-        r = f32(1.0 / float(d))
-        base = (np.arange(1, 4000, dtype=f32) * d).astype(f32)
-        xs = [base]
-        for _ in range(12):
-            xs.append(np.nextafter(xs[-1], f32(1e9)))
-        lo = base
-        for _ in range(12):
-            lo = np.nextafter(lo, f32(-1e9))
-            xs.append(lo)
-        x = np.concatenate(xs + [rng.uniform(0, 4000 * d, 20000).astype(f32)])
-        with np.errstate(all="ignore"):
-            q = (x * r).astype(f32)
-            f = (q - np.trunc(q)).astype(f32)
-            tol = (q * f32(1e-6)).astype(f32)
-            fast = (f > tol) & (f < f32(1) - tol)
-            got = np.where(fast, q.astype(np.int64), (x / d).astype(f32).astype(np.int64))
-            want = (x / d).astype(f32).astype(np.int64)
-        assert np.array_equal(got, want)
-        assert fast[-20000:].mean() > 0.99      # random positions rarely need the division
