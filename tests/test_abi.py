@@ -46,3 +46,15 @@ def test_null_arguments_are_errors():
                                   None) != abi.DSMC_OK
     with pytest.raises(abi.DSMCError):
         abi.check(lib.dsmc_sync(None))
+
+
+def test_batch_null_arguments_are_errors():
+    """The sweep-monitoring and retirement entry points reject a null batch
+    or null outputs before touching any device state."""
+    lib = abi.load()
+    keep = (abi.C.c_int * 1)(0)
+    out = (abi.C.c_double * 4)()
+    assert lib.dsmc_batch_download_counts(None, out, None) == abi.DSMC_ERR_ARG
+    assert lib.dsmc_batch_wall_distance(None, out) == abi.DSMC_ERR_ARG
+    assert lib.dsmc_batch_compact(None, keep, 1) == abi.DSMC_ERR_ARG
+    assert lib.dsmc_batch_destroy(None) == abi.DSMC_OK
