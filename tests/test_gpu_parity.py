@@ -15,10 +15,28 @@ import dsmc_testlib as T
 
 pytestmark = pytest.mark.gpu
 
+def _wedge():
+    w = cases.flow_case("wedge", mach=6.0, kn=0.02, tw_ratio=3.0, nx=64, ny=32, ppc=9.0, half_angle_deg=25.0,
+                        seed=21)
+    return w["params"], w["body"]
+
+
+def _box_diffuse():
+    """Closed box whose four faces are diffuse walls at 200, 300, 400 and
+    500 K (the face-wall re-emission path of the move)."""
+    p, body = T.small_box(16, 8, seed=9)
+    for f, Tf in enumerate((200.0, 300.0, 400.0, 500.0)):
+        p.face_bc[f] = abi.BC_DIFFUSE
+        p.face_T[f] = Tf
+    return p, body
+
+
 CASES = {
     "box16": lambda: T.small_box(16, 8, seed=7),
     "cyl80": lambda: T.small_cylinder(80, 40, 10, 64, seed=11),
     "cyl80_specular": lambda: T.small_cylinder(80, 40, 10, 64, seed=13, wall=abi.BC_SPECULAR),
+    "wedge64": _wedge,
+    "box16_diffuse": _box_diffuse,
 }
 
 
@@ -74,7 +92,7 @@ def test_phase_parity(gpu_lib, name):
     r.close()
 
 
-@pytest.mark.parametrize("name,steps", [("box16", 30), ("cyl80", 30)])
+@pytest.mark.parametrize("name,steps", [("box16", 30), ("cyl80", 30), ("wedge64", 60), ("box16_diffuse", 40)])
 def test_trajectory_bitexact(gpu_lib, name, steps):
     """Many full steps on both sides from the same fill: identical states."""
     p, body, oc, info, tables = _make(name)
