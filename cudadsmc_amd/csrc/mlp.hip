@@ -46,7 +46,7 @@ int fail(int code, const std::string &m)
         if (e_ != hipSuccess) return fail(-2, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum Epi { EPI_SILU = 0, EPI_LINEAR = 1, EPI_DSILU = 2, EPI_ATOMIC = 3 };
+enum Epi { EPI_SILU = 0, EPI_LINEAR = 1, EPI_DSILU = 2, EPI_SPLIT = 3 };
 
 struct GemmArgs {
     const bf16 *A;
@@ -55,7 +55,7 @@ struct GemmArgs {
     int M, N, K, kchunk;
     const float *bias;       // SILU / LINEAR: [N]
     bf16 *Z, *Y, *YT;        // SILU: pre-activation and SiLU [M][N] (ld N), SiLU^T [N][M] (ld ldt)
-    float *C;                // LINEAR: fp32 [M][N];  ATOMIC: fp32 accumulator [M][N]
+    float *C;                // LINEAR: fp32 [M][N];  SPLIT: fp32 partials [gridDim.z][M][N]
     const bf16 *Zp;          // DSILU: pre-activation of the layer below [M][N]
     bf16 *D, *DT;            // DSILU: d loss / d Z below [M][N] and transposed [N][M] (ld ldt)
     int ldt;
@@ -81,8 +81,15 @@ template <int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 {
     using S = Shape<BM, BN>;
-    __shared__ __attribute__((aligned(16))) bf16 sA[BM * LDS_ROW];
-    __shared__ __attribute__((aligned(16))) bf16 sB[BN * LDS_ROW];
+    // the SiLU and dSiLU epilogues also write their tile transposed: it is
+    // staged in LDS (the k-loop's staging area, reused) and stored as
+    // contiguous 16-byte rows
+    constexpr bool TR = EPI == EPI_SILU || EPI == EPI_DSILU;
+    constexpr int TROW = BM + 8;                    // bf16 per transposed row: 16-B aligned, padded
+    constexpr int STAGE = (BM + BN) * LDS_ROW;      // one k-step's A and B tiles
+    constexpr int TSIZE = TR ? BN * TROW : 0;
+    __shared__ __attribute__((aligned(16))) bf16 smem[2 * STAGE > TSIZE ? 2 * STAGE : TSIZE];
+    bf16 *const sT = smem;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / S::WN, wn = wave % S::WN;
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
@@ -94,32 +101,53 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 #pragma unroll
         for (int b = 0; b < S::TN; ++b) acc[a][b] = f32x4{ 0.f, 0.f, 0.f, 0.f };
     const bf16x8 zero8 = { (bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f };
-    for (int k0 = kb; k0 < ke; k0 += KSTEP) {
-        for (int p = tid; p < BM * 4; p += 256) {
-            const int r = p >> 2, q = p & 3;
+    // two LDS stages: the tiles of k-step k+1 are loaded into registers while
+    // the MFMAs of step k run, then stored into the other stage (one barrier
+    // per k-step)
+    constexpr int NA = (BM * 4 + 255) / 256, NB = (BN * 4 + 255) / 256;
+    bf16x8 ra[NA], rb[NB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int p = tid + 256 * i, r = p >> 2, kk = k0 + 8 * (p & 3);
             bf16x8 v = zero8;
-            const int kk = k0 + 8 * q;
-            if (m0 + r < g.M) {
+            if (p < BM * 4 && m0 + r < g.M) {
                 const bf16 *src = g.A + (size_t)(m0 + r) * g.lda + kk;
                 if (kk + 8 <= g.K) v = *reinterpret_cast<const bf16x8 *>(src);
                 else
                     for (int j = 0; j < 8 && kk + j < g.K; ++j) v[j] = src[j];
             }
-            *reinterpret_cast<bf16x8 *>(sA + r * LDS_ROW + 8 * q) = v;
+            ra[i] = v;
         }
-        for (int p = tid; p < BN * 4; p += 256) {
-            const int r = p >> 2, q = p & 3;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int p = tid + 256 * i, r = p >> 2, kk = k0 + 8 * (p & 3);
             bf16x8 v = zero8;
-            const int kk = k0 + 8 * q;
-            if (n0 + r < g.N) {
+            if (p < BN * 4 && n0 + r < g.N) {
                 const bf16 *src = g.B + (size_t)(n0 + r) * g.ldb + kk;
                 if (kk + 8 <= g.K) v = *reinterpret_cast<const bf16x8 *>(src);
                 else
                     for (int j = 0; j < 8 && kk + j < g.K; ++j) v[j] = src[j];
             }
-            *reinterpret_cast<bf16x8 *>(sB + r * LDS_ROW + 8 * q) = v;
+            rb[i] = v;
+        }
+    };
+    if (kb < ke) fetch(kb);
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += KSTEP) {
+        bf16 *const sA = smem + buf * STAGE, *const sB = sA + BM * LDS_ROW;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int p = tid + 256 * i;
+            if (p < BM * 4) *reinterpret_cast<bf16x8 *>(sA + (p >> 2) * LDS_ROW + 8 * (p & 3)) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int p = tid + 256 * i;
+            if (p < BN * 4) *reinterpret_cast<bf16x8 *>(sB + (p >> 2) * LDS_ROW + 8 * (p & 3)) = rb[i];
         }
         __syncthreads();
+        if (k0 + KSTEP < ke) fetch(k0 + KSTEP);
         bf16x8 af[S::TM], bfr[S::TN];
 #pragma unroll
         for (int a = 0; a < S::TM; ++a)
@@ -134,35 +162,84 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 #pragma unroll
             for (int b = 0; b < S::TN; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
-        __syncthreads();
+        buf ^= 1;
     }
+    __syncthreads();      // every wave is done with the stages before the epilogue reuses them
 #pragma unroll
     for (int a = 0; a < S::TM; ++a)
 #pragma unroll
         for (int b = 0; b < S::TN; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int row = m0 + wm * S::TM * 16 + a * 16 + 4 * (lane >> 4) + i;
-                const int col = n0 + wn * S::TN * 16 + b * 16 + (lane & 15);
+                const int rl = wm * S::TM * 16 + a * 16 + 4 * (lane >> 4) + i;
+                const int cl = wn * S::TN * 16 + b * 16 + (lane & 15);
+                const int row = m0 + rl, col = n0 + cl;
                 if (row >= g.M || col >= g.N) continue;
                 const float v = acc[a][b][i];
                 const size_t o = (size_t)row * g.N + col;
                 if (EPI == EPI_SILU) {
                     const float z = v + g.bias[col];
-                    const float y = silu(z);
+                    const bf16 y = (bf16)silu(z);
                     g.Z[o] = (bf16)z;
-                    g.Y[o] = (bf16)y;
-                    if (g.YT) g.YT[(size_t)col * g.ldt + row] = (bf16)y;
+                    g.Y[o] = y;
+                    sT[cl * TROW + rl] = y;
                 } else if (EPI == EPI_LINEAR) {
                     g.C[o] = v + g.bias[col];
                 } else if (EPI == EPI_DSILU) {
-                    const float d = v * dsilu((float)g.Zp[o]);
-                    g.D[o] = (bf16)d;
-                    g.DT[(size_t)col * g.ldt + row] = (bf16)d;
+                    const bf16 d = (bf16)(v * dsilu((float)g.Zp[o]));
+                    g.D[o] = d;
+                    sT[cl * TROW + rl] = d;
                 } else {
-                    unsafeAtomicAdd(g.C + o, v);
+                    g.C[(size_t)blockIdx.z * g.M * g.N + o] = v;
                 }
             }
+    if constexpr (TR) {
+        bf16 *const dst = EPI == EPI_SILU ? g.YT : g.DT;
+        if (dst) {
+            __syncthreads();
+            for (int p = tid; p < BN * (BM / 8); p += 256) {
+                const int cl = p / (BM / 8), r8 = (p % (BM / 8)) * 8;
+                const int col = n0 + cl, row = m0 + r8;
+                if (col >= g.N || row >= g.M) continue;
+                bf16 *const o = dst + (size_t)col * g.ldt + row;
+                if (row + 8 <= g.M) *reinterpret_cast<bf16x8 *>(o) = *reinterpret_cast<const bf16x8 *>(sT + cl * TROW + r8);
+                else
+                    for (int j = 0; j < 8 && row + j < g.M; ++j) o[j] = sT[cl * TROW + r8 + j];
+            }
+        }
+    }
+}
+
+// Sum of the split-K partials: out[i] = sum over s of part[s][i] (n a
+// multiple of 4).  A block covers 256 outputs as 64 float4 columns; its four
+// waves take every fourth split (float4 loads, unrolled), and the four
+// partial sums are added in LDS in wave order, so the result is
+// deterministic.
+__global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ part, int splits, int n,
+                                                    float *__restrict__ out)
+{
+    __shared__ float4 s_p[256];
+    const int t = threadIdx.x, c = t & 63, ph = t >> 6;
+    const int i4 = blockIdx.x * 64 + c;
+    const bool in = 4 * i4 < n;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in) {
+#pragma unroll 4
+        for (int k = ph; k < splits; k += 4) {
+            const float4 v = *reinterpret_cast<const float4 *>(part + (size_t)k * n + 4 * (size_t)i4);
+            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+        }
+    }
+    s_p[t] = s;
+    __syncthreads();
+    if (ph == 0 && in) {
+        float4 r = s_p[c];
+        for (int q = 1; q < 4; ++q) {
+            const float4 v = s_p[q * 64 + c];
+            r.x += v.x; r.y += v.y; r.z += v.z; r.w += v.w;
+        }
+        *reinterpret_cast<float4 *>(out + 4 * (size_t)i4) = r;
+    }
 }
 
 // Minibatch rows: index from Philox (with replacement), or the identity.
@@ -217,23 +294,52 @@ __global__ void k_loss(const float *__restrict__ out, const float *__restrict__ 
     }
 }
 
-// Bias gradient: row sums of dZ^T [N][B], one block per output neuron.
-__global__ void k_rowsum(const bf16 *__restrict__ DT, int B, float *__restrict__ gb)
+// Bias gradient, first stage: column sums of dZ [B][Np] (row-major, Np a
+// multiple of 32) over `rows` rows per block, written to part[block][Np]
+// (k_sum_splits adds the blocks).  Threads read 16-byte runs of 8 columns,
+// Np/8 threads per row and 256/(Np/8) rows at a time, so every row is read
+// contiguously; the row phases are reduced in LDS.
+__global__ __launch_bounds__(256) void k_colsum(const bf16 *__restrict__ D, int B, int Np, int rows,
+                                                float *__restrict__ part)
 {
-    __shared__ float s_w[4];
-    const bf16 *r = DT + (size_t)blockIdx.x * B;
-    float s = 0.f;
-    for (int b = threadIdx.x; b < B; b += blockDim.x) s += (float)r[b];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __shared__ float s_p[2048];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * rows, r1 = min(B, r0 + rows);
+    const int vpr = Np / 8;
+    float *const out = part + (size_t)blockIdx.x * Np;
+    if (vpr > 256) {                      // wide layers: one column per thread
+        for (int c = tid; c < Np; c += 256) {
+            float s = 0.f;
+            for (int r = r0; r < r1; ++r) s += (float)D[(size_t)r * Np + c];
+            out[c] = s;
+        }
+        return;
+    }
+    const int phases = 256 / vpr, v = tid % vpr, ph = tid / vpr;
+    float s[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+    if (ph < phases)
+#pragma unroll 4
+        for (int r = r0 + ph; r < r1; r += phases) {
+            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(D + (size_t)r * Np + 8 * v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += (float)x[j];
+        }
+    // phases * Np <= 2048 floats
+    if (ph < phases)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s_p[ph * Np + 8 * v + j] = s[j];
     __syncthreads();
-    if (threadIdx.x == 0) gb[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    for (int c = tid; c < Np; c += 256) {
+        float t = 0.f;
+        for (int k = 0; k < phases; ++k) t += s_p[k * Np + c];
+        out[c] = t;
+    }
 }
 
 // Adam on the real (unpadded) entries of one layer; writes the BF16 weights
 // and their transpose, clears the weight-gradient accumulator.
 __global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, float *__restrict__ gW,
-                       const float *__restrict__ gb, int nout, int nin, int Np, int Kp, float lr_t, float b1, float b2,
+                       float *__restrict__ gb, int nout, int nin, int Np, int Kp, float lr_t, float b1, float b2,
                        float eps, bf16 *__restrict__ W, bf16 *__restrict__ WT, float *__restrict__ bias)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -248,6 +354,7 @@ __global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__re
         gW[(size_t)o * Kp + i] = 0.f;
     } else {
         g = gb[p - nw];
+        gb[p - nw] = 0.f;
     }
     const float mm = b1 * m[p] + (1.f - b1) * g;
     const float vv = b2 * v[p] + (1.f - b2) * g * g;
@@ -294,6 +401,7 @@ struct dsmc_mlp_s {
     std::vector<float *> bias, gW, gb;                      // [Np], [Np][Kp], [Np]
     std::vector<bf16 *> Y, YT, Z, D, DT;                    // activations and gradients per level
     float *out = nullptr, *Tb = nullptr, *loss = nullptr;
+    float *ws = nullptr;                                    // split-K partials of the weight gradients
     float *dX = nullptr, *dT = nullptr;                     // resident training set
     int64_t nset = 0;
     int64_t adam_t = 0;
@@ -384,9 +492,18 @@ int backward(dsmc_mlp m, int B)
         g.K = B;
         const int ks = ksplit_for(B);
         g.kchunk = ((B + ks - 1) / ks + KSTEP - 1) / KSTEP * KSTEP;
-        g.C = m->gW[l];
-        HCHK(gemm_any<EPI_ATOMIC>(g, (B + g.kchunk - 1) / g.kchunk, m->st));
-        hipLaunchKernelGGL(k_rowsum, dim3(Np), dim3(256), 0, m->st, m->DT[l + 1], B, m->gb[l]);
+        const int splits = (B + g.kchunk - 1) / g.kchunk;
+        g.C = m->ws;
+        HCHK(gemm_any<EPI_SPLIT>(g, splits, m->st));
+        hipLaunchKernelGGL(k_sum_splits, dim3((Np * Kp + 255) / 256), dim3(256), 0, m->st, m->ws, splits, Np * Kp,
+                           m->gW[l]);      // Np, Kp multiples of 32
+        HCHK(hipGetLastError());
+        // bias gradient: per-block column sums into the workspace (free again
+        // after the weight-gradient sum), then their sum
+        const int rows = std::max(256, (B + 127) / 128), nblk = (B + rows - 1) / rows;
+        hipLaunchKernelGGL(k_colsum, dim3(nblk), dim3(256), 0, m->st, m->D[l + 1], B, Np, rows, m->ws);
+        HCHK(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_splits, dim3((Np + 255) / 256), dim3(256), 0, m->st, m->ws, nblk, Np, m->gb[l]);
         HCHK(hipGetLastError());
         if (l > 0) {
             GemmArgs d{};
@@ -495,6 +612,12 @@ int dsmc_mlp_create(const int *widths, int nlayers, int max_batch, dsmc_mlp *out
             (e = m->alloc(&m->gW[l], wsz)) || (e = m->alloc(&m->gb[l], m->Wp[l + 1])))
             return bad(e);
     }
+    {
+        // split-K workspace: the largest layer at the largest split count
+        size_t wsz = 1;
+        for (int l = 0; l < nlayers; ++l) wsz = std::max(wsz, (size_t)m->Wp[l + 1] * m->Wp[l]);
+        if ((e = m->alloc(&m->ws, wsz * (size_t)(ksplit_for(m->maxB) + 1)))) return bad(e);
+    }
     m->Y.assign(nlayers + 1, nullptr); m->YT.assign(nlayers + 1, nullptr); m->Z.assign(nlayers + 1, nullptr);
     m->D.assign(nlayers + 1, nullptr); m->DT.assign(nlayers + 1, nullptr);
     for (int l = 0; l <= nlayers; ++l) {
@@ -539,6 +662,7 @@ int dsmc_mlp_set_params(dsmc_mlp m, const float *params)
                            m->Wp[l + 1], m->Wp[l], m->W[l], m->WT[l], m->bias[l]);
         HCHK(hipGetLastError());
         HCHK(hipMemsetAsync(m->gW[l], 0, sizeof(float) * (size_t)m->Wp[l + 1] * m->Wp[l], m->st));
+        HCHK(hipMemsetAsync(m->gb[l], 0, sizeof(float) * (size_t)m->Wp[l + 1], m->st));
     }
     m->adam_t = 0;
     HCHK(hipStreamSynchronize(m->st));
@@ -597,7 +721,8 @@ int dsmc_mlp_gradient(dsmc_mlp m, const float *x, const float *t, int n, float *
             break;
         }
         for (int l = 0; l < m->L && !r; ++l)
-            if (hipMemsetAsync(m->gW[l], 0, sizeof(float) * (size_t)m->Wp[l + 1] * m->Wp[l], m->st) != hipSuccess)
+            if (hipMemsetAsync(m->gW[l], 0, sizeof(float) * (size_t)m->Wp[l + 1] * m->Wp[l], m->st) != hipSuccess ||
+                hipMemsetAsync(m->gb[l], 0, sizeof(float) * (size_t)m->Wp[l + 1], m->st) != hipSuccess)
                 r = fail(-2, "memset");
         if (r || (r = gather(m, dx, dt, n, n, 0, 0, false)) || (r = forward(m, n, true)) ||
             (r = loss_and_grad(m, n)) || (r = backward(m, n)))
