@@ -81,13 +81,11 @@ template <int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 {
     using S = Shape<BM, BN>;
-    // the SiLU and dSiLU epilogues also write their tile transposed: it is
-    // staged in LDS (the k-loop's staging area, reused) and stored as
-    // contiguous 16-byte rows
+    // the SiLU and dSiLU epilogues stage their outputs in LDS (see below)
     constexpr bool TR = EPI == EPI_SILU || EPI == EPI_DSILU;
     constexpr int TROW = BM + 8;                    // bf16 per transposed row: 16-B aligned, padded
     constexpr int STAGE = (BM + BN) * LDS_ROW;      // one k-step's A and B tiles
-    constexpr int TSIZE = TR ? BN * TROW : 0;
+    constexpr int TSIZE = TR ? (BN * TROW > BM * (BN + 8) ? BN * TROW : BM * (BN + 8)) : 0;
     __shared__ __attribute__((aligned(16))) bf16 smem[2 * STAGE > TSIZE ? 2 * STAGE : TSIZE];
     bf16 *const sT = smem;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -165,47 +163,116 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
         buf ^= 1;
     }
     __syncthreads();      // every wave is done with the stages before the epilogue reuses them
+    if constexpr (!TR) {
 #pragma unroll
-    for (int a = 0; a < S::TM; ++a)
+        for (int a = 0; a < S::TM; ++a)
 #pragma unroll
-        for (int b = 0; b < S::TN; ++b)
+            for (int b = 0; b < S::TN; ++b)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int rl = wm * S::TM * 16 + a * 16 + 4 * (lane >> 4) + i;
-                const int cl = wn * S::TN * 16 + b * 16 + (lane & 15);
-                const int row = m0 + rl, col = n0 + cl;
-                if (row >= g.M || col >= g.N) continue;
-                const float v = acc[a][b][i];
-                const size_t o = (size_t)row * g.N + col;
-                if (EPI == EPI_SILU) {
-                    const float z = v + g.bias[col];
-                    const bf16 y = (bf16)silu(z);
-                    g.Z[o] = (bf16)z;
-                    g.Y[o] = y;
-                    sT[cl * TROW + rl] = y;
-                } else if (EPI == EPI_LINEAR) {
-                    g.C[o] = v + g.bias[col];
-                } else if (EPI == EPI_DSILU) {
-                    const bf16 d = (bf16)(v * dsilu((float)g.Zp[o]));
-                    g.D[o] = d;
-                    sT[cl * TROW + rl] = d;
-                } else {
-                    g.C[(size_t)blockIdx.z * g.M * g.N + o] = v;
+                for (int i = 0; i < 4; ++i) {
+                    const int row = m0 + wm * S::TM * 16 + a * 16 + 4 * (lane >> 4) + i;
+                    const int col = n0 + wn * S::TN * 16 + b * 16 + (lane & 15);
+                    if (row >= g.M || col >= g.N) continue;
+                    const size_t o = (size_t)row * g.N + col;
+                    if (EPI == EPI_LINEAR) g.C[o] = acc[a][b][i] + g.bias[col];
+                    else g.C[(size_t)blockIdx.z * g.M * g.N + o] = acc[a][b][i];
                 }
+    } else {
+        // SiLU / SiLU' epilogues: every bf16 output goes through LDS (the
+        // k-loop stages, reused: row-major with rows of BN + 8, or
+        // transposed with rows of BM + 8) and leaves as contiguous 16-byte
+        // runs; the layer below's pre-activation comes in the same way.
+        // N is a multiple of BN here, so only rows need a bound.
+        bf16 *const sR = smem;
+        constexpr int RROW = BN + 8;
+        auto rl_of = [&](int a, int i) { return wm * S::TM * 16 + a * 16 + 4 * (lane >> 4) + i; };
+        auto cl_of = [&](int b) { return wn * S::TN * 16 + b * 16 + (lane & 15); };
+        auto store_rows = [&](bf16 *dst) {          // sR -> dst [M][N]
+            __syncthreads();
+            for (int p = tid; p < BM * (BN / 8); p += 256) {
+                const int r = p / (BN / 8), c8 = (p % (BN / 8)) * 8;
+                if (m0 + r < g.M)
+                    *reinterpret_cast<bf16x8 *>(dst + (size_t)(m0 + r) * g.N + n0 + c8) =
+                        *reinterpret_cast<const bf16x8 *>(sR + r * RROW + c8);
             }
-    if constexpr (TR) {
-        bf16 *const dst = EPI == EPI_SILU ? g.YT : g.DT;
-        if (dst) {
+            __syncthreads();
+        };
+        auto store_cols = [&](bf16 *dst) {          // sT -> dst [N][ldt], transposed
             __syncthreads();
             for (int p = tid; p < BN * (BM / 8); p += 256) {
-                const int cl = p / (BM / 8), r8 = (p % (BM / 8)) * 8;
-                const int col = n0 + cl, row = m0 + r8;
-                if (col >= g.N || row >= g.M) continue;
-                bf16 *const o = dst + (size_t)col * g.ldt + row;
-                if (row + 8 <= g.M) *reinterpret_cast<bf16x8 *>(o) = *reinterpret_cast<const bf16x8 *>(sT + cl * TROW + r8);
+                const int c = p / (BM / 8), r8 = (p % (BM / 8)) * 8;
+                const int row = m0 + r8;
+                if (row >= g.M) continue;
+                bf16 *const o = dst + (size_t)(n0 + c) * g.ldt + row;
+                if (row + 8 <= g.M) *reinterpret_cast<bf16x8 *>(o) = *reinterpret_cast<const bf16x8 *>(sT + c * TROW + r8);
                 else
-                    for (int j = 0; j < 8 && row + j < g.M; ++j) o[j] = sT[cl * TROW + r8 + j];
+                    for (int j = 0; j < 8 && row + j < g.M; ++j) o[j] = sT[c * TROW + r8 + j];
             }
+            __syncthreads();
+        };
+        if constexpr (EPI == EPI_SILU) {
+#pragma unroll
+            for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < S::TN; ++b) {
+                    const float bias = g.bias[n0 + cl_of(b)];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[a][b][i] += bias;        // acc now holds z
+                }
+#pragma unroll
+            for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < S::TN; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sR[rl_of(a, i) * RROW + cl_of(b)] = (bf16)silu(acc[a][b][i]);
+            store_rows(g.Y);
+            if (g.YT) {
+#pragma unroll
+                for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                    for (int b = 0; b < S::TN; ++b)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) sT[cl_of(b) * TROW + rl_of(a, i)] = (bf16)silu(acc[a][b][i]);
+                store_cols(g.YT);
+            }
+#pragma unroll
+            for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < S::TN; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sR[rl_of(a, i) * RROW + cl_of(b)] = (bf16)acc[a][b][i];
+            store_rows(g.Z);
+        } else {
+            // the layer below's pre-activation, row-major, into sR
+            for (int p = tid; p < BM * (BN / 8); p += 256) {
+                const int r = p / (BN / 8), c8 = (p % (BN / 8)) * 8;
+                if (m0 + r < g.M)
+                    *reinterpret_cast<bf16x8 *>(sR + r * RROW + c8) =
+                        *reinterpret_cast<const bf16x8 *>(g.Zp + (size_t)(m0 + r) * g.N + n0 + c8);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < S::TN; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc[a][b][i] = (float)(bf16)(acc[a][b][i] * dsilu((float)sR[rl_of(a, i) * RROW + cl_of(b)]));
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < S::TN; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sR[rl_of(a, i) * RROW + cl_of(b)] = (bf16)acc[a][b][i];
+            store_rows(g.D);
+#pragma unroll
+            for (int a = 0; a < S::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < S::TN; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sT[cl_of(b) * TROW + rl_of(a, i)] = (bf16)acc[a][b][i];
+            store_cols(g.DT);
         }
     }
 }

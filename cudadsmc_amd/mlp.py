@@ -124,3 +124,32 @@ class MLP:
         ms, fl = C.c_double(), C.c_double()
         _check(self.lib, self.lib.dsmc_mlp_train_stats(self.h, C.byref(ms), C.byref(fl)))
         return ms.value, fl.value
+
+
+def _train_rate(steps=100, batch=65536, width=256, depth=4, log=print):
+    """Time `steps` training steps of the surrogate's shape (7 -> width x depth
+    -> 4) on a synthetic smooth target, after 20 untimed steps."""
+    import time
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(1 << 20, 7)).astype(np.float32)
+    t = np.tanh(x[:, :4] + 0.5 * x[:, 3:7]).astype(np.float32)
+    net = MLP([7] + [width] * depth + [4], max_batch=batch, seed=1)
+    net.set_data(x, t)
+    net.train(20, batch=batch, lr=1e-3, seed=1)
+    t0 = time.perf_counter()
+    losses = net.train(steps, batch=batch, lr=1e-3, seed=2)
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ms, flop = net.train_stats()
+    log(f"train {steps} steps at batch {batch}: {wall:.3f} ms/step wall, {ms:.3f} ms/step (events), "
+        f"{flop / ms / 1e9:.1f} TFLOP/s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    net.close()
+    return ms, flop
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="time the surrogate's training step on the GPU")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=65536)
+    a = ap.parse_args()
+    _train_rate(a.steps, a.batch, log=lambda m: print(m, flush=True))
