@@ -78,8 +78,8 @@ def main():
                     help="run the launch-settings autotuner in the untimed warm-up (always on for sweep64)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
-              "(multi-GPU sharding is a design note, not built); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the hot path runs on a single MI355X "
+              "(sharding across GPUs is designed, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "sweep64":
         return bench_sweep(args)

@@ -477,7 +477,8 @@ int build_graph(dsmc_handle h)
     return DSMC_OK;
 }
 
-// Algorithmic bytes of the four kernels of one step (formulas in DESIGN.md).
+// Algorithmic bytes of the four kernels of one step: what each must read and
+// write at least, per particle, candidate, collision and cell, as below.
 void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, double out[4])
 {
     const double C = h->ncell;

@@ -53,11 +53,21 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def init_params(widths, seed=0):
+    """Glorot-uniform weights and zero biases (float32), in the flat layout
+    dsmc_mlp_set_params takes: per layer, W (out x in, row-major), then b."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i, j in zip(widths[:-1], widths[1:]):
+        a = np.sqrt(6.0 / (i + j))
+        parts += [rng.uniform(-a, a, i * j), np.zeros(j)]
+    return np.concatenate(parts).astype(np.float32)
+
+
 class MLP:
     """widths[0] inputs -> hidden widths (SiLU) -> widths[-1] outputs."""
 
     def __init__(self, widths, max_batch=65536, params=None, seed=0):
-        from oracle import mlp_oracle  # only for the default initialisation
         self.lib = _lib()
         self.widths = list(map(int, widths))
         w = (C.c_int * len(self.widths))(*self.widths)
@@ -67,7 +77,7 @@ class MLP:
         _check(self.lib, self.lib.dsmc_mlp_nparams(self.h, C.byref(n)))
         self.nparams = n.value
         self.max_batch = int(max_batch)
-        self.set_params(mlp_oracle.init_params(self.widths, seed) if params is None else params)
+        self.set_params(init_params(self.widths, seed) if params is None else params)
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

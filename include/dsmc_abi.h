@@ -39,7 +39,7 @@ extern "C" {
 
 /* Case parameters.  Field names follow the reference's src/Param.h:5-37
  * (boltz, mass, diam, Vtl, L1, L2, L3, N1, N2, T, T_wall, V_jet, tau, Fn);
- * the rest are the runtime knobs the BASELINE configs sweep. */
+ * the rest are the run settings a case or a sweep of cases varies. */
 typedef struct dsmc_params {
     double boltz;        /* J/K */
     double mass;         /* kg */

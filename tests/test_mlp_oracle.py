@@ -44,3 +44,19 @@ def test_adam_closed_form_first_step():
     g = np.array([1.0, -2.0, 0.5, 3.0, -0.25])
     q, m, v = M.adam(p, g, np.zeros(5), np.zeros(5), 1, 0.01, 0.9, 0.999, 1e-12)
     np.testing.assert_allclose(q, -0.01 * np.sign(g), rtol=1e-9)
+
+
+def test_package_initialisation_matches_the_oracle_layout():
+    """The package's default weights (cudadsmc_amd.mlp.init_params) have the
+    oracle's flat layout and Glorot bounds: every weight within
+    sqrt(6 / (fan_in + fan_out)) of zero, every bias zero."""
+    from cudadsmc_amd import mlp as P
+    widths = [7, 16, 12, 4]
+    p = P.init_params(widths, seed=3)
+    assert p.dtype == np.float32 and p.shape == M.init_params(widths, seed=3).shape
+    layers = M.unflatten(widths, p.astype(np.float64))
+    for (W, b), i, j in zip(layers, widths[:-1], widths[1:]):
+        assert W.shape == (j, i)
+        assert np.all(np.abs(W) <= np.sqrt(6.0 / (i + j)))
+        assert np.all(b == 0.0)
+    assert not np.array_equal(P.init_params(widths, seed=3), P.init_params(widths, seed=4))
