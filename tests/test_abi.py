@@ -58,3 +58,15 @@ def test_batch_null_arguments_are_errors():
     assert lib.dsmc_batch_wall_distance(None, out) == abi.DSMC_ERR_ARG
     assert lib.dsmc_batch_compact(None, keep, 1) == abi.DSMC_ERR_ARG
     assert lib.dsmc_batch_destroy(None) == abi.DSMC_OK
+
+
+@pytest.mark.parametrize("n", ["2", "8"])
+def test_bench_refuses_more_than_one_gpu(n):
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    so, before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", n], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
