@@ -143,7 +143,7 @@ static inline int cell_of(const or_case *cs, float x, float y)
 }
 
 /* Diffuse re-emission from a wall with unit normal n (into the fluid).
- * Reference: GPUSegment.cuh:59-84 and WallBoundary.cpp:317-327.  D7 fix:
+ * Reference: GPUSegment.cuh:59-84 and WallBoundary.cpp:33-43.  D7 fix:
  * independent draws -- normal speed from the flux-weighted (Rayleigh) law
  * v_n = sqrt(-ln U) sqrt(2kTw/m), tangential and z components N(0, kTw/m). */
 static void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
@@ -260,7 +260,8 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
         } else {
             const float *S = cs->seg + 6 * (kind - 4);
             if (cs->body_bc == OR_BC_SPECULAR) {
-                /* Segment.cpp:152 specular law v' = v - 2 (v.n) n */
+                /* specular law v' = v + 2 |v.n| n = v - 2 (v.n) n for v.n < 0
+                 * (Segment.cpp:40, WallBoundary.cpp:46) */
                 float vn = vx * S[4] + vy * S[5];
                 vx = vx - 2.0f * vn * S[4];
                 vy = vy - 2.0f * vn * S[5];
@@ -655,8 +656,11 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
         int *hist = (int *)calloc((size_t)nt * (size_t)C, sizeof(int));
 #pragma omp parallel num_threads(nt)
         {
+            /* the team may be smaller than asked for (OMP_THREAD_LIMIT,
+             * nested regions): split by the threads that actually run */
+            const int T = omp_get_num_threads();
             int tid = omp_get_thread_num();
-            int lo = (int)((long long)ntot * tid / nt), hi = (int)((long long)ntot * (tid + 1) / nt);
+            int lo = (int)((long long)ntot * tid / T), hi = (int)((long long)ntot * (tid + 1) / T);
             int *h = hist + (size_t)tid * C;
             for (int i = lo; i < hi; ++i) {
                 if (st->dead[i]) { st->cell[i] = -1; st->slot[i] = 0; continue; }
@@ -668,7 +672,7 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
 #pragma omp for schedule(static)
             for (int c = 0; c < C; ++c) {
                 int s = 0;
-                for (int t = 0; t < nt; ++t) { int v = hist[(size_t)t * C + c]; hist[(size_t)t * C + c] = s; s += v; }
+                for (int t = 0; t < T; ++t) { int v = hist[(size_t)t * C + c]; hist[(size_t)t * C + c] = s; s += v; }
                 st->count[c] = s;
             }
 #pragma omp barrier

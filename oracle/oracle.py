@@ -16,6 +16,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# A build of a deliberately altered copy of the oracle (the planted-mistake
+# check, tests/test_oracle_mutations.py) is loaded in place of the real one
+# when this names it.
+LIB_ENV = "DSMC_ORACLE_LIB"
 
 BC_STREAM, BC_SPECULAR, BC_DIFFUSE = 0, 1, 2
 SUB_MOVE, SUB_INJECT, SUB_COLLIDE, SUB_INIT = 1, 2, 3, 4
@@ -74,9 +78,12 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        build()
-    lib = C.CDLL(LIB_PATH)
+    path = os.environ.get(LIB_ENV)
+    if not path:
+        path = LIB_PATH
+        if not os.path.exists(path):
+            build()
+    lib = C.CDLL(path)
     P = C.POINTER(or_case)
     sig = {
         "or_philox": [_U, C.c_uint32, C.c_uint32, _U],

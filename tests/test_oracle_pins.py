@@ -282,6 +282,60 @@ def test_anisotropic_state_relaxes_to_isotropy():
     assert abs(ex / tot - 1 / 3) < 0.01
 
 
+def test_scattering_law_is_isotropic_vhs():
+    """One forced collision in each of 65,536 two-particle cells (remainder
+    1, (sigma c)max below any pair's, so the single candidate is accepted):
+    the post-collision relative velocity keeps |g| and the centre-of-mass
+    velocity (Particle.cpp:84-85,99-100) and points uniformly over the
+    sphere (cos chi = 2U - 1, phi = 2 pi U; Particle.cpp:87-95).  The law is
+    one of the ordered pair (this, other): g' = v_this - v_other.  The pair's
+    order is itself drawn at random, so g is oriented here by the first
+    particle of the pair, whose index i1 = floor(2 w0 / 2^32) comes from
+    the candidate's Philox draw (counter (0, step, collide substream, 0),
+    key (cell, seed); the generator is pinned by its known answers).
+    Every component of g'/|g'| has mean 0 and mean square 1/3, the
+    components are uncorrelated, and the deflection from the incoming g
+    has <cos chi> = 0, <cos^2 chi> = 1/3."""
+    oc, info, _ = _box_case(n_cells=128)
+    C = info.ncell
+    start = (2 * np.arange(C + 1)).astype(np.int32)
+    perm = np.arange(2 * C, dtype=np.int32)
+    rng = np.random.default_rng(17)
+    cells = np.arange(C)
+    g0s, g1s = [], []
+    for step in range(4):
+        v = [rng.normal(0, 300, 2 * C).astype(np.float32) for _ in range(3)]
+        w = [a.copy() for a in v]
+        sm = np.full(C, 1e-30, np.float32)
+        mr = np.ones(C, np.float32)
+        nc = oc.collide(step, start, perm, *w, sm, mr)
+        assert np.all(nc == 1)
+        ctr = np.zeros((C, 4), np.uint32)
+        ctr[:, 1] = step
+        ctr[:, 2] = O.SUB_COLLIDE
+        i1 = (O.philox(ctr, cells.astype(np.uint32), info.seed)[:, 0] >> 31).astype(np.int64)
+        p1, p2 = 2 * cells + i1, 2 * cells + 1 - i1
+        v64 = [a.astype(np.float64) for a in v]
+        w64 = [a.astype(np.float64) for a in w]
+        for a, b in zip(v64, w64):
+            np.testing.assert_allclose(a[p1] + a[p2], b[p1] + b[p2], rtol=0, atol=1e-3)
+        g0s.append(np.stack([a[p1] - a[p2] for a in v64], 1))
+        g1s.append(np.stack([b[p1] - b[p2] for b in w64], 1))
+    g0, g1 = np.concatenate(g0s), np.concatenate(g1s)
+    n = g0.shape[0]
+    m0, m1 = np.linalg.norm(g0, axis=1), np.linalg.norm(g1, axis=1)
+    np.testing.assert_allclose(m1, m0, rtol=3e-6)
+    d = g1 / m1[:, None]
+    for k in range(3):
+        assert _z(d[:, k].mean(), math.sqrt(1 / 3 / n)) < 4, k
+        assert _z(np.mean(d[:, k] ** 2) - 1 / 3, math.sqrt(4 / 45 / n)) < 4, k
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert _z(np.mean(d[:, a] * d[:, b]), math.sqrt(1 / 15 / n)) < 4, (a, b)
+    cosx = np.sum(g0 * g1, 1) / (m0 * m1)
+    assert _z(cosx.mean(), math.sqrt(1 / 3 / n)) < 4
+    assert _z(np.mean(cosx ** 2) - 1 / 3, math.sqrt(4 / 45 / n)) < 4
+
+
 # ---------------------------------------------------------------- sampling / fill
 def test_sample_sums_match_numpy():
     oc, info, _ = _box_case(n_cells=16)
@@ -345,6 +399,12 @@ def test_fill_counts_positions_and_moments():
     assert _z(parts["vy"].mean(), sig / math.sqrt(n)) < 4
     for k in ("vy", "vz"):
         assert _z(np.var(parts[k].astype(np.float64)) - sig * sig, sig * sig * math.sqrt(2 / n)) < 4
+    # the three components are independent normals (Random.cpp:30-46 draws
+    # each on its own; a Box-Muller pair gives two independent normals)
+    vs = [parts[k].astype(np.float64) for k in ("vx", "vy", "vz")]
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs(np.corrcoef(vs[a], vs[b])[0, 1]) < 4 / math.sqrt(n), (a, b)
+        assert abs(np.corrcoef(vs[a] ** 2, vs[b] ** 2)[0, 1]) < 4 / math.sqrt(n), (a, b)
 
 
 def test_no_particle_enters_the_body():
