@@ -127,6 +127,7 @@ SIGNATURES = {
     "dsmc_batch_download_samples": [_h, C.c_int, _f64p, _P(C.c_uint64)],
     "dsmc_batch_download_fields": [_h, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],
     "dsmc_batch_download_counts": [_h, _f64p, _P(C.c_uint64)],
+    "dsmc_batch_download_moments": [_h, _f64p, _P(C.c_uint64)],
     "dsmc_batch_wall_distance": [_h, _f64p],
     "dsmc_batch_compact": [_h, _i32p, C.c_int],
     "dsmc_batch_set_timing": [_h, C.c_int],

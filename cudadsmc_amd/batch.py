@@ -135,6 +135,15 @@ class Batch:
         check(self.lib.dsmc_batch_download_counts(self.h, ptr(out, C.c_double), ptr(ns, C.c_uint64)))
         return out, ns
 
+    def moments(self):
+        """The five moment sums of every cell of every instance -- rows sum 1,
+        sum vx, sum vy, sum vz, sum v^2; columns in instance order, one
+        transfer -- and the per-instance sampled-step counts."""
+        out = np.empty((5, int(self.cbase[-1])), np.float64)
+        ns = np.empty(self.n, np.uint64)
+        check(self.lib.dsmc_batch_download_moments(self.h, ptr(out, C.c_double), ptr(ns, C.c_uint64)))
+        return out, ns
+
     def wall_distance(self):
         """Signed distance (m) of every cell centre to its instance's body
         outline, negative inside (instance order; +inf without a body)."""

@@ -106,7 +106,8 @@ def cylinder_mach10(ppc=30, nx=400, ny=200, n_points=256, steps=20000, sample_fr
 def flow_case(geometry="cylinder", mach=10.0, kn=0.01, tw_ratio=2.5, nx=400, ny=200, ppc=30.0,
               half_angle_deg=20.0, T_inf=200.0, steps=20000, sample_from=10000, seed=20251203, n_points=256):
     """Supersonic argon flow over a cylinder (R = 0.1 m, centred) or a wedge
-    (length 0.3 m, apex at x = 0.15 m) in a 0.6 m x 0.3 m domain:
+    (apex at x = 0.15 m, length 0.3 m or less so that its base stays 1 cm
+    inside the domain) in a 0.6 m x 0.3 m domain:
     freestream inflow/outflow on all faces, diffuse wall at tw_ratio*T_inf,
     Kn on the body size (cylinder diameter, wedge length) with the
     hard-sphere mean free path, dt = 0.3 dx / (U + 3 vmp)."""
@@ -118,8 +119,11 @@ def flow_case(geometry="cylinder", mach=10.0, kn=0.01, tw_ratio=2.5, nx=400, ny=
         size = 0.2
         body = circle_polygon(n_points, 0.5 * L1, 0.5 * L2, 0.1)
     elif geometry == "wedge":
-        size = 0.3
-        body = wedge_polygon(0.15, 0.5 * L2, 0.3, half_angle_deg)
+        # 0.3 m long unless its base would come closer than 1 cm to the
+        # y faces (half-angles above about 25 deg): the body stays inside
+        # the domain, away from the stream faces
+        size = min(0.3, (0.5 * L2 - 0.01) / math.tan(math.radians(half_angle_deg)))
+        body = wedge_polygon(0.15, 0.5 * L2, size, half_angle_deg)
     else:
         raise ValueError(f"unknown geometry {geometry!r}")
     n = hard_sphere_density_for_mfp(kn * size)
@@ -157,7 +161,8 @@ def sweep(n=64, seed=20251203, particles=(2e5, 6e6), ppc=(30.0, 200.0), steps=20
         npart = float(math.exp(rng.uniform(math.log(particles[0]), math.log(particles[1]))))
         target = npart / float(math.exp(rng.uniform(math.log(ppc[0]), math.log(ppc[1]))))
         nx, ny = min(SWEEP_GRIDS, key=lambda g: abs(math.log(g[0] * g[1] / target)))
-        solid = 0.3 * 0.3 * math.tan(math.radians(half)) if geom == "wedge" else math.pi * 0.01
+        wl = min(0.3, 0.14 / math.tan(math.radians(half)))
+        solid = wl * wl * math.tan(math.radians(half)) if geom == "wedge" else math.pi * 0.01
         fluid_cells = nx * ny * (1.0 - solid / 0.18)
         cfg = flow_case(geom, mach, kn, tw, nx, ny, ppc=npart / fluid_cells, half_angle_deg=half,
                         steps=steps, sample_from=sample_from, seed=int(rng.integers(1, 2**62)),

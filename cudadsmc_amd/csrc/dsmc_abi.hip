@@ -657,6 +657,19 @@ int dsmc_restore_counters(dsmc_handle h, const dsmc_status *s)
     Ctrl c;
     if ((r = read_ctrl(h, c))) return r;
     if (s->next_pid < c.next_pid) return fail(DSMC_ERR_ARG, "next_pid below the uploaded particle ids");
+    // The current state lives in A on even steps and in B on odd ones; an
+    // upload always writes A (and sets step 0), so an odd step moves it.
+    if ((s->step & 1u) != (c.step & 1u) && c.n_live > 0) {
+        const SoA &from = (c.step & 1u) ? h->B : h->A, &to = (c.step & 1u) ? h->A : h->B;
+        const size_t b = sizeof(float) * (size_t)c.n_live;
+        HIPCHK(hipMemcpyAsync(to.x, from.x, b, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(to.y, from.y, b, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(to.vx, from.vx, b, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(to.vy, from.vy, b, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(to.vz, from.vz, b, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(to.pid, from.pid, sizeof(uint32_t) * (size_t)c.n_live, hipMemcpyDeviceToDevice,
+                              h->stream));
+    }
     c.step = s->step;
     c.next_pid = s->next_pid;
     c.psteps = s->particle_steps;

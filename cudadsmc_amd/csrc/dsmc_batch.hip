@@ -595,6 +595,22 @@ int dsmc_batch_download_counts(dsmc_batch b, double *counts, uint64_t *nsamples)
     return DSMC_OK;
 }
 
+int dsmc_batch_download_moments(dsmc_batch b, double *sums, uint64_t *nsamples)
+{
+    if (!b || !sums) return fail(DSMC_ERR_ARG, "null argument");
+    int r = set_device(b);
+    if (r) return r;
+    std::vector<InstCtrl> ic(b->ninst);
+    // the first five accumulator blocks lie back to back over all cells
+    HIPCHK(hipMemcpyAsync(sums, b->acc, sizeof(double) * 5 * (size_t)b->ncell, hipMemcpyDeviceToHost, b->stream));
+    if (nsamples)
+        HIPCHK(hipMemcpyAsync(ic.data(), b->ictl, sizeof(InstCtrl) * b->ninst, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (nsamples)
+        for (int i = 0; i < b->ninst; ++i) nsamples[i] = ic[i].nsamples;
+    return DSMC_OK;
+}
+
 int dsmc_batch_wall_distance(dsmc_batch b, double *d)
 {
     if (!b || !d) return fail(DSMC_ERR_ARG, "null argument");

@@ -271,6 +271,17 @@ __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, 
                 // vertex must not slip between two rounded tests
                 float u = (wx * vy - wy * vx) / den;
                 if (u < -SEG_TOL || u > 1.0f + SEG_TOL) continue;
+                // past the segment's own end (in the overlap): a hit only for
+                // a flight that also enters the solid side of the segment
+                // sharing that vertex (one closed polygon, segment k from
+                // vertex k to k+1).  The overlaps of the two faces of an acute
+                // vertex would otherwise bound a thin region outside the body
+                // in which a re-emitted particle bounces without end.
+                if (u < 0.0f || u > 1.0f) {
+                    const int nb = u > 1.0f ? (k + 1 == cs.nseg ? 0 : k + 1) : (k == 0 ? cs.nseg - 1 : k - 1);
+                    const float *Q = cs.seg + 6 * nb;
+                    if (!(vx * Q[4] + vy * Q[5] < 0.0f)) continue;
+                }
                 best = s;
                 kind = 4 + k;
             }

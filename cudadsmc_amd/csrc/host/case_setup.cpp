@@ -241,6 +241,10 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
     for (int k = 0; k < np; ++k) {
         poly[k] = V2{ body->xy[2 * k], body->xy[2 * k + 1] };
         if (!std::isfinite(poly[k].x) || !std::isfinite(poly[k].y)) return "non-finite body point";
+        // a body reaching a face would cover part of it: inflow there would
+        // start inside the body
+        if (poly[k].x < 0.0 || poly[k].x > P.in.L1 || poly[k].y < 0.0 || poly[k].y > P.in.L2)
+            return "body polygon must lie inside the domain [0, L1] x [0, L2]";
     }
     for (int k = 0; k < np; ++k) {
         const V2 &a = poly[k], &b = poly[(k + 1) % np];

@@ -702,6 +702,9 @@ int dsmc_mlp_destroy(dsmc_mlp m)
     if (!m) return 0;
     if (m->st) (void)hipStreamSynchronize(m->st);
     for (void *p : m->allocs) (void)hipFree(p);
+    // the training set is owned apart from allocs: set_data replaces it
+    if (m->dX) (void)hipFree(m->dX);
+    if (m->dT) (void)hipFree(m->dT);
     for (auto &ev : m->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (m->st) (void)hipStreamDestroy(m->st);
@@ -844,14 +847,26 @@ int dsmc_mlp_set_data(dsmc_mlp m, const float *x, const float *t, int64_t n)
 {
     if (!m || !x || !t || n < 1 || n >= (1LL << 32)) return fail(-1, "bad argument (1 <= n < 2^32)");
     const int nin = m->width[0], nout = m->width[m->L];
-    if (m->dX) { (void)hipFree(m->dX); m->dX = nullptr; }
-    if (m->dT) { (void)hipFree(m->dT); m->dT = nullptr; }
-    HCHK(hipMalloc(&m->dX, sizeof(float) * (size_t)n * nin));
-    HCHK(hipMalloc(&m->dT, sizeof(float) * (size_t)n * nout));
-    m->allocs.push_back(m->dX);
-    m->allocs.push_back(m->dT);
-    HCHK(hipMemcpy(m->dX, x, sizeof(float) * (size_t)n * nin, hipMemcpyHostToDevice));
-    HCHK(hipMemcpy(m->dT, t, sizeof(float) * (size_t)n * nout, hipMemcpyHostToDevice));
+    // the previous set is dropped first; on any failure below the model is
+    // left without data (dX = dT = null, nset = 0), never half-set
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    if (m->dX) (void)hipFree(m->dX);
+    if (m->dT) (void)hipFree(m->dT);
+    m->dX = nullptr;
+    m->dT = nullptr;
+    m->nset = 0;
+    float *dx = nullptr, *dt = nullptr;
+    hipError_t e = hipMalloc(&dx, sizeof(float) * (size_t)n * nin);
+    if (e == hipSuccess) e = hipMalloc(&dt, sizeof(float) * (size_t)n * nout);
+    if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(float) * (size_t)n * nin, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, t, sizeof(float) * (size_t)n * nout, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (dx) (void)hipFree(dx);
+        if (dt) (void)hipFree(dt);
+        return fail(-2, std::string("set_data: ") + hipGetErrorString(e));
+    }
+    m->dX = dx;
+    m->dT = dt;
     m->nset = n;
     return 0;
 }

@@ -255,6 +255,12 @@ int dsmc_batch_download_fields(dsmc_batch b, int i, double *n, double *ux, doubl
  * dsmc_batch_reset_samples calls, so differences of two reads give the
  * counts of the steps in between. */
 int dsmc_batch_download_counts(dsmc_batch b, double *counts, uint64_t *nsamples);
+/* The same for the five moment sums (no counterpart in the reference): sums[5
+ * * total cells] as five blocks -- sum 1, sum vx, sum vy, sum vz, sum v^2 --
+ * each over every cell of every instance in instance order, one transfer.
+ * Differences of two reads give the moments of the steps in between, from
+ * which a steady-state test of density, velocity and temperature follows. */
+int dsmc_batch_download_moments(dsmc_batch b, double *sums, uint64_t *nsamples);
 /* Signed distance (m) from every cell centre to its instance's body outline,
  * negative inside the body, +inf without a body (d[total cells], instance
  * order; no counterpart in the reference: an input feature of the learned

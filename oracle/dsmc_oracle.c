@@ -230,6 +230,19 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
                 }
                 float u = (wx * vy - wy * vx) / den;
                 if (u < -OR_SEG_TOL || u > 1.0f + OR_SEG_TOL) continue;
+                /* within the end overlap, past the segment's own end: a hit
+                 * only for a flight that also enters the solid side of the
+                 * segment sharing that vertex (the body is one closed polygon,
+                 * segment k from vertex k to k+1) -- only such a flight could
+                 * slip into the body between the two rounded tests.  Without
+                 * it, the overlaps of the two faces at an acute vertex bound a
+                 * thin region outside the body in which a re-emitted particle
+                 * can bounce from one overlap to the other without end. */
+                if (u < 0.0f || u > 1.0f) {
+                    int nb = u > 1.0f ? (k + 1 == cs->nseg ? 0 : k + 1) : (k == 0 ? cs->nseg - 1 : k - 1);
+                    const float *Q = cs->seg + 6 * nb;
+                    if (!(vx * Q[4] + vy * Q[5] < 0.0f)) continue;
+                }
                 best = s;
                 kind = 4 + k;
             }

@@ -203,3 +203,19 @@ def test_bad_body_is_rejected():
         describe(cfg["params"], cfg["body"][::-1].copy())
     with pytest.raises(DSMCError, match=">= 3"):
         describe(cfg["params"], cfg["body"][:2].copy())
+    out = cfg["body"].copy()
+    out[:, 1] += 0.12                  # the circle now crosses y = L2
+    with pytest.raises(DSMCError, match="inside the domain"):
+        describe(cfg["params"], out)
+
+
+def test_sweep_wedges_stay_inside_the_domain():
+    """Wedges of every half-angle of the sweep (up to 30 deg) keep their base
+    at least 1 cm from the y faces."""
+    for half in (10.0, 20.0, 26.0, 28.0, 30.0):
+        c = cases.flow_case("wedge", nx=64, ny=32, half_angle_deg=half)
+        b = c["body"]
+        assert b[:, 1].min() >= 0.01 - 1e-12 and b[:, 1].max() <= 0.29 + 1e-12, half
+        describe(c["params"], b)
+    for c in cases.sweep(64):
+        describe(c["params"], c["body"])

@@ -34,25 +34,33 @@ def test_forward_matches_oracle(gpu_lib):
     yb = O.forward(WIDE, p, x, bf16=True)
     yx = O.forward(WIDE, p, x, bf16=False)
     print(f"[mlp] forward rel L2 vs bf16 oracle {_rel(y, yb):.2e}, vs fp64 {_rel(y, yx):.2e}", flush=True)
-    assert _rel(y, yb) < 1e-2
-    assert _rel(y, yx) < 5e-2
+    # measured 1.5e-4 against the BF16-rounding oracle (accumulation order
+    # only) and 4.5e-3 against the unrounded network
+    assert _rel(y, yb) < 2e-3
+    assert _rel(y, yx) < 2e-2
     net.close()
 
 
-def test_gradient_matches_oracle(gpu_lib):
+@pytest.mark.parametrize("batch", [2048, 65536])
+def test_gradient_matches_oracle(gpu_lib, batch):
+    """Loss and every layer's weight and bias gradient against the
+    BF16-rounding FP64 oracle; batch 65,536 is the training batch, whose
+    weight gradients are summed over 64 split-K slices of 1,024 rows."""
     rng = np.random.default_rng(3)
     p = O.init_params(WIDE, seed=4)
     p = p + rng.normal(0, 0.02, p.size).astype(np.float32)
-    net = MLP(WIDE, max_batch=2048, params=p)
-    x = rng.normal(size=(2048, 7)).astype(np.float32)
-    t = rng.normal(size=(2048, 4)).astype(np.float32)
+    net = MLP(WIDE, max_batch=batch, params=p)
+    x = rng.normal(size=(batch, 7)).astype(np.float32)
+    t = rng.normal(size=(batch, 4)).astype(np.float32)
+    print(f"[mlp] gradient at batch {batch}", flush=True)
     loss, g = net.gradient(x, t)
     lo, go = O.loss_and_grad(WIDE, p, x, t, bf16=True)
-    assert abs(loss - lo) / lo < 1e-2
+    assert abs(loss - lo) / lo < 1e-3
+    # measured at most 2.5e-4 per layer
     for l, ((gw, gb), (ow, ob)) in enumerate(zip(_layers(WIDE, g), _layers(WIDE, go))):
         print(f"[mlp] layer {l}: dW rel {_rel(gw, ow):.2e}, db rel {_rel(gb, ob):.2e}", flush=True)
-        assert _rel(gw, ow) < 3e-2, l
-        assert _rel(gb, ob) < 3e-2, l
+        assert _rel(gw, ow) < 3e-3, l
+        assert _rel(gb, ob) < 3e-3, l
     net.close()
 
 
@@ -91,18 +99,37 @@ def test_adam_step_matches_closed_form(gpu_lib):
 
 
 def test_training_fits_a_smooth_field(gpu_lib):
-    """Minibatch Adam on the device-resident set lowers the loss of a smooth
-    4-output function of 7 inputs by more than 20x."""
+    """Minibatch Adam at the training batch of 65,536 on a device-resident
+    set lowers the loss of a smooth 4-output function of 7 inputs by more
+    than 20x."""
     rng = np.random.default_rng(9)
-    x = rng.uniform(-1, 1, size=(200000, 7)).astype(np.float32)
+    x = rng.uniform(-1, 1, size=(1 << 20, 7)).astype(np.float32)
     t = np.stack([np.sin(2 * x[:, 0]) * x[:, 2], np.cos(x[:, 1]) + 0.5 * x[:, 3], x[:, 4] * x[:, 5],
                   np.tanh(x[:, 6] + x[:, 0])], 1).astype(np.float32)
-    net = MLP(WIDE, max_batch=8192, seed=10)
+    net = MLP(WIDE, max_batch=65536, seed=10)
     net.set_data(x, t)
-    losses = net.train(300, batch=8192, lr=2e-3, seed=11)
+    losses = net.train(300, batch=65536, lr=2e-3, seed=11)
     ms, flop = net.train_stats()
     print(f"[mlp] loss {losses[0]:.4f} -> {losses[-1]:.5f}; {ms:.3f} ms/step, {flop / ms / 1e9:.1f} TFLOP/s", flush=True)
     assert losses[-1] < losses[0] / 20
     yh = net.forward(x[:4096])
     assert np.mean((yh - t[:4096]) ** 2) < losses[0] / 20
     net.close()
+
+
+def test_set_data_twice_then_destroy(gpu_lib):
+    """Replacing the training set frees the old one once: a second, smaller
+    set trains, and destroying the model afterwards frees each buffer once."""
+    rng = np.random.default_rng(12)
+    net = MLP(WIDE, max_batch=1024, seed=13)
+    for n in (50000, 20000):
+        x = rng.uniform(-1, 1, size=(n, 7)).astype(np.float32)
+        t = x[:, :4] * 0.5
+        net.set_data(x, t)
+        losses = net.train(20, batch=1024, lr=1e-3, seed=n)
+        assert np.all(np.isfinite(losses))
+    net.close()
+    again = MLP(WIDE, max_batch=1024, seed=14)      # the allocator is still sound
+    again.set_data(x, t)
+    assert np.all(np.isfinite(again.train(5, batch=1024)))
+    again.close()

@@ -201,14 +201,16 @@ def test_graph_dense_cells(gpu_lib, tile):
     r.close()
 
 
-def test_checkpoint_resume_bitexact(gpu_lib, tmp_path):
-    """Save after 20 steps, resume in a fresh handle: the next 17 steps
+@pytest.mark.parametrize("at", [20, 21])
+def test_checkpoint_resume_bitexact(gpu_lib, tmp_path, at):
+    """Save after 20 (even) or 21 (odd: the state then lives in the second
+    particle buffer) steps, resume in a fresh handle: the next 17 steps
     (graph-launched, sampling on) match the uninterrupted run exactly."""
     p, body = T.small_cylinder(80, 40, 10, 64, seed=23)
     p.sample_start = 10
     a = Run(p, body)
     a.initialize()
-    a.solver(20, graph=False)
+    a.solver(at, graph=False)
     f = tmp_path / "state.npz"
     a.save_state(str(f))
     a.solver(17, graph=True)

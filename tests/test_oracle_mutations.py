@@ -42,6 +42,10 @@ MUTATIONS = [
     ("first_segment_not_earliest", "the first segment hit in list order instead of the earliest",
      "                best = s;\n                kind = 4 + k;\n            }",
      "                best = s;\n                kind = 4 + k;\n                break;\n            }"),
+    ("end_overlap_any_flight", "segment end overlap taken by every flight, not only one entering the neighbour",
+     "if (!(vx * Q[4] + vy * Q[5] < 0.0f)) continue;", "if (0) continue;"),
+    ("no_end_overlap", "segment ends without overlap (a ray through a vertex slips between the tests)",
+     "if (u < -OR_SEG_TOL || u > 1.0f + OR_SEG_TOL) continue;", "if (u < 0.0f || u > 1.0f) continue;"),
     ("segment_reach_x10", "flights up to 10 cells test only their cell's segment list",
      "int all = reach > cs->margin;", "int all = reach > 10.0f * cs->margin;"),
     ("diffuse_vn_sigma", "diffuse normal speed scaled by sigma instead of v_mp",

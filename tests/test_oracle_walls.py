@@ -452,3 +452,72 @@ def test_diffuse_body_segment_emission_moments():
     ex_y = hy + vy * t_hit
     np.testing.assert_allclose(s["x"], ex_x, rtol=0, atol=1e-5 * p.L1)
     np.testing.assert_allclose(s["y"], ex_y, rtol=0, atol=1e-5 * p.L1)
+
+
+# ---------------------------------------------------------------- vertices
+@pytest.mark.parametrize("half", [10.0, 17.0])
+def test_no_trap_at_an_acute_apex(half):
+    """Particles just upstream of a wedge's apex, between the lines of its
+    two faces (within the faces' end overlap, a few um), with diffuse walls:
+    none needs the 64-event limit, and none ends inside the body.  This is
+    the region where a particle was re-emitted from one face's overlap into
+    the other's over and over (seen in batched sweep runs)."""
+    c = cases.flow_case("wedge", mach=5.0, kn=0.02, tw_ratio=3.0, nx=128, ny=64, ppc=30.0, half_angle_deg=half)
+    p, body = c["params"], c["body"]
+    oc, info, tables = O.case_from_params(p, body)
+    ax, ay = body[0]
+    a = math.radians(half)
+    n = 20000
+    rng = np.random.default_rng(1)
+    r = rng.uniform(1e-8, 1e-5 * np.hypot(*(body[1] - body[0])), n)
+    th = math.pi + rng.uniform(-a, a, n)
+    spd = rng.uniform(100, 3000, n)
+    ph = rng.uniform(0, 2 * math.pi, n)
+    s = dict(x=(ax + r * np.cos(th)).astype(np.float32), y=(ay + r * np.sin(th)).astype(np.float32),
+             vx=(spd * np.cos(ph)).astype(np.float32), vy=(spd * np.sin(ph)).astype(np.float32),
+             vz=np.zeros(n, np.float32), pid=np.arange(n, dtype=np.uint32))
+    dead, err = oc.move(s, 7)
+    assert err == 0
+    assert not _inside(s["x"].astype(np.float64), s["y"].astype(np.float64), body).any()
+
+
+@pytest.mark.parametrize("geometry", ["cylinder", "wedge"])
+def test_rays_through_vertices_do_not_enter_the_body(geometry):
+    """Flights aimed within 1e-7 m of a polygon vertex, from outside, in
+    directions spread over the body's interior angle there: every one is
+    reflected, none ends inside the body (the segment ends overlap so that a
+    ray through a shared vertex cannot pass between two rounded tests)."""
+    c = cases.flow_case(geometry, mach=5.0, kn=0.01, nx=80, ny=40, ppc=10.0, half_angle_deg=15.0, n_points=64)
+    p, body = c["params"], c["body"]
+    oc, info, tables = O.case_from_params(p, body)
+    rng = np.random.default_rng(5)
+    nv = body.shape[0]
+    per = 2000
+    xs, ys, vxs, vys = [], [], [], []
+    dt = float(np.float32(info.dt))
+    for k in range(nv):
+        V = body[k]
+        a_in = body[k - 1] - V                  # edges leaving V along the outline
+        b_in = body[(k + 1) % nv] - V
+        ta = math.atan2(a_in[1], a_in[0])
+        tb = math.atan2(b_in[1], b_in[0])
+        span = (ta - tb) % (2 * math.pi)        # interior angle, counter-clockwise from b to a
+        d = tb + span * rng.uniform(0.02, 0.98, per)   # directions into the body at V
+        spd = rng.uniform(500, 2000, per)
+        aim = V + rng.normal(0, 1e-7, (per, 2))
+        t_hit = dt * rng.uniform(0.2, 0.8, per)
+        xs.append(aim[:, 0] - np.cos(d) * spd * t_hit)
+        ys.append(aim[:, 1] - np.sin(d) * spd * t_hit)
+        vxs.append(np.cos(d) * spd)
+        vys.append(np.sin(d) * spd)
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    ok = ~_inside(x, y, body)
+    n = int(ok.sum())
+    s = dict(x=x[ok].astype(np.float32), y=y[ok].astype(np.float32), vx=np.concatenate(vxs)[ok].astype(np.float32),
+             vy=np.concatenate(vys)[ok].astype(np.float32), vz=np.zeros(n, np.float32),
+             pid=np.arange(n, dtype=np.uint32))
+    v0 = (s["vx"].copy(), s["vy"].copy())
+    dead, err = oc.move(s, 3)
+    assert err == 0 and n > 0.9 * per * nv
+    assert not _inside(s["x"].astype(np.float64), s["y"].astype(np.float64), body).any()
+    assert np.mean((s["vx"] != v0[0]) | (s["vy"] != v0[1])) > 0.999

@@ -128,3 +128,63 @@ def test_ramp_keeps_a_drift_from_passing_early():
         det.update(a, b)
     assert det.history[0] < 1.5 < det.history[1] < det.history[2]
     assert det.steady_at is None
+
+
+def _three_field_windows(C, windows, drift, seed=0, sigma=0.01):
+    """Half-window density, velocity (2 x C) and temperature fields with
+    noise of s.d. sigma; `drift` names the field that relaxes with time
+    constant 15 half-windows (the others stay steady)."""
+    rng = np.random.default_rng(seed)
+    base = _base(C)
+    out = []
+    for w in range(windows):
+        halves = []
+        for q in (0, 1):
+            k = 2 * w + q
+            r = 0.4 * math.exp(-k / 15.0)
+            n = base * (1 + (r if drift == "n" else 0)) + sigma * rng.normal(0, 1, C)
+            u = np.stack([base * (1 + (r if drift == "u" else 0)), 0.3 * base]) + sigma * rng.normal(0, 1, (2, C))
+            T = 2 * base * (1 + (r if drift == "T" else 0)) + sigma * rng.normal(0, 1, C)
+            halves.append([n, u, T])
+        out.append(halves)
+    return out
+
+
+@pytest.mark.parametrize("drift", ["n", "u", "T"])
+def test_each_field_is_watched(drift):
+    """A relaxation in any one of density, velocity and temperature, with
+    the other two steady, holds the detector back: it confirms well after a
+    run with no relaxation at all (a density-only detector would confirm a
+    velocity or temperature relaxation as early as the steady run)."""
+    at = {}
+    for which in ("n", "u", "T", None):
+        det = SteadyDetector(1.5, 3, lag=4)
+        for a, b in _three_field_windows(3000, 60, which, seed=3):
+            if det.update(a, b):
+                break
+        at[which] = det.steady_at
+    assert at[None] is not None and at[drift] is not None
+    assert at[drift] > at[None] + 5
+
+
+def test_count_weights_keep_sparse_cells_from_hiding_a_drift():
+    """Velocity noise falls as 1/sqrt(samples): a drift in the well-sampled
+    cells is seen under count weights even when a few sparse cells carry
+    100 times the noise; unweighted, the sparse cells' noise hides it."""
+    rng = np.random.default_rng(4)
+    C = 4000
+    counts = np.where(np.arange(C) < 200, 1.0, 400.0)
+    sig = 0.5 / np.sqrt(counts)
+    res = {}
+    for weighted in (True, False):
+        det = SteadyDetector(1.5, 3, lag=4)
+        for w in range(40):
+            halves = []
+            for q in (0, 1):
+                r = 0.02 * (2 * w + q)          # a steady linear drift, never settling
+                u = np.stack([1.0 + r * (counts > 1) + sig * rng.normal(0, 1, C), sig * rng.normal(0, 1, C)])
+                halves.append([u])
+            det.update(*halves, weights=[counts] if weighted else None)
+        res[weighted] = det.steady_at
+    assert res[True] is None
+    assert res[False] is not None
