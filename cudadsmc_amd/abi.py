@@ -71,6 +71,10 @@ class dsmc_launch(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class dsmc_grid(C.Structure):
+    _fields_ = [("N1", C.c_int), ("N2", C.c_int), ("L1", C.c_double), ("L2", C.c_double)]
+
+
 _P = C.POINTER
 _f32p = _P(C.c_float)
 _f64p = _P(C.c_double)
@@ -95,6 +99,8 @@ SIGNATURES = {
     "dsmc_upload_particles": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p],
     "dsmc_download_particles": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, _P(C.c_int64)],
     "dsmc_move": [_h],
+    "dsmc_bin_sort": [_P(dsmc_grid), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                      C.c_void_p, C.c_void_p],
     "dsmc_reassign": [_h],
     "dsmc_collide": [_h],
     "dsmc_run": [_h, C.c_int, C.c_int],

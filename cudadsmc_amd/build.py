@@ -20,6 +20,7 @@ SOURCES = [
     os.path.join(CSRC, "dsmc_abi.hip"),
     os.path.join(CSRC, "dsmc_batch.hip"),
     os.path.join(CSRC, "mlp.hip"),
+    os.path.join(CSRC, "dsmc_sort.hip"),
     os.path.join(CSRC, "host", "case_setup.cpp"),
 ]
 

@@ -198,6 +198,31 @@ int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples);
 int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,
                          double *uz, double *T, double *p);
 
+/* ---- Binning on caller device arrays -----------------------------------
+ * The reference's GPUCells::CalparticleNum -> CalparticleStartIndex ->
+ * GPUParticles::Sort chain (src/cell/Cell.cuh:14-20, GPUCell.cu:93-120,169-198;
+ * src/particle/Particle.cuh, GPUParticle.cu:142-147,365-373) on arrays the
+ * caller owns, with no handle: for n particles at FP32 (x, y), device
+ * pointers all,
+ *   cell[n]      cell id j + i*N2, i = (int)(x/dx), j = (int)(y/dy), clamped
+ *                into the grid (the reference leaves out-of-grid ids stale);
+ *   start[C+1]   exclusive scan of the per-cell counts (thrust::exclusive_scan
+ *                of d_particleNum), start[C] = n;
+ *   perm[n]      cell-sorted permutation, ascending particle index inside
+ *                each cell -- the serial order of the reference's atomicAdd
+ *                ranks, where the GPU's own order is arbitrary;
+ *   local_id[n]  rank of each particle in its cell in that order (d_localID);
+ *                may be NULL.
+ * dx = L1/N1, dy = L2/N2 in double, then rounded to FP32 (as a case does).
+ * stream: a hipStream_t, or NULL for the null stream; the call returns when
+ * the results are ready.  Scratch is allocated and freed per call. */
+typedef struct dsmc_grid {
+    int N1, N2;
+    double L1, L2;
+} dsmc_grid;
+int dsmc_bin_sort(const dsmc_grid *g, const float *x, const float *y, int64_t n, int *cell, int *start, int *perm,
+                  int *local_id, void *stream);
+
 /* Launch settings (no reference counterpart; the reference hard-codes
  * 128-thread blocks, GPUParticle.cu:60-74).  Results never depend on them --
  * only the time does -- which is what lets the autotuner change them.
