@@ -101,6 +101,8 @@ SIGNATURES = {
     "dsmc_move": [_h],
     "dsmc_bin_sort": [_P(dsmc_grid), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.c_void_p, C.c_void_p],
+    "dsmc_bin_sort_noalloc": [_P(dsmc_grid), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p],
     "dsmc_reassign": [_h],
     "dsmc_collide": [_h],
     "dsmc_run": [_h, C.c_int, C.c_int],

@@ -19,10 +19,14 @@
 // ascending gives the stable order whatever the order of the atomics.
 //
 // Scratch (per-cell counters, scan partials, the long-cell list, a control
-// word) comes from hipMalloc, one buffer per call.
+// word) comes from hipMalloc, one buffer per call, in dsmc_bin_sort; from a
+// static device array in dsmc_bin_sort_noalloc, for callers that cannot
+// allocate.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <climits>
+#include <mutex>
 
 #include "../../include/dsmc_abi.h"
 #include "host_common.h"
@@ -268,6 +272,35 @@ int launch_all(const BinGrid &bg, const float *x, const float *y, int n, int *ce
     return DSMC_OK;
 }
 
+// The allocation-free variant's scratch: one array per device (each GPU the
+// library runs on holds its own copy until the process ends).
+constexpr size_t kNoallocBytes = sizeof(int) * ((3 * (size_t)DSMC_BIN_SORT_MAX_CELLS +
+                                                 2 * (((size_t)DSMC_BIN_SORT_MAX_CELLS + SCAN_TILE - 1) / SCAN_TILE) +
+                                                 63) / 64 * 64) +
+                                 sizeof(Ctrl);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+std::array<void *, kMaxDevices> g_scratch_ptr{};   // device address of `scratch` on each device
+
+int scratch_on_device(void **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(DSMC_ERR_ARG, "bin_sort_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DSMC_ERR_HIP, std::string("bin_sort_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return DSMC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -287,6 +320,21 @@ int dsmc_bin_sort(const dsmc_grid *g, const float *x, const float *y, int64_t n,
     if (r) return r;
     if (e != hipSuccess) return fail(DSMC_ERR_HIP, std::string("bin_sort: ") + hipGetErrorString(e));
     return DSMC_OK;
+}
+
+int dsmc_bin_sort_noalloc(const dsmc_grid *g, const float *x, const float *y, int64_t n, int *cell, int *start,
+                          int *perm, int *local_id, void *stream)
+{
+    BinGrid bg;
+    int r = check_args(g, x, y, n, cell, start, perm, bg);
+    if (r) return r;
+    if (bg.ncell > DSMC_BIN_SORT_MAX_CELLS)
+        return fail(DSMC_ERR_ARG, "bin_sort_noalloc: at most " + std::to_string(DSMC_BIN_SORT_MAX_CELLS) +
+                                      " cells (N1*N2); dsmc_bin_sort takes larger grids");
+    void *p = nullptr;
+    if ((r = scratch_on_device(&p))) return r;
+    return launch_all(bg, x, y, (int)n, cell, start, perm, local_id, carve(p, bg.ncell),
+                      static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -222,6 +222,19 @@ typedef struct dsmc_grid {
 } dsmc_grid;
 int dsmc_bin_sort(const dsmc_grid *g, const float *x, const float *y, int64_t n, int *cell, int *start, int *perm,
                   int *local_id, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dsmc_bin_sort with the same arguments and results, whose
+ * scratch is a static device array of the library (about 12.6 MB per GPU,
+ * held until the process ends) instead of a per-call hipMalloc.  It calls no
+ * allocating routine; grids above DSMC_BIN_SORT_MAX_CELLS cells are rejected
+ * (DSMC_ERR_ARG).  It is asynchronous on `stream`: the results are ready when
+ * the stream reaches the end of the call.  All calls on one GPU share that
+ * GPU's scratch, so calls that overlap on one GPU (two streams, two threads)
+ * race; each call clears what it needs. */
+#define DSMC_BIN_SORT_MAX_CELLS 1048576
+int dsmc_bin_sort_noalloc(const dsmc_grid *g, const float *x, const float *y, int64_t n, int *cell, int *start,
+                          int *perm, int *local_id, void *stream);
 
 /* Launch settings (no reference counterpart; the reference hard-codes
  * 128-thread blocks, GPUParticle.cu:60-74).  Results never depend on them --
