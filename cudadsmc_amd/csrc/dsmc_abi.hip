@@ -59,6 +59,7 @@ struct dsmc_handle_s {
     int graph_steps = 0;
     hipEvent_t ev[5]{};
     hipEvent_t ev_flight[2]{};   // in-flight throttle of graph replays
+    std::vector<hipEvent_t> tev; // timing mode: 5 events per step of a ring
     int flight_k = 0;
     double t_ms[4]{}, t_bytes[4]{};
     int64_t t_launches = 0;
@@ -295,6 +296,8 @@ int dsmc_destroy(dsmc_handle h)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     for (int k = 0; k < 2; ++k)
         if (h->ev_flight[k]) (void)hipEventDestroy(h->ev_flight[k]);
+    for (auto &e : h->tev)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DSMC_OK;
@@ -477,22 +480,87 @@ int build_graph(dsmc_handle h)
     return DSMC_OK;
 }
 
-// Algorithmic bytes of the four kernels of one step: what each must read and
-// write at least, per particle, candidate, collision and cell, as below.
-void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, double out[4])
+// Algorithmic bytes of the four kernels over the K steps between two reads
+// of the counters -- what each must read and write at least, per particle,
+// candidate, collision and cell, as below -- added to the timing totals.
+// Sums over the steps: live at the start (the particle-steps), injected,
+// moved (live + injected), live at the end.
+void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
 {
-    const double C = h->ncell;
-    const double nl = before.n_live, nm = after.n_move, nn = after.n_live;
-    const double J = nm - nl;
-    const bool sampled = after.nsamples > before.nsamples;
-    out[0] = 40.0 * nl + 24.0 * J + 12.0 * nm + 4.0 * C;
-    out[1] = 8.0 * C;
-    out[2] = 12.0 * nm;
-    out[3] = 52.0 * nn + 32.0 * after.last_cand + 24.0 * after.last_coll + 16.0 * C +
-             (sampled ? 16.0 * nn + 96.0 * C : 0.0);
+    const double C = h->ncell, k = K;
+    const double snl = (double)(after.psteps - before.psteps);
+    const double sJ = (double)(after.next_pid - before.next_pid);
+    const double snm = snl + sJ;
+    const double snn = snl - (double)before.n_live + (double)after.n_live;
+    const double S = (double)(after.nsamples - before.nsamples);      // sampled steps
+    h->t_bytes[0] += 40.0 * snl + 24.0 * sJ + 12.0 * snm + 4.0 * C * k;
+    h->t_bytes[1] += 8.0 * C * k;
+    h->t_bytes[2] += 12.0 * snm;
+    h->t_bytes[3] += 52.0 * snn + 32.0 * (double)(after.ncand - before.ncand) +
+                     24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * k +
+                     (S > 0 ? 16.0 * snn * (S / k) + 96.0 * C * S : 0.0);
 }
 
 }  // namespace
+
+}  // extern "C"
+
+namespace {
+
+// Eager steps with HIP events between the four launches of every step (the
+// batched handle's timing mode, dsmc_batch.hip): a ring of kTimeRing steps'
+// events in flight, kernel durations added up as slots are reused, and
+// algorithmic bytes from the counters before and after the steps.
+constexpr int kTimeRing = 64;
+
+int harvest(dsmc_handle h, int slot)
+{
+    hipEvent_t *ev = &h->tev[5 * (size_t)slot];
+    HIPCHK(hipEventSynchronize(ev[4]));
+    for (int q = 0; q < 4; ++q) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+        h->t_ms[q] += ms;
+    }
+    return DSMC_OK;
+}
+
+int timed_steps(dsmc_handle h, int nsteps)
+{
+    if (h->tev.empty()) {
+        h->tev.resize(5 * (size_t)kTimeRing, nullptr);
+        for (auto &e : h->tev) HIPCHK(hipEventCreate(&e));
+    }
+    int r;
+    Ctrl before, after;
+    if ((r = read_ctrl(h, before))) return r;
+    hipStream_t st = h->stream;
+    for (int k = 0; k < nsteps; ++k) {
+        const int slot = k % kTimeRing;
+        if (k >= kTimeRing && (r = harvest(h, slot))) return r;
+        hipEvent_t *ev = &h->tev[5 * (size_t)slot];
+        HIPCHK(hipEventRecord(ev[0], st));
+        if ((r = launch_step(h, 1))) return r;
+        HIPCHK(hipEventRecord(ev[1], st));
+        HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
+        HIPCHK(hipEventRecord(ev[2], st));
+        HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
+                              h->tile_p, h->ctrl, h->grid_scatter, st));
+        HIPCHK(hipEventRecord(ev[3], st));
+        if ((r = launch_step(h, 4))) return r;
+        HIPCHK(hipEventRecord(ev[4], st));
+    }
+    for (int k = std::max(0, nsteps - kTimeRing); k < nsteps; ++k)
+        if ((r = harvest(h, k % kTimeRing))) return r;
+    if ((r = read_ctrl(h, after))) return r;
+    step_bytes(h, before, after, nsteps);
+    h->t_launches += nsteps;
+    return DSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int dsmc_run(dsmc_handle h, int nsteps, int graph)
 {
@@ -526,33 +594,7 @@ int dsmc_run(dsmc_handle h, int nsteps, int graph)
             if ((r = launch_step(h, 7))) return r;
         return DSMC_OK;
     }
-    for (int k = 0; k < nsteps; ++k) {
-        Ctrl before, after;
-        if ((r = read_ctrl(h, before))) return r;
-        hipStream_t st = h->stream;
-        HIPCHK(hipEventRecord(h->ev[0], st));
-        if ((r = launch_step(h, 1))) return r;
-        HIPCHK(hipEventRecord(h->ev[1], st));
-        HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
-        HIPCHK(hipEventRecord(h->ev[2], st));
-        HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
-                              h->tile_p, h->ctrl, h->grid_scatter, st));
-        HIPCHK(hipEventRecord(h->ev[3], st));
-        if ((r = launch_step(h, 4))) return r;
-        HIPCHK(hipEventRecord(h->ev[4], st));
-        HIPCHK(hipEventSynchronize(h->ev[4]));
-        if ((r = read_ctrl(h, after))) return r;
-        double by[4];
-        step_bytes(h, before, after, by);
-        for (int q = 0; q < 4; ++q) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, h->ev[q], h->ev[q + 1]));
-            h->t_ms[q] += ms;
-            h->t_bytes[q] += by[q];
-        }
-        h->t_launches += 1;
-    }
-    return DSMC_OK;
+    return timed_steps(h, nsteps);
 }
 
 int dsmc_sync(dsmc_handle h)

@@ -60,6 +60,7 @@ struct dsmc_batch_s {
     hipEvent_t ev_flight[2]{};
     int flight_k = 0;
     int timing = 0;
+    std::vector<hipEvent_t> tev;        // timing mode: 5 events per step of a ring
     double t_ms[4]{}, t_bytes[4]{};
     int64_t t_launches = 0;
     std::vector<void *> allocs;
@@ -213,6 +214,8 @@ int dsmc_batch_destroy(dsmc_batch b)
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : b->ev_flight)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : b->tev)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -421,6 +424,83 @@ int dsmc_batch_init_fields(dsmc_batch b, const double *n, const double *ux, cons
     return fill(b, e, fx, fy, sg);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Eager steps with HIP events between the four launches of every step, on
+// the batch's stream.  The events of the last kTimeRing steps are in flight
+// at once: before a step reuses its ring slot, the slot's previous step is
+// waited for and its kernel durations are added up, so the queue is
+// throttled without being drained.  Algorithmic bytes follow from the
+// device counters read once before and once after the steps (sums over the
+// steps of live, moved and new particle counts, candidates, collisions and
+// sampled steps), so no step waits for the host.
+constexpr int kTimeRing = 64;
+
+int harvest(dsmc_batch b, int slot)
+{
+    hipEvent_t *ev = &b->tev[5 * (size_t)slot];
+    HIPCHK(hipEventSynchronize(ev[4]));
+    for (int q = 0; q < 4; ++q) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+        b->t_ms[q] += ms;
+    }
+    return DSMC_OK;
+}
+
+int timed_steps(dsmc_batch b, int nsteps)
+{
+    if (b->tev.empty()) {
+        b->tev.resize(5 * (size_t)kTimeRing, nullptr);
+        for (auto &e : b->tev) HIPCHK(hipEventCreate(&e));
+    }
+    int r;
+    Ctrl before, after;
+    std::vector<InstCtrl> ib, ia;
+    if ((r = read_ctrl(b, before, &ib))) return r;
+    hipStream_t st = b->stream;
+    for (int k = 0; k < nsteps; ++k) {
+        const int slot = k % kTimeRing;
+        if (k >= kTimeRing && (r = harvest(b, slot))) return r;
+        hipEvent_t *ev = &b->tev[5 * (size_t)slot];
+        HIPCHK(hipEventRecord(ev[0], st));
+        for (int q = 0; q < 4; ++q) {
+            if ((r = launch_step(b, 1 << q))) return r;
+            HIPCHK(hipEventRecord(ev[q + 1], st));
+        }
+    }
+    for (int k = std::max(0, nsteps - kTimeRing); k < nsteps; ++k)
+        if ((r = harvest(b, k % kTimeRing))) return r;
+    if ((r = read_ctrl(b, after, &ia))) return r;
+    // sums over the steps: live at the start (the step's particle-steps),
+    // injected, moved (live + injected), live at the end
+    const double K = nsteps, C = b->ncell;
+    const double snl = (double)(after.psteps - before.psteps);
+    const double sJ = (double)(after.next_pid - before.next_pid);
+    const double snm = snl + sJ;
+    const double snn = snl - (double)before.n_live + (double)after.n_live;
+    double sampled = 0.0;
+    for (int i = 0; i < b->ninst; ++i) {
+        const double S = (double)(ia[i].nsamples - ib[i].nsamples);
+        if (S <= 0) continue;
+        const double nn_i = (double)(ia[i].psteps - ib[i].psteps) - ib[i].n_live + ia[i].n_live;
+        sampled += 16.0 * nn_i * (S / K) + 96.0 * (double)(b->cbase[i + 1] - b->cbase[i]) * S;
+    }
+    b->t_bytes[0] += 40.0 * snl + 24.0 * sJ + 12.0 * snm + 4.0 * C * K;
+    b->t_bytes[1] += 8.0 * C * K;
+    b->t_bytes[2] += 12.0 * snm;
+    b->t_bytes[3] += 52.0 * snn + 32.0 * (double)(after.ncand - before.ncand) +
+                     24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * K + sampled;
+    b->t_launches += nsteps;
+    return DSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int dsmc_batch_run(dsmc_batch b, int nsteps, int graph)
 {
     if (!b || nsteps < 0) return fail(DSMC_ERR_ARG, "bad argument");
@@ -444,32 +524,7 @@ int dsmc_batch_run(dsmc_batch b, int nsteps, int graph)
             if ((r = launch_step(b, 15))) return r;
         return DSMC_OK;
     }
-    for (int k = 0; k < nsteps; ++k) {
-        Ctrl before, after;
-        if ((r = read_ctrl(b, before))) return r;
-        hipStream_t st = b->stream;
-        HIPCHK(hipEventRecord(b->ev[0], st));
-        for (int q = 0; q < 4; ++q) {
-            if ((r = launch_step(b, 1 << q))) return r;
-            HIPCHK(hipEventRecord(b->ev[q + 1], st));
-        }
-        HIPCHK(hipEventSynchronize(b->ev[4]));
-        if ((r = read_ctrl(b, after))) return r;
-        const double C = b->ncell, nl = before.n_live, nm = after.n_move, nn = after.n_live;
-        double by[4];
-        by[0] = 40.0 * nl + 24.0 * (nm - nl) + 12.0 * nm + 4.0 * C;
-        by[1] = 8.0 * C;
-        by[2] = 12.0 * nm;
-        by[3] = 52.0 * nn + 32.0 * after.last_cand + 24.0 * after.last_coll + 16.0 * C;
-        for (int q = 0; q < 4; ++q) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]));
-            b->t_ms[q] += ms;
-            b->t_bytes[q] = by[q];
-        }
-        b->t_launches += 1;
-    }
-    return DSMC_OK;
+    return timed_steps(b, nsteps);
 }
 
 int dsmc_batch_sync(dsmc_batch b)
@@ -783,7 +838,7 @@ int dsmc_batch_kernel_times(dsmc_batch b, double ms[4], double bytes[4], int64_t
     if (!b) return fail(DSMC_ERR_ARG, "null batch");
     for (int q = 0; q < 4; ++q) {
         if (ms) ms[q] = b->t_launches ? b->t_ms[q] / (double)b->t_launches : 0.0;
-        if (bytes) bytes[q] = b->t_bytes[q];
+        if (bytes) bytes[q] = b->t_launches ? b->t_bytes[q] / (double)b->t_launches : 0.0;
     }
     if (launches) *launches = b->t_launches;
     return DSMC_OK;

@@ -383,6 +383,7 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
         fb, res = start(sim, pts, cfgs, tables, sur, check_steps=check_steps, log=log, detectors=det)
         t_init = time.perf_counter() - t0
         steps_at, wall = run_to_steady(sim, tables, window=window, log=log, tag=mode, retire=True, detectors=det)
+        chi_history = [[[round(c, 3) for c in f] for f in d.field_history] for d in det]
         sim.close()
         sim = Batch([c["params"] for c in cfgs], [c["body"] for c in cfgs])
         det = [SteadyDetector(**STEADY) for _ in pts]
@@ -401,6 +402,7 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
             ref = averaged_fields(sim, average_steps)
         err = sim.status(-1).err
         out[mode] = dict(steps_to_steady=steps_at.tolist(), wall_to_steady_s=wall + t_init, init_s=t_init,
+                         chi_history=chi_history,
                          wall_keep_all_s=wall_keep + t_init_keep,
                          fallback=fb.tolist(), residual=res.tolist(), fields=fields, ref=ref, err=int(err))
         sim.close()
@@ -441,7 +443,8 @@ def evaluate(surrogate, pts, window=100, average_steps=12000, settle_steps=6000,
                 # detector misfires: arms declared steady with a field error above 2 %
                 misfires_warm=[i for i, e in enumerate(errs) if max(e.values()) > 0.02],
                 misfires_plain=[i for i, e in enumerate(errs_plain) if max(e.values()) > 0.02],
-                window=window, average_steps=average_steps, settle_steps=settle_steps,
+                window=window, average_steps=average_steps, settle_steps=settle_steps, detector=dict(STEADY),
+                chi_history_plain=out["plain"]["chi_history"], chi_history_warm=out["warm"]["chi_history"],
                 wall_alone_plain_s=alone["plain"]["wall"], wall_alone_warm_s=alone["warm"]["wall"],
                 speedup_alone_median=float(np.median(ratio)), alone_steps_match_batch=bool(same),
                 speedup_alone_total=float(sum(alone["plain"]["wall"]) / sum(alone["warm"]["wall"])),
@@ -502,13 +505,14 @@ def pipeline(outdir, runs=1024, n_eval=32, fit_steps=8000, steady_steps=4000, sa
     return res
 
 
-def evaluate_saved(model, outdir, n_eval=32, window=100, grids="fine", log=print):
+def evaluate_saved(model, outdir, n_eval=32, window=100, grids="fine", average_steps=12000, log=print):
     """The held-out evaluation of a saved surrogate (no data generation)."""
     os.makedirs(outdir, exist_ok=True)
     sur = Surrogate(seed=3)
     sur.load(model)
     eval_pts = draw_points(n_eval, seed=97, grids=EVAL_GRIDS[grids])
-    res = evaluate(sur, eval_pts, window=window, log=log)
+    res = evaluate(sur, eval_pts, window=window, average_steps=average_steps, settle_steps=average_steps // 2,
+                   log=log)
     res["grids"] = grids
     with open(os.path.join(outdir, "warmstart.json"), "w") as f:
         json.dump(res, f, indent=1)
@@ -531,9 +535,15 @@ if __name__ == "__main__":
     ap.add_argument("--model", default=None, help="evaluate: a surrogate saved by pipeline")
     ap.add_argument("--window", type=int, default=100)
     ap.add_argument("--grids", default="fine", choices=sorted(EVAL_GRIDS))
+    ap.add_argument("--threshold", type=float, default=STEADY["threshold"], help="steady-state detector threshold")
+    ap.add_argument("--patience", type=int, default=STEADY["patience"])
+    ap.add_argument("--lag", type=int, default=STEADY["lag"])
+    ap.add_argument("--average", type=int, default=12000, help="evaluate: steps the fields are averaged over")
     a = ap.parse_args()
+    STEADY.update(threshold=a.threshold, patience=a.patience, lag=a.lag)
     say = lambda m: print(m, flush=True)
     if a.cmd == "pipeline":
         pipeline(a.out, runs=a.runs, n_eval=a.eval, fit_steps=a.fit_steps, grids=a.grids, log=say)
     else:
-        evaluate_saved(a.model, a.out, n_eval=a.eval, window=a.window, grids=a.grids, log=say)
+        evaluate_saved(a.model, a.out, n_eval=a.eval, window=a.window, grids=a.grids, average_steps=a.average,
+                       log=say)
