@@ -247,13 +247,14 @@ def bench_sweep(args):
         tuned, _ = autotune.tune(autotune.BatchTarget(b, lambda s: s.initialize(), warm_steps=5),
                                  autotune.device_cus(), steps=6, graph_steps_measure=32, log=log)
         b.initialize()
+    dev_graph = not args.profile_run
     done = 0
     while done < develop:
         k = min(250, develop - done)
-        b.solver(k, graph=True)
+        b.solver(k, graph=dev_graph)
         done += k
         log(f"[bench] {done}/{develop} development steps")
-    b.solver(args.warmup, graph=True)
+    b.solver(args.warmup, graph=dev_graph)
     s0 = b.status(-1)
     dt, kt = timed(b, steps, 50)
     s1 = b.status(-1)
@@ -263,9 +264,10 @@ def bench_sweep(args):
     sampling = all(b.status(i).nsamples > 0 for i in range(b.n))
     roof = roofline_of(kt, matching_traffic(max(kt, key=lambda k: kt[k]["ms"]), "sweep64", develop, s1.n_live,
                                             sampling))
-    graph = graph_phase(b, steps, 50, lambda: b.status(-1))
-    log(f"[bench] graph replay: {graph['value']:.4g} particle-steps/s")
-    cpu = None if args.no_cpu else cpu_baseline_batch(b, sw, args.cpu_seconds)
+    graph = None if args.profile_run else graph_phase(b, steps, 50, lambda: b.status(-1))
+    if graph:
+        log(f"[bench] graph replay: {graph['value']:.4g} particle-steps/s")
+    cpu = None if (args.no_cpu or args.profile_run) else cpu_baseline_batch(b, sw, args.cpu_seconds)
     first_timed = develop + args.warmup
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
@@ -278,13 +280,14 @@ def bench_sweep(args):
             "sampling_in_timed_steps": sampling, "collisions": s1.collisions - s0.collisions,
             "launch": "eager with per-kernel HIP events", "launch_settings": b.get_launch(),
             "autotuned": tuned is not None, "seed": 20251203,
+            "development_launch": "hipgraph" if dev_graph else "eager",
         },
         "graph_replay": graph,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     b.close()
-    if not args.no_steady:
+    if not (args.no_steady or args.profile_run):
         out["time_to_steady"] = time_to_steady(args)
     print(json.dumps(out))
 
@@ -345,9 +348,12 @@ def main():
     ap.add_argument("--steady-runs", type=int, default=512, help="DSMC runs that train the surrogate")
     ap.add_argument("--steady-eval", type=int, default=32, help="held-out cases")
     ap.add_argument("--steady-fit-steps", type=int, default=4000)
-    ap.add_argument("--steady-average", type=int, default=6000,
+    ap.add_argument("--steady-average", type=int, default=8000,
                     help="steps the fields are averaged over for the field errors")
     ap.add_argument("--tune", action="store_true", help="run the launch-settings autotuner first (untimed)")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for rocprofv3: development and warm-up launched eagerly, no graph phase, no CPU "
+                         "baseline, no time to steady -- the timed steps are then the last dispatches")
     ap.add_argument("--save-state", default=None, help="cylinder: checkpoint of the state the timed steps start from")
     args = ap.parse_args()
     if args.gpus != 1:
