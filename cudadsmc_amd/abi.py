@@ -75,6 +75,11 @@ class dsmc_grid(C.Structure):
     _fields_ = [("N1", C.c_int), ("N2", C.c_int), ("L1", C.c_double), ("L2", C.c_double)]
 
 
+class dsmc_cut_tables(C.Structure):
+    _fields_ = [("nseg", C.c_int), ("seg", C.POINTER(C.c_double)), ("frac", C.POINTER(C.c_double)),
+                ("cut", C.POINTER(C.c_uint8))]
+
+
 _P = C.POINTER
 _f32p = _P(C.c_float)
 _f64p = _P(C.c_double)
@@ -89,6 +94,7 @@ SIGNATURES = {
     "dsmc_device_count": [_i32p],
     "dsmc_device_cus": [_i32p],
     "dsmc_create": [_P(dsmc_params), _P(dsmc_body), _P(C.c_void_p)],
+    "dsmc_create_from_tables": [_P(dsmc_params), _P(dsmc_cut_tables), _P(C.c_void_p)],
     "dsmc_destroy": [_h],
     "dsmc_case_info_get": [_h, _P(dsmc_case_info)],
     "dsmc_case_tables": [_h, _f32p, _i32p, _i32p, _f64p, _f32p, _u8p, _f32p, _u64p],
@@ -99,6 +105,7 @@ SIGNATURES = {
     "dsmc_upload_particles": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p],
     "dsmc_download_particles": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, _P(C.c_int64)],
     "dsmc_move": [_h],
+    "dsmc_sample": [_h],
     "dsmc_bin_sort": [_P(dsmc_grid), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.c_void_p, C.c_void_p],
     "dsmc_bin_sort_noalloc": [_P(dsmc_grid), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,

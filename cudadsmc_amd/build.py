@@ -55,7 +55,8 @@ def build_library(verbose=False):
 
 
 HOST = os.path.join(PKG, "dsmc_run")
-HOST_SOURCES = [os.path.join(PKG, "cpp", f) for f in ("Param.cpp", "Geom.cpp", "Output.cpp", "Run.cpp", "main.cpp")]
+HOST_SOURCES = [os.path.join(PKG, "cpp", f)
+                for f in ("Param.cpp", "Geom.cpp", "CartesianMesh.cpp", "Output.cpp", "Run.cpp", "main.cpp")]
 
 
 def build_host():

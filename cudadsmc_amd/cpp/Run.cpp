@@ -9,6 +9,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "CartesianMesh.h"
 #include "Geom.h"
 
 namespace dsmc_host {
@@ -39,9 +40,10 @@ long arg_i(const char *s, const char *name)
     return v;
 }
 
-// Supersonic argon over a cylinder (R = 0.1 m, centred) or a wedge (length
-// 0.3 m, apex at x = 0.15 m) in a 0.6 m x 0.3 m domain: the case family the
-// sweeps draw from (cudadsmc_amd/cases.py flow_case).
+// Supersonic argon over a cylinder (R = 0.1 m, centred) or a wedge (apex at
+// x = 0.15 m, 0.3 m long or shorter so that its base stays 1 cm inside the
+// domain) in a 0.6 m x 0.3 m domain: the case family the sweeps draw from
+// (cudadsmc_amd/cases.py flow_case).
 Param flow_case(const std::string &geometry, double mach, double kn, double tw_ratio, int nx, int ny, double ppc,
                 double half_angle_deg)
 {
@@ -51,6 +53,8 @@ Param flow_case(const std::string &geometry, double mach, double kn, double tw_r
     p.T_wall = tw_ratio * p.T;
     p.geometry = geometry;
     p.wedge_half_angle_deg = half_angle_deg;
+    // 0.3 m long unless its base would come closer than 1 cm to the y faces
+    p.wedge_length = std::min(0.3, (0.5 * p.L2 - 0.01) / std::tan(half_angle_deg * M_PI / 180.0));
     const double size = geometry == "wedge" ? p.wedge_length : 2.0 * p.Radius;
     p.n_inf = hard_sphere_density_for_mfp(kn * size, p.diam);
     const double dx = p.L1 / nx;
@@ -77,7 +81,7 @@ Param parse_args(int argc, char **argv)
     std::string geometry = "cylinder";
     double mach = 10.0, kn = 0.01, tw = 2.5, half = 20.0, ppc = 30.0;
     int nx = 400, ny = 200;
-    long steps = -1, sample_from = -2, every = 0, seed = -1;
+    long steps = -1, sample_from = -2, every = 0, seed = -1, sample_every = 0;
     std::string out = "./res";
     bool eager = false, phases = false;
     for (int k = 1; k < argc; ++k) {
@@ -98,6 +102,7 @@ Param parse_args(int argc, char **argv)
         else if (a == "--steps") steps = arg_i(val(), "--steps");
         else if (a == "--sample-from") sample_from = arg_i(val(), "--sample-from");
         else if (a == "--output-every") every = arg_i(val(), "--output-every");
+        else if (a == "--sample-every") sample_every = arg_i(val(), "--sample-every");
         else if (a == "--out") out = val();
         else if (a == "--eager") eager = true;
         else if (a == "--phases") phases = true;
@@ -109,6 +114,9 @@ Param parse_args(int argc, char **argv)
     Param p = flow_case(geometry == "none" ? "cylinder" : geometry, mach, kn, tw, nx, ny, ppc, half);
     p.geometry = geometry;
     if (steps >= 0) p.steps = (int)steps;
+    if (sample_every < 0) throw std::invalid_argument("--sample-every must be >= 0");
+    p.sample_every = (int)sample_every;
+    if (sample_every > 0) p.sample_start = -1;   // the steps' own sampling off: Sample() instead
     if (sample_from >= -1) p.sample_start = (int)sample_from;
     if (seed >= 0) p.seed = (uint64_t)seed;
     p.output_every = (int)every;
@@ -132,9 +140,10 @@ void Run::initialize(const Param &p)
 {
     m_param = p;
     m_body = body_of(p);
-    const dsmc_params d = to_abi(p);
-    dsmc_body b{ (int)(m_body.size() / 2), m_body.empty() ? nullptr : m_body.data() };
-    check(dsmc_create(&d, m_body.empty() ? nullptr : &b, &m_h), "dsmc_create");
+    // mesh part (Run.cpp:37-62): the grid, then the cells cut against the body
+    m_mesh = std::make_unique<CartesianMesh>(p.N1, p.N2, p.L1, p.L2);
+    m_mesh->cutcell(m_body);
+    TransferCellsFromHostToDevice();
     m_output = std::make_unique<Output>(this);
     make_dir(p.out_dir);
     write_case(p.out_dir + "/case.json");
@@ -151,6 +160,24 @@ void Run::assignParticle(const double &coef)
     if (coef != 1.0) throw std::invalid_argument("assignParticle: only the full freestream fill (coef = 1)");
     check(dsmc_init_freestream(m_h), "dsmc_init_freestream");
 }
+
+void Run::TransferCellsFromHostToDevice()
+{
+    // the cut flags, fluid fractions and wall segments of the host mesh
+    // (GPUCells::UploadFromHost's ifCut and GPUSegment tables, Run.cpp:197-223);
+    // the handle that holds them on the device is created here
+    const dsmc_params d = to_abi(m_param);
+    const std::vector<double> &seg = m_mesh->segments();
+    dsmc_cut_tables t{ (int)(seg.size() / 6), seg.empty() ? nullptr : seg.data(), m_mesh->fractions().data(),
+                       m_mesh->ifcut().data() };
+    if (m_h) {
+        check(dsmc_destroy(m_h), "dsmc_destroy");
+        m_h = nullptr;
+    }
+    check(dsmc_create_from_tables(&d, &t, &m_h), "dsmc_create_from_tables");
+}
+
+void Run::Sample() { check(dsmc_sample(m_h), "dsmc_sample"); }
 
 void Run::particlemove() { check(dsmc_move(m_h), "dsmc_move"); }
 void Run::ressignParticle() { check(dsmc_reassign(m_h), "dsmc_reassign"); }
@@ -234,6 +261,17 @@ void Run::solver()
                     std::printf("step %d: move %.3f ms, reassign %.3f ms, collision %.3f ms, total %.3f ms\n", iter,
                                 ms(a, b), ms(b, c), ms(c, d), ms(a, d));
                 }
+            }
+        } else if (p.sample_every > 0) {
+            // GPUCells::Sample after every sample_every-th step, the first
+            // after step 0 (Run.cpp:294-296)
+            const int se = p.sample_every;
+            while (iter < next) {
+                const int s = (iter + se - 1) / se * se;     // next sampled step index >= iter
+                const int target = s < next ? s + 1 : next;
+                check(dsmc_run(m_h, target - iter, p.graph ? 1 : 0), "dsmc_run");
+                iter = target;
+                if ((iter - 1) % se == 0) Sample();
             }
         } else {
             check(dsmc_run(m_h, next - iter, p.graph ? 1 : 0), "dsmc_run");

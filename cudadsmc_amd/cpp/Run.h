@@ -7,10 +7,12 @@
 // collision() are the three phases of a step (Run.cpp:159-162, 245-256),
 // assignParticle() the initial fill (Run.cpp:108-157) and
 // TransferCellsFromDeviceToHost() brings the sampled fields back for
-// Output.  The device state lives behind a dsmc_handle (include/dsmc_abi.h),
-// so the reference's Transfer*FromHostToDevice steps happen inside
-// dsmc_create / dsmc_init_freestream.  Every ABI status is checked: a
-// failure throws std::runtime_error with dsmc_last_error().
+// Output.  The device state lives behind a dsmc_handle (include/dsmc_abi.h):
+// the host mesh (CartesianMesh: getIndex, cutcell) builds the cut-cell
+// tables and TransferCellsFromHostToDevice() hands them over
+// (dsmc_create_from_tables); the particle fill happens inside
+// dsmc_init_freestream; Sample() is GPUCells::Sample.  Every ABI status is
+// checked: a failure throws std::runtime_error with dsmc_last_error().
 #pragma once
 
 #include <memory>
@@ -18,6 +20,7 @@
 #include <vector>
 
 #include "../../include/dsmc_abi.h"
+#include "CartesianMesh.h"
 #include "Output.h"
 #include "Param.h"
 
@@ -34,7 +37,7 @@ public:
     // Options (all optional; the default is the Mach-10 cylinder benchmark):
     //   --case cylinder|wedge|none  --mach M --kn KN --tw RATIO --half-angle DEG
     //   --nx N1 --ny N2 --ppc P --seed S --steps K --sample-from STEP
-    //   --output-every K --out DIR --eager --phases
+    //   --sample-every K --output-every K --out DIR --eager --phases
     void initialize(int argc, char **argv);
     void initialize(const Param &p);
     /*============ Main loop ============*/
@@ -46,7 +49,10 @@ public:
     void particlemove();
     void ressignParticle();
     void collision();
+    void Sample();
+    void TransferCellsFromHostToDevice();
     void TransferCellsFromDeviceToHost();
+    const CartesianMesh &mesh() const { return *m_mesh; }
 
     const Param &param() const { return m_param; }
     const std::vector<double> &body() const { return m_body; }
@@ -59,6 +65,7 @@ protected:
     Param m_param;
     std::vector<double> m_body;
     dsmc_handle m_h = nullptr;
+    std::unique_ptr<CartesianMesh> m_mesh;
     FieldSet m_fields;
     std::unique_ptr<Output> m_output;
 

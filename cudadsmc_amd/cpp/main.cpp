@@ -5,11 +5,16 @@
 //
 // `dsmc_run --write-fields DIR` writes a synthetic field set through Output
 // (no device needed): a check of the .npz and .vts writers.
+// `dsmc_run --write-mesh FILE [case options]` writes the host mesh's cut-cell
+// tables (no device needed): a check against the library's own tables.
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
 
+#include "CartesianMesh.h"
+#include "Geom.h"
 #include "Run.h"
 
 namespace {
@@ -42,6 +47,26 @@ int write_fields(const char *dir)
     return 0;
 }
 
+// The host mesh of a case, cut against its body, as one binary file: int32
+// N1, N2, nseg, then N1*N2 float64 fluid fractions, N1*N2 uint8 cut flags
+// and 6*nseg float64 segment values (no device needed).
+int write_mesh(const char *path, int argc, char **argv)
+{
+    const dsmc_host::Param p = dsmc_host::parse_args(argc, argv);
+    dsmc_host::CartesianMesh m(p.N1, p.N2, p.L1, p.L2);
+    m.cutcell(dsmc_host::body_of(p));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot write ") + path);
+    const int32_t hdr[3] = { p.N1, p.N2, (int32_t)(m.segments().size() / 6) };
+    bool ok = std::fwrite(hdr, sizeof(hdr), 1, f) == 1;
+    ok = ok && std::fwrite(m.fractions().data(), sizeof(double), m.fractions().size(), f) == m.fractions().size();
+    ok = ok && std::fwrite(m.ifcut().data(), 1, m.ifcut().size(), f) == m.ifcut().size();
+    ok = ok && std::fwrite(m.segments().data(), sizeof(double), m.segments().size(), f) == m.segments().size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) throw std::runtime_error(std::string("write failed: ") + path);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -49,8 +74,17 @@ int main(int argc, char **argv)
     if (argc > 1 && (!std::strcmp(argv[1], "--help") || !std::strcmp(argv[1], "-h"))) {
         std::printf("usage: dsmc_run [--case cylinder|wedge|none] [--mach M] [--kn KN] [--tw RATIO]\n"
                     "                [--half-angle DEG] [--nx N1] [--ny N2] [--ppc P] [--seed S] [--steps K]\n"
-                    "                [--sample-from STEP] [--output-every K] [--out DIR] [--eager] [--phases]\n");
+                    "                [--sample-from STEP] [--sample-every K] [--output-every K] [--out DIR]\n"
+                    "                [--eager] [--phases]\n");
         return 0;
+    }
+    if (argc >= 3 && !std::strcmp(argv[1], "--write-mesh")) {
+        try {
+            return write_mesh(argv[2], argc - 2, argv + 2);
+        } catch (const std::exception &e) {
+            std::fprintf(stderr, "dsmc_run: %s\n", e.what());
+            return 1;
+        }
     }
     if (argc == 3 && !std::strcmp(argv[1], "--write-fields")) {
         try {

@@ -54,6 +54,7 @@ struct dsmc_handle_s {
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
     int tile_p = CELL_P, graph_len = 8;
     int keep_perm = 0, timing = 0;
+    bool cell_order = false;     // the current state is in cell order, start[] its (a step completed)
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
@@ -91,10 +92,14 @@ int launch_step(dsmc_handle h, int phase_mask)
         HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
                               h->tile_p, h->ctrl, h->grid_scatter, st));
     }
-    if (phase_mask & 4)
+    if (phase_mask & 4) {
         HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->tile_first, h->sigcmax, h->mrem,
                            h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->keep_perm, h->tile_p,
                            h->grid_cell, st));
+        h->cell_order = true;
+    } else if (phase_mask & 1) {
+        h->cell_order = false;
+    }
     return DSMC_OK;
 }
 
@@ -176,6 +181,7 @@ int fill(dsmc_handle h, const std::vector<double> &expected, const std::vector<f
     HIPCHK(launch_fill(h->cs, h->A, h->d_expected, h->d_cut, h->d_ux, h->d_uy, h->d_sig, h->count, h->lstart,
                        h->tsum, h->toff, h->ctrl, h->grid_fill, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    h->cell_order = false;
     return DSMC_OK;
 }
 
@@ -204,17 +210,25 @@ int dsmc_device_cus(int *n)
     return DSMC_OK;
 }
 
-int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
+}  // extern "C"
+
+namespace {
+
+// Creates a handle: parameters and cut-cell tables are derived and checked
+// on the host first (build fills the tables), then the device buffers are
+// allocated and the tables uploaded.
+template <class Build>
+int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
 {
     if (!p || !out) return fail(DSMC_ERR_ARG, "null argument");
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(DSMC_ERR_NODEVICE, "no HIP device");
     dsmc_handle h = new dsmc_handle_s();
     std::string msg = h->P.derive(*p);
     if (!msg.empty()) { delete h; return fail(DSMC_ERR_ARG, msg); }
-    msg = build_cut_tables(h->P, body, h->T);
+    msg = build(h->P, h->T);
     if (!msg.empty()) { delete h; return fail(DSMC_ERR_GEOMETRY, msg); }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { delete h; return fail(DSMC_ERR_NODEVICE, "no HIP device"); }
     const Param &P = h->P;
     const CutTables &T = h->T;
     h->ncell = T.ncell;
@@ -283,6 +297,21 @@ int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
     if (r) { std::string m = g_err; dsmc_destroy(h); return fail(r, m); }
     *out = h;
     return DSMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out)
+{
+    return create_handle(p, [&](const Param &P, CutTables &T) { return build_cut_tables(P, body, T); }, out);
+}
+
+int dsmc_create_from_tables(const dsmc_params *p, const dsmc_cut_tables *t, dsmc_handle *out)
+{
+    if (!t) return fail(DSMC_ERR_ARG, "null host tables");
+    return create_handle(p, [&](const Param &P, CutTables &T) { return build_cut_tables_from_host(P, t, T); }, out);
 }
 
 int dsmc_destroy(dsmc_handle h)
@@ -396,6 +425,7 @@ int dsmc_upload_particles(dsmc_handle h, int64_t n, const float *x, const float 
     HIPCHK(hipMemsetAsync(h->count, 0, sizeof(int) * h->ncell, h->stream));
     HIPCHK(launch_ctrl_reset(h->ctrl, (uint32_t)n, next, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    h->cell_order = false;
     return DSMC_OK;
 }
 
@@ -732,6 +762,17 @@ int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples)
     unsigned long long ns = nsamples;
     HIPCHK(hipMemcpyAsync(&h->ctrl->nsamples, &ns, sizeof(ns), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+int dsmc_sample(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    if (!h->cell_order)
+        return fail(DSMC_ERR_ARG, "dsmc_sample: the state is in cell order only after a completed step");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(launch_sample(h->cs, h->A, h->B, h->start, h->ncoll_cell, h->acc, h->ctrl, h->stream));
     return DSMC_OK;
 }
 

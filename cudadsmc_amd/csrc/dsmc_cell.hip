@@ -34,7 +34,6 @@ constexpr int CELL_MAXC = 128;       // cells whose metadata is staged at once
 constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
 constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
 constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
-constexpr int SAMPLE_G = 8;          // lanes per cell in the sampling sums
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

@@ -13,6 +13,7 @@ constexpr int CELL_P_MIN = 256;              // smallest tile size the launch se
 constexpr int CELL_CAP = 2048;               // LDS staging capacity of one tile
 constexpr int CELL_BIG = 64;                 // cells with > 64 particles listed per tile
 constexpr int CELL_WAVES = 4;                // waves per k_cell block
+constexpr int SAMPLE_G = 8;                  // lanes per cell in the FP64 sampling sums (k_cell, k_sample)
 
 hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, int2 *cellslot, int *count,
                        Ctrl *ctrl, int grid, hipStream_t st);
@@ -37,6 +38,10 @@ hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, 
                        const float *ux, const float *uy, const float *sig, int *count, int *lstart,
                        int *tsum, int *toff, Ctrl *ctrl, int grid, hipStream_t st);
 hipError_t launch_ctrl_reset(Ctrl *ctrl, uint32_t n, uint32_t next_pid, hipStream_t st);
+// One sample of the current (cell-ordered) state into acc, counted as a
+// sampled step: the sums of k_cell's sampling, in its order.
+hipError_t launch_sample(const DevCase &cs, const SoA &A, const SoA &B, const int *start, const int *ncoll_cell,
+                         double *acc, Ctrl *ctrl, hipStream_t st);
 
 // One instance of a wall-distance launch: its cells start at cbase in the
 // output; cell c = j + i*ny has its centre at ((i + 0.5) dx, (j + 0.5) dy);

@@ -15,6 +15,8 @@
 //                                                          GPUParticle.cu:76-140)
 // All launches have fixed grids (persistent, grid-stride over the
 // device-resident particle count), so a step can be captured in a hipGraph.
+#include <algorithm>
+
 #include "dsmc_kernels.h"
 
 namespace dsmc {
@@ -351,6 +353,54 @@ __global__ void k_ctrl_reset(Ctrl *ctrl, int use_n_next, uint32_t n, uint32_t ne
 }
 
 // ---------------------------------------------------------------------------
+// k_sample: one sample of the state after a step (GPUCells::Sample,
+// GPUCell.cu:147-167,336-402): the state is in cell order, cell c holding
+// [start[c], start[c+1]).  The sums are k_cell's: SAMPLE_G lanes per cell,
+// each over every SAMPLE_G-th particle, a fixed butterfly, one FP64 atomic
+// per sum from one lane -- so a sample taken here equals the one a step
+// takes by itself, bit for bit.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sample(DevCase cs, SoA A, SoA B, const int *__restrict__ start,
+                                                const int *__restrict__ ncoll_cell, double *__restrict__ acc,
+                                                Ctrl *__restrict__ ctrl)
+{
+    const uint32_t step = ctrl->step;
+    const SoA S = (step & 1u) ? B : A;
+    const int lane = lane_id(), g = lane / SAMPLE_G, sub = lane % SAMPLE_G;
+    constexpr int CPW = 64 / SAMPLE_G;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    const size_t C = (size_t)cs.ncell;
+    for (int base = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW; base < cs.ncell;
+         base += waves * CPW) {
+        const int c = base + g;
+        const int o = c < cs.ncell ? start[c] : 0;
+        const int k = c < cs.ncell ? start[c + 1] - o : 0;
+        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+        for (int q = sub; q < k; q += SAMPLE_G) {
+            const double u = (double)S.vx[o + q], v = (double)S.vy[o + q], w = (double)S.vz[o + q];
+            sx += u; sy += v; sz += w;
+            s2 += u * u + v * v + w * w;
+        }
+#pragma unroll
+        for (int d = SAMPLE_G / 2; d > 0; d >>= 1) {
+            sx += __shfl_xor(sx, d, SAMPLE_G);
+            sy += __shfl_xor(sy, d, SAMPLE_G);
+            sz += __shfl_xor(sz, d, SAMPLE_G);
+            s2 += __shfl_xor(s2, d, SAMPLE_G);
+        }
+        if (k > 0 && sub == 0) {
+            unsafeAtomicAdd(&acc[c], (double)k);
+            unsafeAtomicAdd(&acc[C + c], sx);
+            unsafeAtomicAdd(&acc[2 * C + c], sy);
+            unsafeAtomicAdd(&acc[3 * C + c], sz);
+            unsafeAtomicAdd(&acc[4 * C + c], s2);
+            unsafeAtomicAdd(&acc[5 * C + c], (double)ncoll_cell[c]);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->nsamples += 1ull;
+}
+
+// ---------------------------------------------------------------------------
 // k_move_batch: k_move for many independent instances in one launch.  Live
 // particles are ordered by instance (instance i at [P_i, P_i + n_i) with P
 // the prefix of the live counts); inflow particles follow all live ones in
@@ -495,6 +545,15 @@ hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, 
     if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_move_batch, dim3(grid), dim3(256), 0, st, cases, ninst, cbase, A, B, cellslot, count,
                        ctrl, ictl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(const DevCase &cs, const SoA &A, const SoA &B, const int *start, const int *ncoll_cell,
+                         double *acc, Ctrl *ctrl, hipStream_t st)
+{
+    const int cells_per_block = 4 * (64 / SAMPLE_G);
+    const int grid = std::max(1, std::min((cs.ncell + cells_per_block - 1) / cells_per_block, 8192));
+    hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, st, cs, A, B, start, ncoll_cell, acc, ctrl);
     return hipGetLastError();
 }
 

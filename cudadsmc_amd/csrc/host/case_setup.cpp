@@ -1,11 +1,15 @@
 // Host-side case construction (see case_setup.h).
 #include "case_setup.h"
 
+#include "geometry.h"
+
 #include <algorithm>
 #include <cmath>
 #include <sstream>
 
 namespace dsmc {
+
+using geom::V2;
 
 namespace {
 
@@ -86,166 +90,23 @@ std::string Param::derive(const dsmc_params &p)
 
 namespace {
 
-struct V2 { double x, y; };
-
-// Area of (polygon intersect axis-aligned rectangle), Sutherland-Hodgman.
-double clipped_area(const std::vector<V2> &poly, double x0, double y0, double x1, double y1)
+// Everything a case needs from its body outline `poly` (counter-clockwise,
+// checked by the callers): the wall segments -- from `seg_in` (6 per edge, as
+// a host mesh layer computed them) or from the outline's edges -- the
+// per-cell reach lists, the fluid fractions -- from `frac_in` (and the cut
+// flags `cut_in`) or by clipping -- and the fill helpers of the cut cells.
+std::string tables_from_outline(const Param &P, const std::vector<V2> &poly, const double *seg_in,
+                                const double *frac_in, const uint8_t *cut_in, CutTables &T)
 {
-    std::vector<V2> cur = poly, nxt;
-    auto clip = [&](auto inside, auto cross) {
-        nxt.clear();
-        size_t n = cur.size();
-        for (size_t i = 0; i < n; ++i) {
-            V2 a = cur[i], b = cur[(i + 1) % n];
-            bool ia = inside(a), ib = inside(b);
-            if (ia) nxt.push_back(a);
-            if (ia != ib) nxt.push_back(cross(a, b));
-        }
-        cur.swap(nxt);
-    };
-    auto xcut = [](double X) {
-        return [X](V2 a, V2 b) { double t = (X - a.x) / (b.x - a.x); return V2{ X, a.y + t * (b.y - a.y) }; };
-    };
-    auto ycut = [](double Y) {
-        return [Y](V2 a, V2 b) { double t = (Y - a.y) / (b.y - a.y); return V2{ a.x + t * (b.x - a.x), Y }; };
-    };
-    clip([&](V2 p) { return p.x >= x0; }, xcut(x0));
-    if (cur.empty()) return 0.0;
-    clip([&](V2 p) { return p.x <= x1; }, xcut(x1));
-    if (cur.empty()) return 0.0;
-    clip([&](V2 p) { return p.y >= y0; }, ycut(y0));
-    if (cur.empty()) return 0.0;
-    clip([&](V2 p) { return p.y <= y1; }, ycut(y1));
-    if (cur.size() < 3) return 0.0;
-    // shoelace relative to the cell corner (avoids cancellation)
-    double a = 0.0;
-    for (size_t i = 0; i < cur.size(); ++i) {
-        const V2 &p = cur[i], &q = cur[(i + 1) % cur.size()];
-        double px = p.x - x0, py = p.y - y0, qx = q.x - x0, qy = q.y - y0;
-        a += px * qy - qx * py;
-    }
-    return std::fabs(0.5 * a);
-}
-
-// Does segment a-b touch the closed rectangle [x0,x1]x[y0,y1]?  (Liang-Barsky)
-bool segment_hits_rect(V2 a, V2 b, double x0, double y0, double x1, double y1)
-{
-    double t0 = 0.0, t1 = 1.0;
-    double dx = b.x - a.x, dy = b.y - a.y;
-    const double p[4] = { -dx, dx, -dy, dy };
-    const double q[4] = { a.x - x0, x1 - a.x, a.y - y0, y1 - a.y };
-    for (int k = 0; k < 4; ++k) {
-        if (p[k] == 0.0) {
-            if (q[k] < 0.0) return false;
-        } else {
-            double r = q[k] / p[k];
-            if (p[k] < 0.0) { if (r > t1) return false; if (r > t0) t0 = r; }
-            else            { if (r < t0) return false; if (r < t1) t1 = r; }
-        }
-    }
-    return t0 <= t1;
-}
-
-bool point_in_polygon(const std::vector<V2> &poly, double px, double py)
-{
-    bool in = false;
-    const size_t n = poly.size();
-    for (size_t k = 0; k < n; ++k) {
-        const V2 &a = poly[k], &b = poly[(k + 1) % n];
-        if ((a.y > py) != (b.y > py)) {
-            double xi = a.x + (py - a.y) * (b.x - a.x) / (b.y - a.y);
-            if (px < xi) in = !in;
-        }
-    }
-    return in;
-}
-
-// Bounding box of (cell rectangle minus body): the rectangle corners outside
-// the body, the crossings of body edges with the rectangle boundary and the
-// body vertices inside the rectangle.  Grown by one part in 1e5 of the cell
-// and clipped to the cell.  Used to place fill particles in cut cells.
-void fluid_bbox(const std::vector<V2> &poly, double x0, double y0, double x1, double y1, float *out)
-{
-    double bx0 = 1e300, by0 = 1e300, bx1 = -1e300, by1 = -1e300;
-    auto add = [&](double x, double y) {
-        bx0 = std::min(bx0, x); by0 = std::min(by0, y);
-        bx1 = std::max(bx1, x); by1 = std::max(by1, y);
-    };
-    const double cx[4] = { x0, x1, x1, x0 }, cy[4] = { y0, y0, y1, y1 };
-    for (int k = 0; k < 4; ++k)
-        if (!point_in_polygon(poly, cx[k], cy[k])) add(cx[k], cy[k]);
-    const size_t n = poly.size();
-    for (size_t k = 0; k < n; ++k) {
-        const V2 &a = poly[k], &b = poly[(k + 1) % n];
-        if (a.x >= x0 && a.x <= x1 && a.y >= y0 && a.y <= y1) add(a.x, a.y);
-        // crossings with the four rectangle sides
-        for (int s = 0; s < 4; ++s) {
-            const bool vert = (s & 1) == 0;             // s = 0: x = x0, 2: x = x1; 1: y = y0, 3: y = y1
-            const double L = s == 0 ? x0 : s == 2 ? x1 : s == 1 ? y0 : y1;
-            const double pa = vert ? a.x : a.y, pb = vert ? b.x : b.y;
-            if ((pa - L) * (pb - L) > 0.0 || pa == pb) continue;
-            const double t = (L - pa) / (pb - pa);
-            const double q = vert ? a.y + t * (b.y - a.y) : a.x + t * (b.x - a.x);
-            const double lo = vert ? y0 : x0, hi = vert ? y1 : x1;
-            if (q < lo || q > hi) continue;
-            if (vert) add(L, q); else add(q, L);
-        }
-    }
-    if (bx0 > bx1) { bx0 = x0; bx1 = x1; by0 = y0; by1 = y1; }
-    const double gx = 1e-5 * (x1 - x0), gy = 1e-5 * (y1 - y0);
-    out[0] = (float)std::max(x0, bx0 - gx);
-    out[1] = (float)std::max(y0, by0 - gy);
-    out[2] = (float)std::min(x1, bx1 + gx);
-    out[3] = (float)std::min(y1, by1 + gy);
-}
-
-// Which sub-boxes of the 8x8 split of a fluid bounding box touch fluid
-// (are not entirely inside the body).  The fill draws a marked sub-box, then
-// a point in it, then rejects points inside the body: uniform over the fluid,
-// with an acceptance rate that does not collapse for thin fluid slivers.
-uint64_t fluid_mask(const std::vector<V2> &poly, const float *fb)
-{
-    const double x0 = fb[0], y0 = fb[1], w = ((double)fb[2] - x0) / 8.0, h = ((double)fb[3] - y0) / 8.0;
-    uint64_t m = 0;
-    for (int sy = 0; sy < 8; ++sy)
-        for (int sx = 0; sx < 8; ++sx) {
-            const double a0 = x0 + sx * w, b0 = y0 + sy * h;
-            const double solid = clipped_area(poly, a0, b0, a0 + w, b0 + h);
-            if (solid < w * h * (1.0 - 1e-9)) m |= 1ull << (sx + 8 * sy);
-        }
-    return m ? m : ~0ull;
-}
-
-}  // namespace
-
-std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T)
-{
-    const int nx = P.in.N1, ny = P.in.N2;
-    const int C = nx * ny;
-    T = CutTables{};
-    T.ncell = C;
-    T.margin = (float)std::min(P.dx, P.dy);
-    T.vol.assign(C, P.Vcell);
-    T.frac.assign(C, 1.0);
-    T.vol_ntc.assign(C, (float)P.Vcell);
-    T.cut.assign(C, 0);
-    T.fbox.assign(4 * (size_t)C, 0.0f);
-    T.fmask.assign(C, ~0ull);
-    T.cseg_start.assign(C + 1, 0);
-    if (!body || body->n_points == 0) return "";
-    if (body->n_points < 3 || !body->xy) return "body polygon needs >= 3 points";
-
-    const int np = body->n_points;
-    std::vector<V2> poly(np);
-    double area2 = 0.0;
-    for (int k = 0; k < np; ++k) {
-        poly[k] = V2{ body->xy[2 * k], body->xy[2 * k + 1] };
-        if (!std::isfinite(poly[k].x) || !std::isfinite(poly[k].y)) return "non-finite body point";
+    const int nx = P.in.N1, ny = P.in.N2, C = nx * ny, np = (int)poly.size();
+    for (const V2 &v : poly) {
+        if (!std::isfinite(v.x) || !std::isfinite(v.y)) return "non-finite body point";
         // a body reaching a face would cover part of it: inflow there would
         // start inside the body
-        if (poly[k].x < 0.0 || poly[k].x > P.in.L1 || poly[k].y < 0.0 || poly[k].y > P.in.L2)
+        if (v.x < 0.0 || v.x > P.in.L1 || v.y < 0.0 || v.y > P.in.L2)
             return "body polygon must lie inside the domain [0, L1] x [0, L2]";
     }
+    double area2 = 0.0;
     for (int k = 0; k < np; ++k) {
         const V2 &a = poly[k], &b = poly[(k + 1) % np];
         area2 += a.x * b.y - b.x * a.y;
@@ -257,10 +118,14 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
     T.nseg = np;
     T.seg.resize(6 * (size_t)np);
     for (int k = 0; k < np; ++k) {
+        float *s = &T.seg[6 * (size_t)k];
+        if (seg_in) {
+            for (int q = 0; q < 6; ++q) s[q] = (float)seg_in[6 * (size_t)k + q];
+            continue;
+        }
         const V2 &a = poly[k], &b = poly[(k + 1) % np];
         double ex = b.x - a.x, ey = b.y - a.y, L = std::hypot(ex, ey);
         if (!(L > 0.0)) return "degenerate body edge";
-        float *s = &T.seg[6 * (size_t)k];
         s[0] = (float)a.x; s[1] = (float)a.y; s[2] = (float)b.x; s[3] = (float)b.y;
         s[4] = (float)(ey / L); s[5] = (float)(-ex / L);
     }
@@ -279,7 +144,7 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
             for (int j = j0; j <= j1; ++j) {
                 double x0 = i * P.dx - grow, x1 = (i + 1) * P.dx + grow;
                 double y0 = j * P.dy - grow, y1 = (j + 1) * P.dy + grow;
-                if (segment_hits_rect(a, b, x0, y0, x1, y1)) lists[cell_id(i, j, ny)].push_back(k);
+                if (geom::segment_hits_rect(a, b, x0, y0, x1, y1)) lists[cell_id(i, j, ny)].push_back(k);
             }
     }
     for (int c = 0; c < C; ++c) {
@@ -287,34 +152,96 @@ std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T
         for (int k : lists[c]) T.cseg_idx.push_back(k);
     }
 
-    // Fluid volume of every cell: cell area minus its overlap with the body
-    // (the three wall-vertex cases of CartesianMesh.cpp:112-168, computed
-    // exactly by polygon clipping).
-    double bx0 = 1e300, bx1 = -1e300, by0 = 1e300, by1 = -1e300;
-    for (const V2 &v : poly) { bx0 = std::min(bx0, v.x); bx1 = std::max(bx1, v.x); by0 = std::min(by0, v.y); by1 = std::max(by1, v.y); }
-    const double cell_area = P.dx * P.dy;
-    int i0 = std::max(0, (int)std::floor(bx0 / P.dx) - 1), i1 = std::min(nx - 1, (int)std::floor(bx1 / P.dx) + 1);
-    int j0 = std::max(0, (int)std::floor(by0 / P.dy) - 1), j1 = std::min(ny - 1, (int)std::floor(by1 / P.dy) + 1);
-    for (int i = i0; i <= i1; ++i)
-        for (int j = j0; j <= j1; ++j) {
-            double ain = clipped_area(poly, i * P.dx, j * P.dy, (i + 1) * P.dx, (j + 1) * P.dy);
-            int c = cell_id(i, j, ny);
-            double fr = 1.0 - ain / cell_area;
-            if (fr > 1.0 - 1e-9) continue;  // uncut: keep exactly 1.0
-            if (fr < 1e-9) fr = 0.0;
+    // Fluid fraction of every cell (the three wall-vertex cases of
+    // CartesianMesh.cpp:112-168, computed exactly by polygon clipping), or
+    // the host mesh's own; then volumes and the fill helpers of cut cells.
+    if (frac_in) {
+        for (int c = 0; c < C; ++c) {
+            const double fr = frac_in[c];
+            if (!(fr >= 0.0 && fr <= 1.0)) return "host tables: fluid fractions must lie in [0, 1]";
+            const uint8_t ct = cut_in ? cut_in[c] : (uint8_t)(fr > 0.0 && fr < 1.0);
+            if (ct > 1 || (ct == 1) != (fr > 0.0 && fr < 1.0))
+                return "host tables: cut flags must mark exactly the cells with 0 < fraction < 1";
             T.frac[c] = fr;
+        }
+    } else {
+        geom::cell_fluid_fractions(poly, nx, ny, P.dx, P.dy, T.frac);
+    }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const int c = cell_id(i, j, ny);
+            const double fr = T.frac[c];
+            if (fr == 1.0) continue;    // uncut
             T.vol[c] = fr * P.Vcell;
             T.vol_ntc[c] = (float)std::max(T.vol[c], 0.1 * P.Vcell);
             T.cut[c] = fr > 0.0 ? 1 : 0;
             if (fr > 0.0) {
                 ++T.ncut;
-                fluid_bbox(poly, i * P.dx, j * P.dy, (i + 1) * P.dx, (j + 1) * P.dy, &T.fbox[4 * (size_t)c]);
-                T.fmask[c] = fluid_mask(poly, &T.fbox[4 * (size_t)c]);
+                geom::fluid_bbox(poly, i * P.dx, j * P.dy, (i + 1) * P.dx, (j + 1) * P.dy, &T.fbox[4 * (size_t)c]);
+                T.fmask[c] = geom::fluid_mask(poly, &T.fbox[4 * (size_t)c]);
             } else {
                 T.vol_ntc[c] = 0.0f;
             }
         }
     return "";
+}
+
+void empty_tables(const Param &P, CutTables &T)
+{
+    const int C = P.in.N1 * P.in.N2;
+    T = CutTables{};
+    T.ncell = C;
+    T.margin = (float)std::min(P.dx, P.dy);
+    T.vol.assign(C, P.Vcell);
+    T.frac.assign(C, 1.0);
+    T.vol_ntc.assign(C, (float)P.Vcell);
+    T.cut.assign(C, 0);
+    T.fbox.assign(4 * (size_t)C, 0.0f);
+    T.fmask.assign(C, ~0ull);
+    T.cseg_start.assign(C + 1, 0);
+}
+
+}  // namespace
+
+std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T)
+{
+    empty_tables(P, T);
+    if (!body || body->n_points == 0) return "";
+    if (body->n_points < 3 || !body->xy) return "body polygon needs >= 3 points";
+    std::vector<V2> poly(body->n_points);
+    for (int k = 0; k < body->n_points; ++k) poly[k] = V2{ body->xy[2 * k], body->xy[2 * k + 1] };
+    return tables_from_outline(P, poly, nullptr, nullptr, nullptr, T);
+}
+
+std::string build_cut_tables_from_host(const Param &P, const dsmc_cut_tables *h, CutTables &T)
+{
+    empty_tables(P, T);
+    if (!h) return "null host tables";
+    if (h->nseg == 0) {
+        if (h->frac)
+            for (int c = 0; c < T.ncell; ++c)
+                if (h->frac[c] != 1.0) return "host tables: without wall segments every fluid fraction is 1";
+        return "";
+    }
+    if (h->nseg < 3 || !h->seg) return "host tables: a wall outline needs >= 3 segments";
+    // the segments must form one closed outline in order, each with the unit
+    // normal (ey, -ex)/|e| into the fluid
+    const int n = h->nseg;
+    const double L = std::max(P.in.L1, P.in.L2);
+    std::vector<V2> poly(n);
+    for (int k = 0; k < n; ++k) {
+        const double *s = h->seg + 6 * (size_t)k, *t = h->seg + 6 * (size_t)((k + 1) % n);
+        for (int q = 0; q < 6; ++q)
+            if (!std::isfinite(s[q])) return "host tables: non-finite segment value";
+        if (std::hypot(s[2] - t[0], s[3] - t[1]) > 1e-9 * L)
+            return "host tables: segment " + std::to_string(k) + " does not end where the next one starts";
+        const double ex = s[2] - s[0], ey = s[3] - s[1], len = std::hypot(ex, ey);
+        if (!(len > 0.0)) return "host tables: degenerate segment " + std::to_string(k);
+        if (std::fabs(s[4] - ey / len) > 1e-6 || std::fabs(s[5] + ex / len) > 1e-6)
+            return "host tables: segment " + std::to_string(k) + " needs the unit normal (ey, -ex)/|e| into the fluid";
+        poly[k] = V2{ s[0], s[1] };
+    }
+    return tables_from_outline(P, poly, h->seg, h->frac, h->cut, T);
 }
 
 }  // namespace dsmc

@@ -54,6 +54,10 @@ struct CutTables {
 
 // Builds the cut-cell tables.  body may be null (no immersed body).
 std::string build_cut_tables(const Param &P, const dsmc_body *body, CutTables &T);
+// The same from tables a host mesh layer built (dsmc_create_from_tables):
+// its wall segments, fluid fractions and cut flags are taken as given (and
+// checked); the per-cell reach lists and fill helpers are derived.
+std::string build_cut_tables_from_host(const Param &P, const dsmc_cut_tables *h, CutTables &T);
 
 // Cell id of cell (i, j): the reference convention with nz = 1.
 inline int cell_id(int i, int j, int ny) { return j + i * ny; }

@@ -42,20 +42,49 @@ def describe(params, body=None):
     return info, t
 
 
+def outline_segments(body):
+    """Wall segments of a counter-clockwise body outline, 6 per edge (ax ay bx
+    by nx ny, n = (ey, -ex)/|e| into the fluid), in float64: the segment table
+    a host mesh layer hands to dsmc_create_from_tables."""
+    xy = np.asarray(body, dtype=np.float64).reshape(-1, 2)
+    a, b = xy, np.roll(xy, -1, axis=0)
+    e = b - a
+    L = np.hypot(e[:, 0], e[:, 1])
+    return np.ascontiguousarray(np.stack([a[:, 0], a[:, 1], b[:, 0], b[:, 1], e[:, 1] / L, -e[:, 0] / L], 1).ravel())
+
+
 class Run:
-    def __init__(self, params, body=None):
+    def __init__(self, params, body=None, tables=None):
+        """body: the body outline (x, y rows); or tables: host-built cut-cell
+        tables, a dict with seg (6 per wall segment, float64) and optional
+        frac (fluid fraction per cell) and cut (flags) -- GPUCells::
+        UploadFromHost's role (dsmc_create_from_tables)."""
         self.lib = abi.load()
         self.params = params
         self.h = C.c_void_p()
         self._body_xy = None
-        bptr = None
-        if body is not None:
-            xy = np.ascontiguousarray(np.asarray(body, dtype=np.float64).reshape(-1, 2))
-            self._body_xy = xy
-            b = abi.dsmc_body(xy.shape[0], xy.ctypes.data_as(C.POINTER(C.c_double)))
-            self._body = b
-            bptr = C.byref(b)
-        check(self.lib.dsmc_create(C.byref(params), bptr, C.byref(self.h)))
+        if tables is not None:
+            seg = np.ascontiguousarray(tables["seg"], dtype=np.float64)
+            self._tab = dict(seg=seg)
+            t = abi.dsmc_cut_tables(seg.size // 6, ptr(seg, C.c_double), None, None)
+            if tables.get("frac") is not None:
+                self._tab["frac"] = np.ascontiguousarray(tables["frac"], dtype=np.float64)
+                t.frac = ptr(self._tab["frac"], C.c_double)
+            if tables.get("cut") is not None:
+                self._tab["cut"] = np.ascontiguousarray(tables["cut"], dtype=np.uint8)
+                t.cut = ptr(self._tab["cut"], C.c_uint8)
+            if seg.size:
+                self._body_xy = seg.reshape(-1, 6)[:, :2].copy()
+            check(self.lib.dsmc_create_from_tables(C.byref(params), C.byref(t), C.byref(self.h)))
+        else:
+            bptr = None
+            if body is not None:
+                xy = np.ascontiguousarray(np.asarray(body, dtype=np.float64).reshape(-1, 2))
+                self._body_xy = xy
+                b = abi.dsmc_body(xy.shape[0], xy.ctypes.data_as(C.POINTER(C.c_double)))
+                self._body = b
+                bptr = C.byref(b)
+            check(self.lib.dsmc_create(C.byref(params), bptr, C.byref(self.h)))
         self.info = abi.dsmc_case_info()
         check(self.lib.dsmc_case_info_get(self.h, C.byref(self.info)))
         self.ncell = self.info.ncell
@@ -202,6 +231,11 @@ class Run:
                                                            C.c_double), int(z["nsamples"])))
 
     # -- sampling (GPUCells::Sample / DownloadToHost) --
+    def sample(self):
+        """One sample of the current state (GPUCells::Sample, which the
+        reference's Run::solver calls every 100 steps)."""
+        check(self.lib.dsmc_sample(self.h))
+
     def reset_samples(self):
         check(self.lib.dsmc_reset_samples(self.h))
 

@@ -116,6 +116,28 @@ int dsmc_device_cus(int *n);
  * Run.cpp:52-56,102-105,197-223).  body may be NULL (no immersed body). */
 int dsmc_create(const dsmc_params *p, const dsmc_body *body, dsmc_handle *out);
 int dsmc_destroy(dsmc_handle h);
+/* Host-built cut-cell tables (the reference's GPUCells::UploadFromHost ifCut
+ * and GPUSegment tables, src/cell/Cell.cuh:12, as Run::TransferCellsFromHostToDevice
+ * builds them from its mesh, src/Run.cpp:197-223).  The case takes the body
+ * from the wall segments instead of a polygon:
+ *   seg[6*nseg]  ax ay bx by nx ny per wall segment (the reference GPUSegment's
+ *                end points and normal, GPUSegment.cuh:11-19); the segments form
+ *                one closed counter-clockwise outline in order -- segment k ends
+ *                where k+1 starts, the last where the first starts -- and n is
+ *                the unit normal (ey, -ex)/|e| into the fluid;
+ *   frac[ncell]  fluid fraction of every cell (1 uncut, 0 solid; the fluid
+ *                volume is frac*dx*dy*L3), or NULL to clip the outline here;
+ *   cut[ncell]   the ifCut flags (1 exactly where 0 < frac < 1), or NULL.
+ * nseg = 0 is a case without a body.  The per-cell reach lists of the move
+ * and the fill helpers of cut cells are derived from the outline.  Invalid
+ * tables return DSMC_ERR_GEOMETRY before any device is touched. */
+typedef struct dsmc_cut_tables {
+    int nseg;
+    const double *seg;
+    const double *frac;
+    const uint8_t *cut;
+} dsmc_cut_tables;
+int dsmc_create_from_tables(const dsmc_params *p, const dsmc_cut_tables *t, dsmc_handle *out);
 int dsmc_case_info_get(dsmc_handle h, dsmc_case_info *out);
 /* Host-built cut-cell tables: seg[6*nseg] (ax ay bx by nx ny, n into the
  * fluid), cseg_start[ncell+1], cseg_idx[cseg_len], vol[ncell] fluid volume,
@@ -193,6 +215,15 @@ int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples);
 /* Sampling (GPUCells::Sample + DownloadToHost, GPUCell.cu:83-91,147-167).
  * acc[6*ncell]: six SoA blocks sum n, sum vx, sum vy, sum vz, sum v^2,
  * collisions.  Fields: number density n, velocity, temperature, pressure. */
+/* One sample of the current state into the accumulators (GPUCells::Sample,
+ * src/cell/Cell.cuh:22, GPUCell.cu:147-167,336-402, which Run::solver calls
+ * every 100 steps, src/Run.cpp:294-296): per cell, the particle count, the
+ * velocity and v^2 sums and the last step's collisions, added as one sampled
+ * step -- the same sums, in the same order, as the sampling a step does by
+ * itself from params.sample_start on.  The state is in cell order after a
+ * completed step; after an upload or a fill (no step yet) this returns
+ * DSMC_ERR_ARG.  Asynchronous on the handle's stream. */
+int dsmc_sample(dsmc_handle h);
 int dsmc_reset_samples(dsmc_handle h);
 int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples);
 int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,

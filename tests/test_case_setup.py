@@ -219,3 +219,46 @@ def test_sweep_wedges_stay_inside_the_domain():
         describe(c["params"], b)
     for c in cases.sweep(64):
         describe(c["params"], c["body"])
+
+
+# ---------------------------------------------------------------- host-built tables
+def _host_tables_rc(p, seg, frac=None, cut=None):
+    import ctypes as C
+    from cudadsmc_amd.abi import ptr
+    lib = abi.load()
+    seg = np.ascontiguousarray(seg, np.float64)
+    t = abi.dsmc_cut_tables(seg.size // 6, ptr(seg, C.c_double), None, None)
+    keep = [seg]
+    if frac is not None:
+        frac = np.ascontiguousarray(frac, np.float64); keep.append(frac); t.frac = ptr(frac, C.c_double)
+    if cut is not None:
+        cut = np.ascontiguousarray(cut, np.uint8); keep.append(cut); t.cut = ptr(cut, C.c_uint8)
+    h = C.c_void_p()
+    rc = lib.dsmc_create_from_tables(C.byref(p), C.byref(t), C.byref(h))
+    if rc == abi.DSMC_OK:
+        lib.dsmc_destroy(h)
+    return rc, lib.dsmc_last_error()
+
+
+def test_host_tables_are_checked_before_any_device():
+    """dsmc_create_from_tables rejects an outline that does not close, a
+    normal that points into the body, fractions outside [0, 1] and cut
+    flags that disagree with the fractions -- on the host, with
+    DSMC_ERR_GEOMETRY; valid tables get as far as the device check."""
+    from cudadsmc_amd.run import outline_segments
+    cfg = cases.cylinder_mach10(nx=40, ny=20, n_points=32)
+    p, body = cfg["params"], cfg["body"]
+    seg = outline_segments(body).reshape(-1, 6)
+    info, t = describe(p, body)
+    ok_rc, _ = _host_tables_rc(p, seg, t["frac"], t["cut"])
+    assert ok_rc in (abi.DSMC_OK, abi.DSMC_ERR_NODEVICE)
+    bad = seg.copy(); bad[3, 2] += 1e-3
+    assert _host_tables_rc(p, bad) == (abi.DSMC_ERR_GEOMETRY, b"host tables: segment 3 does not end where the next one starts")
+    bad = seg.copy(); bad[:, 4:] *= -1
+    rc, msg = _host_tables_rc(p, bad)
+    assert rc == abi.DSMC_ERR_GEOMETRY and b"unit normal" in msg
+    f = t["frac"].copy(); f[5] = 1.5
+    assert _host_tables_rc(p, seg, f)[0] == abi.DSMC_ERR_GEOMETRY
+    c = t["cut"].copy(); c[np.argmax(c)] = 0
+    rc, msg = _host_tables_rc(p, seg, t["frac"], c)
+    assert rc == abi.DSMC_ERR_GEOMETRY and b"cut flags" in msg

@@ -67,3 +67,35 @@ def test_no_device_is_a_clean_error(exe, tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
     assert "no HIP device" in r.stderr
+
+
+@pytest.mark.parametrize("case", [["--case", "cylinder", "--nx", "120", "--ny", "60"],
+                                  ["--case", "wedge", "--half-angle", "28", "--nx", "96", "--ny", "48"]])
+def test_host_mesh_tables_equal_the_library_tables(exe, tmp_path, case):
+    """The C++ host mesh (CartesianMesh::cutcell, the reference's
+    meshes/CartesianMesh.cpp:100-169 role) builds the same fluid fractions,
+    cut flags and wall segments as the library's own case set-up, bit for
+    bit -- what dsmc_create_from_tables then receives (no device needed);
+    getIndex follows the j + i*N2 convention."""
+    import json
+    from cudadsmc_amd import cases
+    from cudadsmc_amd.run import describe, outline_segments
+    f = tmp_path / "mesh.bin"
+    r = subprocess.run([exe, "--write-mesh", str(f)] + case + ["--out", str(tmp_path)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    raw = f.read_bytes()
+    N1, N2, nseg = np.frombuffer(raw[:12], np.int32)
+    C = int(N1) * int(N2)
+    frac = np.frombuffer(raw[12:12 + 8 * C], np.float64)
+    cut = np.frombuffer(raw[12 + 8 * C:12 + 9 * C], np.uint8)
+    seg = np.frombuffer(raw[12 + 9 * C:], np.float64)
+    geom = case[1]
+    half = float(case[3]) if geom == "wedge" else 20.0
+    c = cases.flow_case(geom, nx=int(N1), ny=int(N2), half_angle_deg=half)
+    info, t = describe(c["params"], c["body"])
+    np.testing.assert_array_equal(frac, t["frac"])
+    np.testing.assert_array_equal(cut, t["cut"])
+    assert nseg == c["body"].shape[0]
+    np.testing.assert_array_equal(seg, outline_segments(c["body"]))
+    np.testing.assert_array_equal(seg.astype(np.float32), t["seg"])

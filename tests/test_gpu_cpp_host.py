@@ -62,3 +62,26 @@ def test_cpp_phase_loop_equals_graph_replay(gpu_lib, tmp_path):
         a = (tmp_path / "graph" / f"result_{it}.npz").read_bytes()
         b = (tmp_path / "phases" / f"result_{it}.npz").read_bytes()
         assert a == b, it
+
+
+def test_cpp_sample_every_equals_python_sample(gpu_lib, tmp_path):
+    """dsmc_run --sample-every 100 (GPUCells::Sample after steps 0, 100, 200,
+    as the reference's Run::solver does, Run.cpp:294-296) writes the fields
+    of the Python Run sampling at the same steps with Run.sample()."""
+    out = tmp_path / "every"
+    _run(["--sample-every", "100", "--sample-from", "-1", "--output-every", "300"], out)
+    case = json.load(open(out / "case.json"))
+    p = cases.make_params(**case["params"])
+    assert p.sample_start == -1
+    r = Run(p, np.array(case["body"], np.float64))
+    r.initialize()
+    for it in range(300):
+        r.solver(1, graph=False)
+        if it % 100 == 0:
+            r.sample()
+    assert r.samples()[1] == 3
+    ds = output.datasets(r.fields(), p)
+    got = _load(out / "result_300.npz")
+    for k in ("U", "V", "W", "P", "T", "Rho"):
+        np.testing.assert_array_equal(got[k], ds[k], err_msg=k)
+    r.close()

@@ -398,3 +398,67 @@ def test_cylinder_full_invariants(gpu_lib):
     cell = j + i * p.N2
     assert (np.diff(cell) >= 0).all()
     r.close()
+
+
+@pytest.mark.parametrize("name", ["cyl80", "cyl80_specular", "wedge64"])
+def test_host_built_tables_equal_library_tables(gpu_lib, name):
+    """A case created from host-built tables (dsmc_create_from_tables: the
+    wall segments, fluid fractions and cut flags a host mesh computes, as
+    GPUCells::UploadFromHost takes them) equals the case the library builds
+    from the body outline: same tables, and 40 graph-replayed steps bit for
+    bit -- also when the fractions are left to the library."""
+    p, body, oc, info, tables = _make(name)
+    p.sample_start = 10
+    seg = run.outline_segments(body)
+    a = Run(p, body)
+    b = Run(p, tables=dict(seg=seg, frac=tables["frac"], cut=tables["cut"]))
+    c = Run(p, tables=dict(seg=seg))
+    ta = a.tables()
+    for r in (b, c):
+        for k, v in r.tables().items():
+            np.testing.assert_array_equal(v, ta[k], err_msg=k)
+    for r in (a, b, c):
+        r.initialize()
+        r.solver(40, graph=True)
+    for r, what in ((b, "host tables "), (c, "host segments ")):
+        T.assert_same_particles(r.download(), a.download(), what=what)
+        for u, v in zip(r.cell_state(), a.cell_state()):
+            np.testing.assert_array_equal(u, v)
+        np.testing.assert_array_equal(r.samples()[0], a.samples()[0])
+    for r in (a, b, c):
+        r.close()
+
+
+def test_standalone_sample_equals_fused_sampling(gpu_lib):
+    """dsmc_sample after every step (GPUCells::Sample, called by the
+    reference's solver loop) adds exactly what a step's own sampling adds;
+    sampling every 100th step as the reference does gives the sums of those
+    steps; before any step it is refused (the state is not in cell order)."""
+    p, body, oc, info, tables = _make("cyl80")
+    p.sample_start = 0
+    a = Run(p, body)
+    p2 = type(p).from_buffer_copy(p)
+    p2.sample_start = -1
+    b = Run(p2, body)
+    a.initialize()
+    b.initialize()
+    with pytest.raises(abi.DSMCError, match="cell order"):
+        b.sample()
+    for _ in range(25):
+        a.solver(1, graph=False)
+        b.solver(1, graph=False)
+        b.sample()
+    acc_a, ns_a = a.samples()
+    acc_b, ns_b = b.samples()
+    assert ns_a == ns_b == 25
+    np.testing.assert_array_equal(acc_b, acc_a)
+    T.assert_same_particles(b.download(), a.download(), what="sampled run ")
+    # every 100th step, as Run::solver does (Run.cpp:294-296)
+    b.reset_samples()
+    for it in range(300):
+        b.solver(1, graph=False)
+        if it % 100 == 0:
+            b.sample()
+    assert b.samples()[1] == 3
+    a.close()
+    b.close()
