@@ -196,16 +196,19 @@ def time_to_steady(args):
 
 
 # ------------------------------------------------------------------ workloads
-def timed(sim, steps, chunk):
-    """K eager steps with per-kernel HIP events (the library's timing mode);
-    returns wall seconds and the kernel summary."""
-    sim.set_timing(True)
+def timed(sim, steps, chunk, events=True):
+    """K eager steps with per-kernel HIP events (the library's timing mode;
+    events=False: algorithmic bytes only); returns wall seconds and the
+    kernel summary."""
+    sim.set_timing(True, events)
     sim.sync()
     t0 = time.perf_counter()
     done = 0
     while done < steps:
         k = min(chunk, steps - done)
         sim.solver(k, graph=False)
+        if not events:
+            sim.sync()     # counter passes: progress lines count finished steps
         done += k
         log(f"[bench] {done}/{steps} timed steps")
     sim.sync()
@@ -249,14 +252,19 @@ def bench_sweep(args):
         b.initialize()
     dev_graph = not args.profile_run
     done = 0
+    # a profiled run syncs every chunk, so its progress lines count finished
+    # steps (the profiler slows the dispatches far behind the host)
+    chunk = 50 if args.profile_run else 250
     while done < develop:
-        k = min(250, develop - done)
+        k = min(chunk, develop - done)
         b.solver(k, graph=dev_graph)
+        if args.profile_run:
+            b.sync()
         done += k
         log(f"[bench] {done}/{develop} development steps")
     b.solver(args.warmup, graph=dev_graph)
     s0 = b.status(-1)
-    dt, kt = timed(b, steps, 50)
+    dt, kt = timed(b, steps, 5 if args.no_events else 50, events=not args.no_events)
     s1 = b.status(-1)
     if s1.err:
         log(f"warning: device error bits {s1.err:#x}")
@@ -278,7 +286,8 @@ def bench_sweep(args):
             "particles_initial": n_init, "particles_timed_start": s0.n_live, "particles_final": s1.n_live,
             "develop_steps": develop, "timed_steps": [first_timed, first_timed + steps],
             "sampling_in_timed_steps": sampling, "collisions": s1.collisions - s0.collisions,
-            "launch": "eager with per-kernel HIP events", "launch_settings": b.get_launch(),
+            "launch": "eager" if args.no_events else "eager with per-kernel HIP events",
+            "launch_settings": b.get_launch(),
             "autotuned": tuned is not None, "seed": 20251203,
             "development_launch": "hipgraph" if dev_graph else "eager",
         },
@@ -354,11 +363,17 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3: development and warm-up launched eagerly, no graph phase, no CPU "
                          "baseline, no time to steady -- the timed steps are then the last dispatches")
+    ap.add_argument("--no-events", action="store_true",
+                    help="with --profile-run: no HIP events around the timed kernels, algorithmic bytes only "
+                         "(for counter passes, where the profiler times the kernels)")
     ap.add_argument("--save-state", default=None, help="cylinder: checkpoint of the state the timed steps start from")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X "
               "(sharding across GPUs is designed, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.no_events and not args.profile_run:
+        print("--no-events is for counter passes and needs --profile-run", file=sys.stderr)
         sys.exit(2)
     if args.workload == "sweep64":
         bench_sweep(args)

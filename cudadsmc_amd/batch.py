@@ -174,8 +174,10 @@ class Batch:
         check(self.lib.dsmc_batch_set_launch(self.h, C.byref(L)))
 
     # -- per-kernel timing --
-    def set_timing(self, on=True):
-        check(self.lib.dsmc_batch_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on=True, events=True):
+        """Per-kernel durations (HIP events) and algorithmic bytes of the
+        eager steps that follow; events=False keeps only the bytes."""
+        check(self.lib.dsmc_batch_set_timing(self.h, 0 if not on else (1 if events else 2)))
 
     def kernel_times(self):
         ms, by = np.zeros(4), np.zeros(4)

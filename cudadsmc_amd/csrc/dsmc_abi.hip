@@ -540,7 +540,9 @@ namespace {
 // Eager steps with HIP events between the four launches of every step (the
 // batched handle's timing mode, dsmc_batch.hip): a ring of kTimeRing steps'
 // events in flight, kernel durations added up as slots are reused, and
-// algorithmic bytes from the counters before and after the steps.
+// algorithmic bytes from the counters before and after the steps.  Mode 2
+// records no events and keeps only the bytes (for runs under a counter
+// profiler, which times the kernels itself).
 constexpr int kTimeRing = 64;
 
 int harvest(dsmc_handle h, int slot)
@@ -557,7 +559,8 @@ int harvest(dsmc_handle h, int slot)
 
 int timed_steps(dsmc_handle h, int nsteps)
 {
-    if (h->tev.empty()) {
+    const bool events = h->timing == 1;
+    if (events && h->tev.empty()) {
         h->tev.resize(5 * (size_t)kTimeRing, nullptr);
         for (auto &e : h->tev) HIPCHK(hipEventCreate(&e));
     }
@@ -565,7 +568,9 @@ int timed_steps(dsmc_handle h, int nsteps)
     Ctrl before, after;
     if ((r = read_ctrl(h, before))) return r;
     hipStream_t st = h->stream;
-    for (int k = 0; k < nsteps; ++k) {
+    for (int k = 0; k < nsteps && !events; ++k)
+        if ((r = launch_step(h, 7))) return r;
+    for (int k = 0; k < nsteps && events; ++k) {
         const int slot = k % kTimeRing;
         if (k >= kTimeRing && (r = harvest(h, slot))) return r;
         hipEvent_t *ev = &h->tev[5 * (size_t)slot];
@@ -580,7 +585,7 @@ int timed_steps(dsmc_handle h, int nsteps)
         if ((r = launch_step(h, 4))) return r;
         HIPCHK(hipEventRecord(ev[4], st));
     }
-    for (int k = std::max(0, nsteps - kTimeRing); k < nsteps; ++k)
+    for (int k = std::max(0, nsteps - kTimeRing); k < nsteps && events; ++k)
         if ((r = harvest(h, k % kTimeRing))) return r;
     if ((r = read_ctrl(h, after))) return r;
     step_bytes(h, before, after, nsteps);
@@ -839,7 +844,8 @@ int dsmc_set_launch(dsmc_handle h, const dsmc_launch *c)
 int dsmc_set_timing(dsmc_handle h, int on)
 {
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
-    h->timing = on ? 1 : 0;
+    if (on < 0 || on > 2) return fail(DSMC_ERR_ARG, "set_timing: 0 off, 1 events and bytes, 2 bytes only");
+    h->timing = on;
     for (int q = 0; q < 4; ++q) h->t_ms[q] = h->t_bytes[q] = 0.0;
     h->t_launches = 0;
     return DSMC_OK;

@@ -435,7 +435,9 @@ namespace {
 // throttled without being drained.  Algorithmic bytes follow from the
 // device counters read once before and once after the steps (sums over the
 // steps of live, moved and new particle counts, candidates, collisions and
-// sampled steps), so no step waits for the host.
+// sampled steps), so no step waits for the host.  Mode 2 records no events
+// and keeps only the bytes (for runs under a counter profiler, which times
+// the kernels itself).
 constexpr int kTimeRing = 64;
 
 int harvest(dsmc_batch b, int slot)
@@ -452,7 +454,8 @@ int harvest(dsmc_batch b, int slot)
 
 int timed_steps(dsmc_batch b, int nsteps)
 {
-    if (b->tev.empty()) {
+    const bool events = b->timing == 1;
+    if (events && b->tev.empty()) {
         b->tev.resize(5 * (size_t)kTimeRing, nullptr);
         for (auto &e : b->tev) HIPCHK(hipEventCreate(&e));
     }
@@ -461,7 +464,9 @@ int timed_steps(dsmc_batch b, int nsteps)
     std::vector<InstCtrl> ib, ia;
     if ((r = read_ctrl(b, before, &ib))) return r;
     hipStream_t st = b->stream;
-    for (int k = 0; k < nsteps; ++k) {
+    for (int k = 0; k < nsteps && !events; ++k)
+        if ((r = launch_step(b, 15))) return r;
+    for (int k = 0; k < nsteps && events; ++k) {
         const int slot = k % kTimeRing;
         if (k >= kTimeRing && (r = harvest(b, slot))) return r;
         hipEvent_t *ev = &b->tev[5 * (size_t)slot];
@@ -471,7 +476,7 @@ int timed_steps(dsmc_batch b, int nsteps)
             HIPCHK(hipEventRecord(ev[q + 1], st));
         }
     }
-    for (int k = std::max(0, nsteps - kTimeRing); k < nsteps; ++k)
+    for (int k = std::max(0, nsteps - kTimeRing); k < nsteps && events; ++k)
         if ((r = harvest(b, k % kTimeRing))) return r;
     if ((r = read_ctrl(b, after, &ia))) return r;
     // sums over the steps: live at the start (the step's particle-steps),
@@ -827,7 +832,8 @@ int dsmc_batch_set_launch(dsmc_batch b, const dsmc_launch *c)
 int dsmc_batch_set_timing(dsmc_batch b, int on)
 {
     if (!b) return fail(DSMC_ERR_ARG, "null batch");
-    b->timing = on ? 1 : 0;
+    if (on < 0 || on > 2) return fail(DSMC_ERR_ARG, "set_timing: 0 off, 1 events and bytes, 2 bytes only");
+    b->timing = on;
     for (int q = 0; q < 4; ++q) b->t_ms[q] = b->t_bytes[q] = 0.0;
     b->t_launches = 0;
     return DSMC_OK;

@@ -8,6 +8,8 @@ traffic figure derived from them.
   python -m cudadsmc_amd.profiling traffic KERNEL FETCH_DIR WRITE_DIR \
         COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB LAST OUT          # traffic per launch
   python -m cudadsmc_amd.profiling window STATE [STEPS]        # resumed timing window
+  python -m cudadsmc_amd.profiling bench-traffic KERNEL BENCH_JSON FETCH_DIR WRITE_DIR \
+        COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB OUT               # traffic of a profiled bench run
 
 rocprofv3 is run with --output-format csv; DIR is its -d directory.  LAST
 restricts means to the last LAST dispatches of each kernel (the bench's
@@ -117,6 +119,41 @@ def traffic(kernel, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_m
                 traffic_bytes=(f * f_scale + w * w_scale) * 1024.0, window=last)
 
 
+def bench_traffic(kernel, bench_json, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib, out):
+    """The traffic file bench.py pairs with its roofline: HBM bytes per
+    launch of `kernel` over the timed steps of a `bench.py --profile-run`
+    (its last `steps` dispatches), with the state that run timed --
+    workload, development steps, sampling, particles at the end -- and the
+    algorithmic bytes it reported for the same launches."""
+    import json
+    b = None
+    for line in open(bench_json):
+        line = line.strip()
+        if line.startswith("{"):
+            b = json.loads(line)
+    if b is None:
+        raise ValueError(f"no bench line in {bench_json}")
+    cfg = b["config"]
+    steps = int(b["steps"])
+    t = traffic(kernel, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib, steps)
+    alg = b["roofline"]["per_kernel"][kernel]["alg_bytes"]
+    lines = [
+        f"# HBM traffic per launch of {kernel}, workload {cfg['workload']}, over the {steps} timed steps "
+        f"{cfg['timed_steps']} of a bench.py --profile-run (the last {steps} dispatches of {kernel}).",
+        f"# FETCH_SIZE and WRITE_SIZE (KiB per dispatch) each scaled by known bytes / reading of a {copy_mib:g} MiB "
+        f"device copy.",
+        f"# traffic / algorithmic = {t['traffic_bytes'] / alg:.3f}",
+        f"kernel = {kernel}", f"workload = {cfg['workload']}", f"develop = {cfg['develop_steps']}",
+        f"sampling = {'on' if cfg.get('sampling_in_timed_steps', True) else 'off'}",
+        f"particles = {cfg['particles_final']}", f"alg_bytes = {alg}",
+        f"fetch_kib = {t['fetch_kib']}", f"write_kib = {t['write_kib']}",
+        f"fetch_scale = {t['fetch_scale']}", f"write_scale = {t['write_scale']}",
+        f"traffic_bytes = {t['traffic_bytes']}", f"window = {steps}",
+    ]
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def window(state, steps=50):
     """Resumes the Mach-10 cylinder bench case from a checkpoint written by
     `bench.py --save-state` and runs its eager timing window: the same
@@ -154,6 +191,9 @@ def main(argv):
         lines = [f"{k} = {v}" for k, v in t.items()]
         open(out, "w").write("\n".join(lines) + "\n")
         print("\n".join(lines))
+    elif cmd == "bench-traffic":
+        kernel, bj, fd, wd, cfd, cwd, mib, out = argv[2:10]
+        bench_traffic(kernel, bj, fd, wd, cfd, cwd, float(mib), out)
     elif cmd == "window":
         window(argv[2], int(argv[3]) if len(argv) > 3 else 50)
     else:

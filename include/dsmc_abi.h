@@ -284,7 +284,8 @@ int dsmc_set_launch(dsmc_handle h, const dsmc_launch *cfg);
 /* Per-kernel timing with HIP events on the handle's stream (eager steps
  * only).  ms[k] = mean duration of kernel k over timed steps, k = 0 move,
  * 1 scan, 2 scatter, 3 cell; bytes[k] = mean algorithmic bytes per launch of
- * kernel k over the same steps. */
+ * kernel k over the same steps.  on = 1 events and bytes, 2 bytes only (no
+ * events: ms stays 0), 0 off; any other value is DSMC_ERR_ARG. */
 int dsmc_set_timing(dsmc_handle h, int on);
 int dsmc_kernel_times(dsmc_handle h, double ms[4], double bytes[4], int64_t *launches);
 
