@@ -28,6 +28,7 @@ class dsmc_params(C.Structure):
         ("face_bc", C.c_int * 4), ("face_T", C.c_double * 4),
         ("tau", C.c_double), ("ppc", C.c_double), ("Fn", C.c_double),
         ("seed", C.c_uint64), ("capacity", C.c_int64), ("sample_start", C.c_int),
+        ("zeta", C.c_int), ("rot_prob", C.c_double),
     ]
 
 
@@ -52,6 +53,9 @@ class dsmc_case_info(C.Structure):
         ("cell_volume", C.c_double),
         ("ppc_eff", C.c_double),
         ("capacity", C.c_int64),
+        ("zeta", C.c_int), ("sample_blocks", C.c_int),
+        ("rot_p", C.c_float), ("lb_ia", C.c_float), ("lb_ib", C.c_float),
+        ("body_ekt", C.c_float), ("face_ekt", C.c_float * 4), ("inj_ekt", C.c_float),
     ]
 
 
@@ -127,6 +131,9 @@ SIGNATURES = {
     "dsmc_reset_samples": [_h],
     "dsmc_download_samples": [_h, _f64p, _P(C.c_uint64)],
     "dsmc_download_fields": [_h, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p],
+    "dsmc_download_rot_temperature": [_h, _f64p],
+    "dsmc_upload_rotational": [_h, C.c_int64, _f32p],
+    "dsmc_download_rotational": [_h, C.c_int, C.c_int64, _f32p, _P(C.c_int64)],
     "dsmc_batch_create": [_P(dsmc_params), _P(dsmc_body), C.c_int, _P(C.c_void_p)],
     "dsmc_batch_destroy": [_h],
     "dsmc_batch_case_info": [_h, C.c_int, _P(dsmc_case_info)],

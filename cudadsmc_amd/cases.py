@@ -12,6 +12,10 @@ BOLTZ = 1.380649e-23          # J/K
 
 # argon, VHS (Bird 1994, Appendix A)
 ARGON = dict(mass=6.63e-26, diam=4.17e-10, Vtl=0.81, Tref=273.0)
+# nitrogen, VHS (Bird 1994, Appendix A), two rotational degrees of freedom:
+# the reference's default gas (src/Param.h:7,14: the N2 mass 4.65116e-26 kg,
+# zeta = 2; it keeps argon's omega 0.81 there)
+NITROGEN = dict(mass=4.65e-26, diam=4.17e-10, Vtl=0.74, Tref=273.0, zeta=2)
 
 
 def vhs_collision_frequency(n, T, gas=ARGON, boltz=BOLTZ):
@@ -38,6 +42,21 @@ def wedge_polygon(apex_x, cy, length, half_angle_deg):
     return np.array([[apex_x, cy], [apex_x + length, cy - h], [apex_x + length, cy + h]], dtype=np.float64)
 
 
+def gas_fields(gas, rot_prob=1.0):
+    """make_params keywords of a gas: VHS constants and rotational degrees
+    of freedom (zeta = 2 with rot_prob = 1/Z_rot for a diatomic gas)."""
+    d = {k: gas[k] for k in ("mass", "diam", "Vtl", "Tref")}
+    d["zeta"] = gas.get("zeta", 0)
+    d["rot_prob"] = rot_prob
+    return d
+
+
+def sound_speed(T, gas=ARGON, boltz=BOLTZ):
+    """sqrt(gamma k T / m) with gamma = (5 + zeta) / (3 + zeta)."""
+    z = gas.get("zeta", 0)
+    return math.sqrt((5.0 + z) / (3.0 + z) * boltz * T / gas["mass"])
+
+
 def make_params(**kw):
     """dsmc_params with defaults; keyword names are the struct's fields."""
     p = abi.dsmc_params()
@@ -55,6 +74,8 @@ def make_params(**kw):
     p.seed = 20251203
     p.capacity = 0
     p.sample_start = -1
+    p.zeta = 0
+    p.rot_prob = 1.0
     for k, v in kw.items():
         if k in ("face_bc", "face_T"):
             arr = getattr(p, k)
@@ -78,41 +99,44 @@ def box_equilibrium(ppc=20, n_cells=64, steps=1000, seed=20251203):
 
 
 def cylinder_mach10(ppc=30, nx=400, ny=200, n_points=256, steps=20000, sample_from=10000,
-                    seed=20251203):
+                    seed=20251203, gas=ARGON, rot_prob=1.0):
     """Plain DSMC, 2D supersonic argon flow over a cut-cell cylinder: Mach 10,
     T_inf = 200 K, Kn = 0.01 on the diameter (hard-sphere mean free path),
     R = 0.1 m centred in a 0.6 m x 0.3 m domain, 400x200 cells, diffuse wall
     at 500 K, freestream inflow/outflow on all faces, 30 particles per fluid
-    cell, dt = 0.3 dx / (U + 3 vmp), 20k steps, sampling from step 10k."""
+    cell, dt = 0.3 dx / (U + 3 vmp), 20k steps, sampling from step 10k.  gas:
+    another gas for the same geometry (NITROGEN: Mach 10 with gamma = 7/5)."""
     T = 200.0
-    m = ARGON["mass"]
-    a = math.sqrt(5.0 / 3.0 * BOLTZ * T / m)
+    m = gas["mass"]
+    a = sound_speed(T, gas)
     U = 10.0 * a
     R, D = 0.1, 0.2
     lam = 0.01 * D
-    n = hard_sphere_density_for_mfp(lam)
+    n = hard_sphere_density_for_mfp(lam, gas)
     L1, L2 = 0.6, 0.3
     dx = L1 / nx
     vmp = math.sqrt(2.0 * BOLTZ * T / m)
     dt = 0.3 * dx / (U + 3.0 * vmp)
     p = make_params(L1=L1, L2=L2, N1=nx, N2=ny, n_inf=n, T=T, V_jet=U, T_wall=500.0,
                     wall_bc=abi.BC_DIFFUSE, face_bc=[abi.BC_STREAM] * 4, tau=dt,
-                    ppc=float(ppc), seed=seed, sample_start=sample_from)
+                    ppc=float(ppc), seed=seed, sample_start=sample_from, **gas_fields(gas, rot_prob))
     body = circle_polygon(n_points, 0.5 * L1, 0.5 * L2, R)
     return dict(name=f"cylinder_mach10_{nx}x{ny}", params=p, body=body, steps=steps, U=U,
                 sample_from=sample_from)
 
 
 def flow_case(geometry="cylinder", mach=10.0, kn=0.01, tw_ratio=2.5, nx=400, ny=200, ppc=30.0,
-              half_angle_deg=20.0, T_inf=200.0, steps=20000, sample_from=10000, seed=20251203, n_points=256):
+              half_angle_deg=20.0, T_inf=200.0, steps=20000, sample_from=10000, seed=20251203, n_points=256,
+              gas=ARGON, rot_prob=1.0):
     """Supersonic argon flow over a cylinder (R = 0.1 m, centred) or a wedge
     (apex at x = 0.15 m, length 0.3 m or less so that its base stays 1 cm
     inside the domain) in a 0.6 m x 0.3 m domain:
     freestream inflow/outflow on all faces, diffuse wall at tw_ratio*T_inf,
     Kn on the body size (cylinder diameter, wedge length) with the
-    hard-sphere mean free path, dt = 0.3 dx / (U + 3 vmp)."""
-    m = ARGON["mass"]
-    a = math.sqrt(5.0 / 3.0 * BOLTZ * T_inf / m)
+    hard-sphere mean free path, dt = 0.3 dx / (U + 3 vmp); gas: argon unless
+    given (Mach on the gas's own speed of sound)."""
+    m = gas["mass"]
+    a = sound_speed(T_inf, gas)
     U = mach * a
     L1, L2 = 0.6, 0.3
     if geometry == "cylinder":
@@ -126,13 +150,13 @@ def flow_case(geometry="cylinder", mach=10.0, kn=0.01, tw_ratio=2.5, nx=400, ny=
         body = wedge_polygon(0.15, 0.5 * L2, size, half_angle_deg)
     else:
         raise ValueError(f"unknown geometry {geometry!r}")
-    n = hard_sphere_density_for_mfp(kn * size)
+    n = hard_sphere_density_for_mfp(kn * size, gas)
     dx = L1 / nx
     vmp = math.sqrt(2.0 * BOLTZ * T_inf / m)
     dt = 0.3 * dx / (U + 3.0 * vmp)
     p = make_params(L1=L1, L2=L2, N1=nx, N2=ny, n_inf=n, T=T_inf, V_jet=U, T_wall=tw_ratio * T_inf,
                     wall_bc=abi.BC_DIFFUSE, face_bc=[abi.BC_STREAM] * 4, tau=dt,
-                    ppc=float(ppc), seed=seed, sample_start=sample_from)
+                    ppc=float(ppc), seed=seed, sample_start=sample_from, **gas_fields(gas, rot_prob))
     return dict(name=f"{geometry}_M{mach:.2f}_Kn{kn:.4f}_Tw{tw_ratio:.2f}_{nx}x{ny}", params=p, body=body,
                 steps=steps, U=U, sample_from=sample_from,
                 point=dict(geometry=geometry, mach=mach, kn=kn, tw_ratio=tw_ratio,

@@ -77,6 +77,7 @@ void Output::write_npz(const std::string &path, const FieldSet &f)
     m.push_back({ "Numchild.npy", npy(f.Numchild, f.N1, f.N2, "<i4") });
     m.push_back({ "Numgrandchild.npy", npy(f.Numgrandchild, f.N1, f.N2, "<i4") });
     m.push_back({ "MFP.npy", npy(f.MFP, f.N1, f.N2, "<f8") });
+    if (!f.Trot.empty()) m.push_back({ "Trot.npy", npy(f.Trot, f.N1, f.N2, "<f8") });
 
     // zip archive, every member stored (no compression)
     std::string out, cd;
@@ -145,6 +146,7 @@ void Output::write_vts(const std::string &path, const FieldSet &f)
     array("Numchild", [&](size_t c) { return std::to_string(f.Numchild[c]); }, true);
     array("Numgrandchild", [&](size_t c) { return std::to_string(f.Numgrandchild[c]); }, true);
     array("MFP", [&](size_t c) { return fmt9(f.MFP[c]); }, false);
+    if (!f.Trot.empty()) array("Trot", [&](size_t c) { return fmt9(f.Trot[c]); }, false);
     s += "</CellData>\n</Piece>\n</StructuredGrid>\n</VTKFile>\n";
     FILE *fp = std::fopen(path.c_str(), "w");
     if (!fp) throw std::runtime_error("cannot write " + path);

@@ -22,6 +22,7 @@ struct FieldSet {
     double L1 = 0, L2 = 0;
     std::vector<double> U, V, W, P, T, Rho, MFP;
     std::vector<int> Numchild, Numgrandchild;   // adaptive sub-cells: none here, all zero
+    std::vector<double> Trot;                   // rotational temperature (diatomic gas only, else empty)
 };
 
 class Output {

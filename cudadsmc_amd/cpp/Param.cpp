@@ -64,6 +64,8 @@ dsmc_params to_abi(const Param &p)
     d.seed = p.seed;
     d.capacity = p.capacity;
     d.sample_start = p.sample_start;
+    d.zeta = p.zeta;
+    d.rot_prob = p.rot_prob;
     return d;
 }
 

@@ -23,6 +23,8 @@ struct Param {
     double diam = 4.17e-10;        // m, VHS reference diameter
     double Vtl = 0.81;             // viscosity index omega
     double Tref = 273.0;           // K, VHS reference temperature
+    int zeta = 0;                  // rotational degrees of freedom (Param.h:14): 0, or 2 for N2
+    double rot_prob = 1.0;         // 1/Z_rot: fraction of collisions exchanging rotational energy
     double tau = 0.0;              // time step (s)
     // flow field (Param.h:16-22)
     double L1 = 0.6, L2 = 0.3, L3 = 1.0;

@@ -44,11 +44,16 @@ long arg_i(const char *s, const char *name)
 // x = 0.15 m, 0.3 m long or shorter so that its base stays 1 cm inside the
 // domain) in a 0.6 m x 0.3 m domain: the case family the sweeps draw from
 // (cudadsmc_amd/cases.py flow_case).
-Param flow_case(const std::string &geometry, double mach, double kn, double tw_ratio, int nx, int ny, double ppc,
-                double half_angle_deg)
+Param flow_case(const Param &gas, const std::string &geometry, double mach, double kn, double tw_ratio, int nx,
+                int ny, double ppc, double half_angle_deg)
 {
     Param p = cylinder_mach10(nx, ny, ppc);
-    const double a = std::sqrt(5.0 / 3.0 * p.boltz * p.T / p.mass);
+    p.mass = gas.mass; p.diam = gas.diam; p.Vtl = gas.Vtl; p.Tref = gas.Tref;
+    p.zeta = gas.zeta; p.rot_prob = gas.rot_prob;
+    // speed of sound with the gas's ratio of specific heats (5/3, or 7/5 with
+    // two rotational degrees of freedom)
+    const double gamma = (5.0 + p.zeta) / (3.0 + p.zeta);
+    const double a = std::sqrt(gamma * p.boltz * p.T / p.mass);
     p.V_jet = mach * a;
     p.T_wall = tw_ratio * p.T;
     p.geometry = geometry;
@@ -82,7 +87,8 @@ Param parse_args(int argc, char **argv)
     double mach = 10.0, kn = 0.01, tw = 2.5, half = 20.0, ppc = 30.0;
     int nx = 400, ny = 200;
     long steps = -1, sample_from = -2, every = 0, seed = -1, sample_every = 0;
-    std::string out = "./res";
+    std::string out = "./res", gas = "argon";
+    double zrot = 1.0;
     bool eager = false, phases = false;
     for (int k = 1; k < argc; ++k) {
         const std::string a = argv[k];
@@ -106,12 +112,25 @@ Param parse_args(int argc, char **argv)
         else if (a == "--out") out = val();
         else if (a == "--eager") eager = true;
         else if (a == "--phases") phases = true;
+        else if (a == "--gas") gas = val();
+        else if (a == "--zrot") zrot = arg_d(val(), "--zrot");
         else throw std::invalid_argument("unknown option " + a);
     }
     if (nx < 1 || ny < 1) throw std::invalid_argument("--nx and --ny must be positive");
+    if (gas != "argon" && gas != "nitrogen") throw std::invalid_argument("--gas must be argon or nitrogen");
+    if (!(zrot >= 1.0)) throw std::invalid_argument("--zrot must be >= 1");
     if (geometry != "cylinder" && geometry != "wedge" && geometry != "none")
         throw std::invalid_argument("--case must be cylinder, wedge or none");
-    Param p = flow_case(geometry == "none" ? "cylinder" : geometry, mach, kn, tw, nx, ny, ppc, half);
+    Param base;
+    if (gas == "nitrogen") {
+        // N2, VHS (Bird 1994, Appendix A), two rotational degrees of freedom:
+        // the reference's gas (Param.h:7,14)
+        base.mass = 4.65e-26;
+        base.Vtl = 0.74;
+        base.zeta = 2;
+        base.rot_prob = 1.0 / zrot;
+    }
+    Param p = flow_case(base, geometry == "none" ? "cylinder" : geometry, mach, kn, tw, nx, ny, ppc, half);
     p.geometry = geometry;
     if (steps >= 0) p.steps = (int)steps;
     if (sample_every < 0) throw std::invalid_argument("--sample-every must be >= 0");
@@ -211,6 +230,11 @@ void Run::TransferCellsFromDeviceToHost()
     f.MFP.assign(C, 0.0);
     f.Numchild.assign(C, 0);
     f.Numgrandchild.assign(C, 0);
+    f.Trot.clear();
+    if (m_param.zeta) {
+        f.Trot.resize(C);
+        check(dsmc_download_rot_temperature(m_h, f.Trot.data()), "dsmc_download_rot_temperature");
+    }
     // MFP as the reference's Cell.cpp:125, m / (sqrt(2) rho pi d^2) (T/T_inf)^(omega - 1/2)
     const double d2 = m_param.diam * m_param.diam;
     for (size_t c = 0; c < C; ++c) {
@@ -310,8 +334,9 @@ void Run::write_case(const std::string &path) const
                  d.wall_bc);
     std::fprintf(f, " \"face_bc\": [%d, %d, %d, %d], \"face_T\": [%.17g, %.17g, %.17g, %.17g],\n", d.face_bc[0],
                  d.face_bc[1], d.face_bc[2], d.face_bc[3], d.face_T[0], d.face_T[1], d.face_T[2], d.face_T[3]);
-    std::fprintf(f, " \"tau\": %.17g, \"ppc\": %.17g, \"Fn\": %.17g, \"seed\": %llu, \"capacity\": %lld, \"sample_start\": %d},\n",
+    std::fprintf(f, " \"tau\": %.17g, \"ppc\": %.17g, \"Fn\": %.17g, \"seed\": %llu, \"capacity\": %lld, \"sample_start\": %d,\n",
                  d.tau, d.ppc, d.Fn, (unsigned long long)d.seed, (long long)d.capacity, d.sample_start);
+    std::fprintf(f, " \"zeta\": %d, \"rot_prob\": %.17g},\n", d.zeta, d.rot_prob);
     std::fprintf(f, " \"geometry\": \"%s\", \"steps\": %d, \"output_every\": %d,\n \"body\": [", m_param.geometry.c_str(),
                  m_param.steps, m_param.output_every);
     for (size_t k = 0; k + 1 < m_body.size(); k += 2)

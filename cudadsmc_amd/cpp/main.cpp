@@ -75,7 +75,7 @@ int main(int argc, char **argv)
         std::printf("usage: dsmc_run [--case cylinder|wedge|none] [--mach M] [--kn KN] [--tw RATIO]\n"
                     "                [--half-angle DEG] [--nx N1] [--ny N2] [--ppc P] [--seed S] [--steps K]\n"
                     "                [--sample-from STEP] [--sample-every K] [--output-every K] [--out DIR]\n"
-                    "                [--eager] [--phases]\n");
+                    "                [--gas argon|nitrogen] [--zrot Z] [--eager] [--phases]\n");
         return 0;
     }
     if (argc >= 3 && !std::strcmp(argv[1], "--write-mesh")) {

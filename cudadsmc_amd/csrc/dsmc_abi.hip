@@ -124,7 +124,7 @@ int reset_cell_state(dsmc_handle h)
     std::vector<float> s0(h->ncell, (float)h->P.sigcmax0);
     HIPCHK(hipMemcpyAsync(h->sigcmax, s0.data(), sizeof(float) * h->ncell, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemsetAsync(h->mrem, 0, sizeof(float) * h->ncell, h->stream));
-    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * 6 * (size_t)h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * h->P.sample_blocks * (size_t)h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(h->count, 0, sizeof(int) * h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(h->ncoll_cell, 0, sizeof(int) * h->ncell, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -147,6 +147,11 @@ void make_info(const Param &P, const CutTables &T, const DevCase &cs, int64_t ca
     o->margin = cs.margin; o->body_bc = cs.body_bc; o->seed = cs.seed; o->cell_volume = P.Vcell;
     o->ppc_eff = P.ppc_eff;
     o->capacity = cap;
+    o->zeta = P.in.zeta;
+    o->sample_blocks = P.sample_blocks;
+    o->rot_p = cs.rot_p; o->lb_ia = cs.lb_ia; o->lb_ib = cs.lb_ib;
+    o->body_ekt = cs.body_ekt; o->inj_ekt = cs.inj_ekt;
+    for (int f = 0; f < 4; ++f) o->face_ekt[f] = cs.face_ekt[f];
 }
 
 void copy_tables(const CutTables &T, float *seg, int *cseg_start, int *cseg_idx, double *vol, float *vol_ntc,
@@ -253,7 +258,8 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
     SoA *bufs[2] = { &h->A, &h->B };
     for (SoA *s : bufs) {
         if ((e = h->alloc(&s->x, N)) || (e = h->alloc(&s->y, N)) || (e = h->alloc(&s->vx, N)) ||
-            (e = h->alloc(&s->vy, N)) || (e = h->alloc(&s->vz, N)) || (e = h->alloc(&s->pid, N)))
+            (e = h->alloc(&s->vy, N)) || (e = h->alloc(&s->vz, N)) || (e = h->alloc(&s->pid, N)) ||
+            (P.in.zeta != 0 && (e = h->alloc(&s->er, N))))
             return bad(e, "particle buffers");
     }
     const int ntiles_cell = (int)((N + CELL_P_MIN - 1) / CELL_P_MIN) + 1;
@@ -261,7 +267,7 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
         (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||
         (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, C + 1)) ||
         (e = h->alloc(&h->tile_first, 2 * (size_t)ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
-        (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, 6 * C)) ||
+        (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, (size_t)P.sample_blocks * C)) ||
         (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
         (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||
         (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) || (e = h->alloc(&h->d_fbox, 4 * C)) || (e = h->alloc(&h->d_fmask, C)) ||
@@ -291,7 +297,7 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
         return bad(e, "device attribute");
     h->grid_move = cu * occupancy_move();
     h->grid_scatter = cu * occupancy_scatter();
-    h->grid_cell = cu * occupancy_cell();
+    h->grid_cell = cu * occupancy_cell(P.in.zeta != 0);
     h->grid_fill = cu * 4;
     int r = reset_cell_state(h);
     if (r) { std::string m = g_err; dsmc_destroy(h); return fail(r, m); }
@@ -421,6 +427,8 @@ int dsmc_upload_particles(dsmc_handle h, int64_t n, const float *x, const float 
         HIPCHK(hipMemcpyAsync(h->A.vy, vy, b, hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->A.vz, vz, b, hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->A.pid, pid, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, h->stream));
+        // no rotational energy until dsmc_upload_rotational supplies it
+        if (h->A.er) HIPCHK(hipMemsetAsync(h->A.er, 0, b, h->stream));
     }
     HIPCHK(hipMemsetAsync(h->count, 0, sizeof(int) * h->ncell, h->stream));
     HIPCHK(launch_ctrl_reset(h->ctrl, (uint32_t)n, next, h->stream));
@@ -746,6 +754,7 @@ int dsmc_restore_counters(dsmc_handle h, const dsmc_status *s)
         HIPCHK(hipMemcpyAsync(to.vz, from.vz, b, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(to.pid, from.pid, sizeof(uint32_t) * (size_t)c.n_live, hipMemcpyDeviceToDevice,
                               h->stream));
+        if (from.er) HIPCHK(hipMemcpyAsync(to.er, from.er, b, hipMemcpyDeviceToDevice, h->stream));
     }
     c.step = s->step;
     c.next_pid = s->next_pid;
@@ -763,7 +772,8 @@ int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples)
     if (!h || !acc) return fail(DSMC_ERR_ARG, "null argument");
     int r = set_device(h);
     if (r) return r;
-    HIPCHK(hipMemcpyAsync(h->acc, acc, sizeof(double) * 6 * (size_t)h->ncell, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->acc, acc, sizeof(double) * h->P.sample_blocks * (size_t)h->ncell,
+                          hipMemcpyHostToDevice, h->stream));
     unsigned long long ns = nsamples;
     HIPCHK(hipMemcpyAsync(&h->ctrl->nsamples, &ns, sizeof(ns), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -786,7 +796,7 @@ int dsmc_reset_samples(dsmc_handle h)
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
     int r = set_device(h);
     if (r) return r;
-    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * 6 * (size_t)h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * h->P.sample_blocks * (size_t)h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(&h->ctrl->nsamples, 0, sizeof(unsigned long long), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return DSMC_OK;
@@ -800,7 +810,8 @@ int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples)
     Ctrl c;
     if ((r = read_ctrl(h, c))) return r;
     if (nsamples) *nsamples = c.nsamples;
-    if (acc) HIPCHK(hipMemcpy(acc, h->acc, sizeof(double) * 6 * (size_t)h->ncell, hipMemcpyDeviceToHost));
+    if (acc)
+        HIPCHK(hipMemcpy(acc, h->acc, sizeof(double) * h->P.sample_blocks * (size_t)h->ncell, hipMemcpyDeviceToHost));
     return DSMC_OK;
 }
 
@@ -808,11 +819,60 @@ int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy, doubl
 {
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
     const size_t C = h->ncell;
-    std::vector<double> a(6 * C);
+    std::vector<double> a(h->P.sample_blocks * C);
     uint64_t ns = 0;
     int r = dsmc_download_samples(h, a.data(), &ns);
     if (r) return r;
     fields_from_sums(h->P, h->T, a.data(), C, ns, n, ux, uy, uz, T, p);
+    return DSMC_OK;
+}
+
+int dsmc_download_rot_temperature(dsmc_handle h, double *Trot)
+{
+    if (!h || !Trot) return fail(DSMC_ERR_ARG, "null argument");
+    if (h->P.in.zeta == 0) return fail(DSMC_ERR_ARG, "rot_temperature: the gas is monatomic (zeta = 0)");
+    const size_t C = h->ncell;
+    std::vector<double> a(h->P.sample_blocks * C);
+    uint64_t ns = 0;
+    int r = dsmc_download_samples(h, a.data(), &ns);
+    if (r) return r;
+    rot_temperature_from_sums(h->P, h->T, a.data(), C, Trot);
+    return DSMC_OK;
+}
+
+int dsmc_upload_rotational(dsmc_handle h, int64_t n, const float *er)
+{
+    if (!h || (n > 0 && !er)) return fail(DSMC_ERR_ARG, "bad argument");
+    if (h->P.in.zeta == 0) return fail(DSMC_ERR_ARG, "upload_rotational: the gas is monatomic (zeta = 0)");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    if (n != (int64_t)c.n_live)
+        return fail(DSMC_ERR_ARG, "upload_rotational: n must equal the particles of the current state");
+    for (int64_t i = 0; i < n; ++i)
+        if (!(er[i] >= 0.0f) || !std::isfinite(er[i]))
+            return fail(DSMC_ERR_ARG, "upload_rotational: energies must be finite and >= 0");
+    const SoA &S = (c.step & 1u) ? h->B : h->A;
+    if (n) HIPCHK(hipMemcpyAsync(S.er, er, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+int dsmc_download_rotational(dsmc_handle h, int moved, int64_t cap, float *er, int64_t *n_out)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    if (h->P.in.zeta == 0) return fail(DSMC_ERR_ARG, "download_rotational: the gas is monatomic (zeta = 0)");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    const uint32_t n = moved ? c.n_move : c.n_live;
+    if (n_out) *n_out = n;
+    if ((int64_t)n > cap) return fail(DSMC_ERR_ARG, "download buffer too small");
+    const SoA &S = (c.step & 1u) ? h->B : h->A;
+    if (n && er) HIPCHK(hipMemcpyAsync(er, S.er, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return DSMC_OK;
 }
 

@@ -240,6 +240,12 @@ int dsmc_batch_create(const dsmc_params *params, const dsmc_body *bodies, int n,
     for (int i = 0; i < n; ++i) {
         std::string msg = b->P[i].derive(params[i]);
         if (!msg.empty()) { delete b; return fail(DSMC_ERR_ARG, "instance " + std::to_string(i) + ": " + msg); }
+        if (params[i].zeta != 0) {
+            delete b;
+            return fail(DSMC_ERR_ARG, "instance " + std::to_string(i) +
+                                          ": batched instances are monatomic (zeta = 0); run a diatomic gas "
+                                          "with a dsmc_handle");
+        }
         const dsmc_body *bd = (bodies && bodies[i].n_points > 0) ? &bodies[i] : nullptr;
         msg = build_cut_tables(b->P[i], bd, b->T[i]);
         if (!msg.empty()) { delete b; return fail(DSMC_ERR_GEOMETRY, "instance " + std::to_string(i) + ": " + msg); }

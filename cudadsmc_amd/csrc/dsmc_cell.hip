@@ -287,12 +287,14 @@ __device__ void group_sort(int *S, const int *st, const int *cells, int count, i
     }
 }
 
-// Collision constants of one cell: VHS sigma*c_r = K1 (c_r^2)^e_exp, and the
-// Philox key (instance-local cell id, instance seed).
+// Collision constants of one cell: VHS sigma*c_r = K1 (c_r^2)^e_exp, the
+// Philox key (instance-local cell id, instance seed) and, for a diatomic gas,
+// the Larsen-Borgnakke exchange fraction and exponents.
 struct CK {
     float K1, e_exp;
     uint32_t seed;
     int lc;
+    float rot_p, lb_ia, lb_ib;
 };
 
 // Number of NTC candidates of a cell with remainder carry:
@@ -310,12 +312,13 @@ __device__ __forceinline__ int ntc_count(float ntc_coef, int np, float vol, floa
 
 // Post-collision velocities: isotropic (VHS) scattering of the relative
 // velocity about the centre of mass (Particle.cpp:84-100, GPUCell.cu:278-310).
-// The scattered relative velocity depends only on the candidate's random
-// numbers and |g|, not on whether it is accepted, so the rounds compute it
-// before the acceptance test (it then overlaps the cross-section chain);
-// scatter_pair applies it to an accepted pair.
+// The scattering direction and |g| depend only on the candidate's random
+// numbers and velocities, not on whether it is accepted, so the rounds
+// compute them before the acceptance test (it then overlaps the
+// cross-section chain); scatter_pair applies them to an accepted pair.
 struct Scatter {
-    float rx, ry, rz;
+    float cx, cy, cz;   // unit direction of the scattered relative velocity
+    float g;            // its magnitude (|g|, or the Larsen-Borgnakke g')
 };
 
 __device__ __forceinline__ Scatter scatter_dir(const CK &k, uint32_t step, int m, uint32_t w3, float g2)
@@ -325,17 +328,38 @@ __device__ __forceinline__ Scatter scatter_dir(const CK &k, uint32_t step, int m
     const uint4 w2 = philox((uint32_t)m, step, SUB_COLLIDE, 1u, (uint32_t)k.lc, k.seed);
     float sp, cp;
     dm_sincos2pi(u01(w2.x), sp, cp);
-    const float g = sqrtf(g2);
-    return Scatter{ g * cosr, g * (sinr * cp), g * (sinr * sp) };
+    return Scatter{ cosr, sinr * cp, sinr * sp, sqrtf(g2) };
 }
 
 __device__ __forceinline__ void scatter_pair(const Scatter &r, float &v1x, float &v1y, float &v1z, float &v2x,
                                              float &v2y, float &v2z)
 {
+    const float rx = r.g * r.cx, ry = r.g * r.cy, rz = r.g * r.cz;
     const float mx = 0.5f * (v1x + v2x), my = 0.5f * (v1y + v2y), mz = 0.5f * (v1z + v2z);
-    v1x = mx + 0.5f * r.rx; v2x = mx - 0.5f * r.rx;
-    v1y = my + 0.5f * r.ry; v2y = my - 0.5f * r.ry;
-    v1z = mz + 0.5f * r.rz; v2z = mz - 0.5f * r.rz;
+    v1x = mx + 0.5f * rx; v2x = mx - 0.5f * rx;
+    v1y = my + 0.5f * ry; v2y = my - 0.5f * ry;
+    v1z = mz + 0.5f * rz; v2z = mz - 0.5f * rz;
+}
+
+// Larsen-Borgnakke exchange of an accepted pair of a diatomic gas (zeta = 2;
+// the scheme of GPUCell.cu:256-275,316-328, written there and commented out):
+// with probability rot_p the collision energy per unit mass
+// E = g^2/4 + e1 + e2 (reduced mass m/2) is split into translation, fraction
+// x ~ Beta(5/2 - omega, 2) drawn as U1^(1/(5/2-omega)) U2^(1/(7/2-omega)),
+// and rotation, shared uniformly between the two; |g'| = 2 sqrt(x E).
+__device__ __forceinline__ void lb_exchange(const CK &k, uint32_t step, int m, float g2, float &e1, float &e2,
+                                            Scatter &sd)
+{
+    const uint4 w = philox((uint32_t)m, step, SUB_COLLIDE, 2u, (uint32_t)k.lc, k.seed);
+    if (!(u01(w.x) < k.rot_p)) return;
+    const float ec = 0.25f * g2 + (e1 + e2);
+    const float x = dm_expf(k.lb_ia * dm_logf(u01o(w.y)) + k.lb_ib * dm_logf(u01o(w.z)));
+    const float et = x * ec;
+    const float erot = ec - et;
+    const float a = erot * u01(w.w);
+    e1 = a;
+    e2 = erot - a;
+    sd.g = 2.0f * sqrtf(et);
 }
 
 __device__ __forceinline__ float vhs_sigc(const CK &k, float g2)
@@ -347,9 +371,9 @@ __device__ __forceinline__ float vhs_sigc(const CK &k, float g2)
 // G lanes per cell, and runs conflict-free rounds of up to G candidates in
 // each cell (the same rule as ntc_rounds, within G-lane segments).  list[]
 // holds the chunk-local cell numbers; results go to sh.sm / sh.nc.
-template <int G, bool GLOBAL, class KF>
+template <int G, bool GLOBAL, bool ROT, class KF>
 __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list, int nlist, CellShared &sh,
-                           float *Vx, float *Vy, float *Vz, int wave, int nwaves)
+                           float *Vx, float *Vy, float *Vz, float *Er, int wave, int nwaves)
 {
     const int lane = lane_id(), g = lane / G, sub = lane % G;
     const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (g * G);
@@ -363,6 +387,7 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
         const int M = has ? sh.M[i] : 0;
         const CK k = kf.key(i, c0 + i);
         float *vx = Vx + o, *vy = Vy + o, *vz = Vz + o;
+        float *er = ROT ? Er + o : nullptr;
         float smax = has ? sh.sm[i] : 0.0f;
         int nc = 0, mb = 0;
         while (__ballot(mb < M)) {
@@ -388,7 +413,7 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
             const bool proc = sub < f;
             float sc = 0.0f, g2 = 0.0f;
             float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
-            Scatter sd{ 0.f, 0.f, 0.f };
+            Scatter sd{ 0.f, 0.f, 0.f, 0.f };
             if (proc) {
                 v1x = vx[i1]; v1y = vy[i1]; v1z = vz[i1];
                 v2x = vx[i2]; v2y = vy[i2]; v2z = vz[i2];
@@ -406,6 +431,11 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
             const float sml = fmaxf(smax, pm);
             const bool accd = proc && (u01(w.z) < sc / sml);
             if (accd) {
+                if constexpr (ROT) {
+                    float e1 = er[i1], e2 = er[i2];
+                    lb_exchange(k, step, m, g2, e1, e2, sd);
+                    er[i1] = e1; er[i2] = e2;
+                }
                 scatter_pair(sd, v1x, v1y, v1z, v2x, v2y, v2z);
                 vx[i1] = v1x; vy[i1] = v1y; vz[i1] = v1z;
                 vx[i2] = v2x; vy[i2] = v2y; vz[i2] = v2z;
@@ -425,9 +455,9 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
 // round; (sigma c)max becomes a prefix max.  Identical results.
 // GLOBAL: velocities in HBM; LMARK: conflict marks in LDS (else found by
 // shuffles).
-template <bool GLOBAL, bool LMARK = !GLOBAL>
+template <bool GLOBAL, bool ROT, bool LMARK = !GLOBAL>
 __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, float *Vy,
-                          float *Vz, float &smax, int *mark)
+                          float *Vz, float *Er, float &smax, int *mark)
 {
     const int lane = lane_id();
     int nc = 0;
@@ -469,7 +499,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
         const bool proc = lane < f;
         float sc = 0.0f, g2 = 0.0f;
         float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
-        Scatter sd{ 0.f, 0.f, 0.f };
+        Scatter sd{ 0.f, 0.f, 0.f, 0.f };
         if (proc) {
             v1x = Vx[i1]; v1y = Vy[i1]; v1z = Vz[i1];
             v2x = Vx[i2]; v2y = Vy[i2]; v2z = Vz[i2];
@@ -487,6 +517,11 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
         const float sml = fmaxf(smax, pm);
         const bool accd = proc && (u01(w.z) < sc / sml);
         if (accd) {
+            if constexpr (ROT) {
+                float e1 = Er[i1], e2 = Er[i2];
+                lb_exchange(k, step, m, g2, e1, e2, sd);
+                Er[i1] = e1; Er[i2] = e2;
+            }
             scatter_pair(sd, v1x, v1y, v1z, v2x, v2y, v2z);
             Vx[i1] = v1x; Vy[i1] = v1y; Vz[i1] = v1z;
             Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
@@ -538,7 +573,7 @@ struct KSingle {
     const DevCase &cs;
     bool sampling;
     __device__ void prepare(CellShared &, int, int) const {}
-    __device__ CK key(int, int c) const { return CK{ cs.K1, cs.e_exp, cs.seed, c }; }
+    __device__ CK key(int, int c) const { return CK{ cs.K1, cs.e_exp, cs.seed, c, cs.rot_p, cs.lb_ia, cs.lb_ib }; }
     __device__ float ntc_coef(int) const { return cs.ntc_coef; }
     __device__ bool sample(int) const { return sampling; }
     __device__ void add(int, uint32_t, uint32_t) const {}
@@ -565,7 +600,7 @@ struct KBatch {
     __device__ CK key(int i, int c) const
     {
         const int u = inst[i];
-        return CK{ b.cc[u].K1, b.cc[u].e_exp, b.cc[u].seed, c - b.cbase[u] };
+        return CK{ b.cc[u].K1, b.cc[u].e_exp, b.cc[u].seed, c - b.cbase[u], 0.f, 0.f, 0.f };   // monatomic
     }
     __device__ float ntc_coef(int i) const { return b.cc[inst[i]].ntc_coef; }
     __device__ bool sample(int i) const
@@ -581,10 +616,11 @@ struct KBatch {
 };
 
 // One tile: cells [ca, cb), particles [p0, p0+n).  LDS selects where the
-// permutation and velocities are staged (block LDS, or HBM in place).
-template <bool LDS, class KF>
+// permutation, velocities and (ROT: diatomic gas) rotational energies are
+// staged (block LDS, or HBM in place).
+template <bool LDS, bool ROT, class KF>
 __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const SoA &nxt, uint32_t step,
-                          int ca, int cb, int p0, int n, int *Sp, float *Vx, float *Vy, float *Vz,
+                          int ca, int cb, int p0, int n, int *Sp, float *Vx, float *Vy, float *Vz, float *Er,
                           CellShared &sh, uint32_t &tcoll, uint32_t &tcand, int *lmarks, int lmark_cap)
 {
     const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
@@ -653,13 +689,14 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     {
         const float *__restrict__ cx = cur.x, *__restrict__ cy = cur.y;
         const float *__restrict__ cvx = cur.vx, *__restrict__ cvy = cur.vy, *__restrict__ cvz = cur.vz;
+        const float *__restrict__ cer = cur.er;
         const uint32_t *__restrict__ cp = cur.pid;
         float *__restrict__ ox = nxt.x + p0, *__restrict__ oy = nxt.y + p0;
         uint32_t *__restrict__ op = nxt.pid + p0;
         for (int k0 = tid; k0 < n; k0 += GATHER_UNROLL * bs) {
             int q[GATHER_UNROLL];
             float gx[GATHER_UNROLL], gy[GATHER_UNROLL], gvx[GATHER_UNROLL], gvy[GATHER_UNROLL],
-                gvz[GATHER_UNROLL];
+                gvz[GATHER_UNROLL], ge[GATHER_UNROLL];
             uint32_t gp[GATHER_UNROLL];
 #pragma unroll
             for (int j = 0; j < GATHER_UNROLL; ++j) {
@@ -671,6 +708,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 if (q[j] >= 0) {
                     gx[j] = cx[q[j]]; gy[j] = cy[q[j]]; gp[j] = cp[q[j]];
                     gvx[j] = cvx[q[j]]; gvy[j] = cvy[q[j]]; gvz[j] = cvz[q[j]];
+                    if constexpr (ROT) ge[j] = cer[q[j]];
                 }
 #pragma unroll
             for (int j = 0; j < GATHER_UNROLL; ++j)
@@ -678,6 +716,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                     const int k = k0 + j * bs;
                     ox[k] = gx[j]; oy[k] = gy[j]; op[k] = gp[j];
                     Vx[k] = gvx[j]; Vy[k] = gvy[j]; Vz[k] = gvz[j];
+                    if constexpr (ROT) Er[k] = ge[j];
                 }
         }
     }
@@ -717,7 +756,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             kf.add(i, 0u, (uint32_t)M);
         }
         __syncthreads();
-        ntc_groups<NTC_G, !LDS>(kf, step, c0, sh.big, sh.nbig, sh, Vx, Vy, Vz, wave, nwaves);
+        ntc_groups<NTC_G, !LDS, ROT>(kf, step, c0, sh.big, sh.nbig, sh, Vx, Vy, Vz, Er, wave, nwaves);
         if (!LDS) __threadfence();
         __syncthreads();
         const int nh = sh.nheavy;
@@ -729,14 +768,16 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             // gather (LDS tiles); for a single cell staged in HBM, the block's
             // idle LDS staging area when the cell fits, else shuffles
             int ncoll;
+            float *eo = ROT ? Er + o : nullptr;
             if (LDS)
-                ncoll = ntc_rounds<false>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax, Sp + o);
+                ncoll = ntc_rounds<false, ROT>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o, eo, smax,
+                                               Sp + o);
             else if (lmarks && k <= lmark_cap)
-                ncoll = ntc_rounds<true, true>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o, smax,
-                                               lmarks);
+                ncoll = ntc_rounds<true, ROT, true>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o, eo,
+                                                    smax, lmarks);
             else
-                ncoll = ntc_rounds<true, false>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o,
-                                                smax, Sp + o);
+                ncoll = ntc_rounds<true, ROT, false>(kf.key(i, c0 + i), step, k, sh.M[i], Vx + o, Vy + o, Vz + o,
+                                                     eo, smax, Sp + o);
             if (lane == 0) { sh.sm[i] = smax; sh.nc[i] = ncoll; }
         }
         if (!LDS) __threadfence();
@@ -763,12 +804,13 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 const int o = i < nc ? sh.st[i] : 0;
                 const int k = i < nc ? sh.st[i + 1] - o : 0;
                 const bool on = k > 0 && kf.sample(i);
-                double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+                double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
                 if (on)
                     for (int q = sub; q < k; q += SAMPLE_G) {
                         const double u = (double)Vx[o + q], v = (double)Vy[o + q], w = (double)Vz[o + q];
                         sx += u; sy += v; sz += w;
                         s2 += u * u + v * v + w * w;
+                        if constexpr (ROT) se += (double)Er[o + q];
                     }
 #pragma unroll
                 for (int d = SAMPLE_G / 2; d > 0; d >>= 1) {
@@ -776,6 +818,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                     sy += __shfl_xor(sy, d, SAMPLE_G);
                     sz += __shfl_xor(sz, d, SAMPLE_G);
                     s2 += __shfl_xor(s2, d, SAMPLE_G);
+                    if constexpr (ROT) se += __shfl_xor(se, d, SAMPLE_G);
                 }
                 // one lane per cell adds into the accumulators: the FP64
                 // adds are done at the memory side (the same IEEE sum, one
@@ -788,6 +831,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                     unsafeAtomicAdd(&a.acc[3 * C + c], sz);
                     unsafeAtomicAdd(&a.acc[4 * C + c], s2);
                     unsafeAtomicAdd(&a.acc[5 * C + c], (double)sh.nc[i]);
+                    if constexpr (ROT) unsafeAtomicAdd(&a.acc[6 * C + c], se);
                 }
             }
         }
@@ -800,20 +844,24 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             nxt.vx[p0 + k] = Vx[k];
             nxt.vy[p0 + k] = Vy[k];
             nxt.vz[p0 + k] = Vz[k];
+            if constexpr (ROT) nxt.er[p0 + k] = Er[k];
         }
     __syncthreads();
 }
 
 struct NoBatch {};
 
-template <bool BATCH>
+template <bool BATCH, bool ROT>
 __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
 {
-    // LDS staging of a tile: permutation and velocities; for a single cell
-    // staged in HBM it holds the collision conflict marks instead
-    __shared__ union { int i[4 * CELL_CAP]; float f[4 * CELL_CAP]; } stage;
+    // LDS staging of a tile: permutation, velocities and (ROT) rotational
+    // energies; for a single cell staged in HBM it holds the collision
+    // conflict marks instead
+    constexpr int NSTAGE = ROT ? 5 : 4;
+    __shared__ union { int i[NSTAGE * CELL_CAP]; float f[NSTAGE * CELL_CAP]; } stage;
     int *const s_perm = stage.i;
     float *const s_vx = stage.f + CELL_CAP, *const s_vy = stage.f + 2 * CELL_CAP, *const s_vz = stage.f + 3 * CELL_CAP;
+    float *const s_er = ROT ? stage.f + 4 * CELL_CAP : nullptr;
     __shared__ CellShared sh;
     __shared__ typename std::conditional<BATCH, BatchShared, NoBatch>::type bsh;
     Ctrl *ctrl = a.ctrl;
@@ -882,11 +930,12 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
         }
         auto run = [&](const auto &kf) {
             if (n <= CELL_CAP)
-                cell_tile<true>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, sh, tcoll, tcand,
-                                nullptr, 0);
+                cell_tile<true, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, s_er, sh, tcoll,
+                                     tcand, nullptr, 0);
             else    // a single cell: its collision rounds alone use the idle staging LDS for marks
-                cell_tile<false>(a, kf, cur, nxt, step, ca, cb, p0, n, a.perm + p0, nxt.vx + p0, nxt.vy + p0,
-                                 nxt.vz + p0, sh, tcoll, tcand, stage.i, 4 * CELL_CAP);
+                cell_tile<false, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, a.perm + p0, nxt.vx + p0, nxt.vy + p0,
+                                      nxt.vz + p0, ROT ? nxt.er + p0 : nullptr, sh, tcoll, tcand, stage.i,
+                                      4 * CELL_CAP);
         };
         if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step });
         else run(KSingle{ a.cs, sampling });
@@ -966,7 +1015,12 @@ hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int 
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     CellArgs a{ cs, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
                 keep_perm, nullptr, nullptr, 0, nullptr, tile_p };
-    hipLaunchKernelGGL(k_cell<false>, dim3(grid), dim3(256), 0, st, a);
+    if (cs.zeta != 0) {
+        if (!A.er || !B.er) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_cell<false, true>), dim3(grid), dim3(256), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_cell<false, false>), dim3(grid), dim3(256), 0, st, a);
+    }
     return hipGetLastError();
 }
 
@@ -979,23 +1033,22 @@ hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int 
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, 0, cc, cbase, ninst,
                 ictl, tile_p };
-    hipLaunchKernelGGL(k_cell<true>, dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_cell<true, false>), dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
-int occupancy_cell()
+int occupancy_cell(bool rot)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell<false>, 256, 0) != hipSuccess ||
-        nb < 1)
-        nb = 1;
+    const void *f = rot ? (const void *)k_cell<false, true> : (const void *)k_cell<false, false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess || nb < 1) nb = 1;
     return nb;
 }
 
 int occupancy_cell_batch()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell<true>, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_cell<true, false>, 256, 0) != hipSuccess ||
         nb < 1)
         nb = 1;
     return nb;

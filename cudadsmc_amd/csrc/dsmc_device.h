@@ -21,10 +21,12 @@ constexpr int FILL_TRIES = 1024;
 constexpr float SEG_TOL = 1e-5f;   // relative overlap at body segment ends
 constexpr float SEG_BAND = 1e-4f;  // wall band (fraction of a cell) caught as an immediate hit
 
-// Structure-of-arrays particle state (24 B per particle).
+// Structure-of-arrays particle state (24 B per particle; 28 B with the
+// rotational energy per unit mass er of a diatomic gas, else er is null).
 struct SoA {
     float *x, *y, *vx, *vy, *vz;
     uint32_t *pid;
+    float *er;
 };
 
 // Per-instance constants, passed by value to every kernel.
@@ -41,6 +43,9 @@ struct DevCase {
     float inj_a[4];
     float inj_vmp, inj_sig, inj_ux, inj_uy;
     uint32_t seed, cap;
+    int zeta;                      // rotational degrees of freedom: 0, or 2 (Larsen-Borgnakke)
+    float rot_p, lb_ia, lb_ib;     // exchange fraction, 1/(5/2-omega), 1/(7/2-omega)
+    float body_ekt, face_ekt[4], inj_ekt;   // kT/m of diffuse walls and the freestream
     const float *seg;
     const int *cseg_start, *cseg_idx;
     const float *vol_ntc;
@@ -166,11 +171,17 @@ __device__ __forceinline__ int cell_of(const DevCase &cs, float x, float y)
     return j + i * cs.ny;
 }
 
+// Rotational energy per unit mass of a molecule with two rotational degrees
+// of freedom in equilibrium at kT/m = ekt: -ln(U) ekt, exponential with mean
+// ekt (cudaDSMC Random::RotationalenergySample, Random.cpp:48-51).
+__device__ __forceinline__ float rot_energy(uint32_t w, float ekt) { return -dm_logf(u01o(w)) * ekt; }
+
 // Diffuse re-emission from a wall with unit normal (nx, ny) into the fluid:
-// flux-weighted normal speed, independent Gaussian tangential and z parts.
-__device__ __forceinline__ void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
-                                             uint32_t seed, uint32_t step, uint32_t b,
-                                             float &vx, float &vy, float &vz)
+// flux-weighted normal speed, independent Gaussian tangential and z parts;
+// with rot, the rotational energy accommodated to the wall (kTw/m = ekt).
+__device__ __forceinline__ void diffuse_emit(float nx, float ny, float vmp, float sig, float ekt, bool rot,
+                                             uint32_t pid, uint32_t seed, uint32_t step, uint32_t b,
+                                             float &vx, float &vy, float &vz, float &er)
 {
     uint4 w = philox(b, step, SUB_MOVE, 0u, pid, seed);
     float vn = sqrtf(-dm_logf(u01o(w.x))) * vmp;
@@ -181,16 +192,20 @@ __device__ __forceinline__ void diffuse_emit(float nx, float ny, float vmp, floa
     vx = vn * nx + vt * (-ny);
     vy = vn * ny + vt * nx;
     vz = r * s;
+    if (rot) er = rot_energy(w.w, ekt);
 }
 
 // Flight of duration t with wall events (faces and body segments), earliest
 // event first.  Returns true if the particle left through a stream face.
 // pid is only read (from *pidp) when a diffuse wall needs random numbers.
+// vchg reports a changed velocity; with zeta = 2 a diffuse wall re-draws er
+// as well, and echg reports that (er is only written then).
 __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, float &vx, float &vy,
-                                         float &vz, const uint32_t *pidp, uint32_t pid_known,
+                                         float &vz, float &er, const uint32_t *pidp, uint32_t pid_known,
                                          bool have_pid, uint32_t step, float t, uint32_t &err,
-                                         bool &vchg)
+                                         bool &vchg, bool &echg)
 {
+    const bool rot = cs.zeta != 0;
     bool dead = false;
     for (int b = 0;; ++b) {
         if (b == MAX_BOUNCE) { err |= ERR_BOUNCE; break; }
@@ -308,8 +323,9 @@ __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, 
                 float nx = kind == 0 ? 1.0f : (kind == 1 ? -1.0f : 0.0f);
                 float ny = kind == 2 ? 1.0f : (kind == 3 ? -1.0f : 0.0f);
                 uint32_t pid = have_pid ? pid_known : *pidp;
-                diffuse_emit(nx, ny, cs.face_vmp[kind], cs.face_sig[kind], pid, cs.seed, step,
-                             (uint32_t)b, vx, vy, vz);
+                diffuse_emit(nx, ny, cs.face_vmp[kind], cs.face_sig[kind], cs.face_ekt[kind], rot, pid, cs.seed,
+                             step, (uint32_t)b, vx, vy, vz, er);
+                echg = rot;
             }
         } else {
             const float *S = cs.seg + 6 * (kind - 4);
@@ -320,8 +336,9 @@ __device__ __forceinline__ bool move_one(const DevCase &cs, float &x, float &y, 
                 vy = vy - 2.0f * vn * sny;
             } else {
                 uint32_t pid = have_pid ? pid_known : *pidp;
-                diffuse_emit(snx, sny, cs.body_vmp, cs.body_sig, pid, cs.seed, step, (uint32_t)b,
-                             vx, vy, vz);
+                diffuse_emit(snx, sny, cs.body_vmp, cs.body_sig, cs.body_ekt, rot, pid, cs.seed, step,
+                             (uint32_t)b, vx, vy, vz, er);
+                echg = rot;
             }
         }
     }
@@ -344,7 +361,7 @@ __device__ __forceinline__ void inject_counts(const DevCase &cs, uint32_t step, 
 // One inflow particle on face f: position on the face, drifting-Maxwellian
 // flux-weighted velocity, then flight for a random fraction of dt.
 __device__ __forceinline__ bool inject_one(const DevCase &cs, int f, uint32_t pid, uint32_t step,
-                                           float &x, float &y, float &vx, float &vy, float &vz,
+                                           float &x, float &y, float &vx, float &vy, float &vz, float &er,
                                            uint32_t &err)
 {
     uint4 w = philox(0u, step, SUB_INJECT, 0u, pid, cs.seed);
@@ -390,8 +407,12 @@ __device__ __forceinline__ bool inject_one(const DevCase &cs, int f, uint32_t pi
     else if (f == 1) { x = cs.L1;        y = upos * cs.L2; vx = -vn; vy = cs.inj_uy + cs.inj_sig * g0; }
     else if (f == 2) { x = upos * cs.L1; y = 0.0f;         vy = vn;  vx = cs.inj_ux + cs.inj_sig * g0; }
     else             { x = upos * cs.L1; y = cs.L2;        vy = -vn; vx = cs.inj_ux + cs.inj_sig * g0; }
-    bool vchg = false;
-    return move_one(cs, x, y, vx, vy, vz, nullptr, pid, true, step, ufrac * cs.dt, err, vchg);
+    // rotational energy of the freestream (GPUParticle.cu:409 draws it and
+    // leaves it unstored, D13)
+    er = 0.0f;
+    if (cs.zeta != 0) er = rot_energy(philox(0u, step, SUB_INJECT, 1u, pid, cs.seed).x, cs.inj_ekt);
+    bool vchg = false, echg = false;
+    return move_one(cs, x, y, vx, vy, vz, er, nullptr, pid, true, step, ufrac * cs.dt, err, vchg, echg);
 }
 
 }  // namespace dsmc

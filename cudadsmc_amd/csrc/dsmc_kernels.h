@@ -56,7 +56,7 @@ hipError_t launch_wall_distance(const WallInst *inst, int ninst, int max_cells, 
 // Resident blocks per CU of the persistent kernels (occupancy query).
 int occupancy_move();
 int occupancy_move_batch();
-int occupancy_cell();
+int occupancy_cell(bool rot = false);
 int occupancy_cell_batch();
 int occupancy_scatter();
 

@@ -94,23 +94,26 @@ __global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, int2 *__
     }
     uint32_t err = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
+    const bool rot = cs.zeta != 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntot; i += stride) {
-        float x, y, vx, vy, vz;
+        float x, y, vx, vy, vz, er = 0.0f;
         bool dead;
         if (i < n_live) {
             x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
-            bool vchg = false;
-            dead = move_one(cs, x, y, vx, vy, vz, S.pid + i, 0u, false, step, cs.dt, err, vchg);
+            bool vchg = false, echg = false;
+            dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
             S.x[i] = x; S.y[i] = y;
             if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
+            if (echg) S.er[i] = er;     // re-drawn at a diffuse wall (zeta = 2 only)
         } else {
             uint32_t q = i - n_live;
             int f = 0;
             uint32_t acc = (uint32_t)J[0];
             while (f < 3 && q >= acc) { ++f; acc += (uint32_t)J[f]; }
             uint32_t pid = next_pid + q;
-            dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, err);
+            dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, er, err);
             S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
+            if (rot) S.er[i] = er;
         }
         const int c = dead ? -1 : cell_of(cs, x, y);
         cellslot[i] = make_int2(c, agg_slot(count, c));
@@ -329,6 +332,10 @@ __global__ __launch_bounds__(256) void k_fill(DevCase cs, SoA A, const int *__re
             A.vy[i] = uy[c] + sig[c] * (r1 * s1);
             A.vz[i] = sig[c] * (r2 * c2);
             A.pid[i] = (uint32_t)i;
+            // rotational energy at the cell's kT/m = sig^2 (Run.cpp:134)
+            if (cs.zeta != 0)
+                A.er[i] = rot_energy(philox((uint32_t)k, 0u, SUB_INIT, 0x80000000u, (uint32_t)c, cs.seed).x,
+                                     sig[c] * sig[c]);
         }
     }
     if (err) atomicOr(&ctrl->err, err);
@@ -375,11 +382,13 @@ __global__ __launch_bounds__(256) void k_sample(DevCase cs, SoA A, SoA B, const 
         const int c = base + g;
         const int o = c < cs.ncell ? start[c] : 0;
         const int k = c < cs.ncell ? start[c + 1] - o : 0;
-        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+        const bool rot = cs.zeta != 0;
+        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
         for (int q = sub; q < k; q += SAMPLE_G) {
             const double u = (double)S.vx[o + q], v = (double)S.vy[o + q], w = (double)S.vz[o + q];
             sx += u; sy += v; sz += w;
             s2 += u * u + v * v + w * w;
+            if (rot) se += (double)S.er[o + q];
         }
 #pragma unroll
         for (int d = SAMPLE_G / 2; d > 0; d >>= 1) {
@@ -387,6 +396,7 @@ __global__ __launch_bounds__(256) void k_sample(DevCase cs, SoA A, SoA B, const 
             sy += __shfl_xor(sy, d, SAMPLE_G);
             sz += __shfl_xor(sz, d, SAMPLE_G);
             s2 += __shfl_xor(s2, d, SAMPLE_G);
+            if (rot) se += __shfl_xor(se, d, SAMPLE_G);
         }
         if (k > 0 && sub == 0) {
             unsafeAtomicAdd(&acc[c], (double)k);
@@ -395,6 +405,7 @@ __global__ __launch_bounds__(256) void k_sample(DevCase cs, SoA A, SoA B, const 
             unsafeAtomicAdd(&acc[3 * C + c], sz);
             unsafeAtomicAdd(&acc[4 * C + c], s2);
             unsafeAtomicAdd(&acc[5 * C + c], (double)ncoll_cell[c]);
+            if (rot) unsafeAtomicAdd(&acc[6 * C + c], se);
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->nsamples += 1ull;
@@ -468,12 +479,13 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
             if (inst != u) continue;
             todo = false;
             const DevCase &cs = cases[u];
-            float x, y, vx, vy, vz;
+            // batched instances are monatomic (zeta = 0): er is never used
+            float x, y, vx, vy, vz, er = 0.0f;
             bool dead;
             if (!inflow) {
                 x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
-                bool vchg = false;
-                dead = move_one(cs, x, y, vx, vy, vz, S.pid + i, 0u, false, step, cs.dt, err, vchg);
+                bool vchg = false, echg = false;
+                dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
                 S.x[i] = x; S.y[i] = y;
                 if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
             } else {
@@ -484,7 +496,7 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
                 uint32_t acc = (uint32_t)J[0];
                 while (f < 3 && q >= acc) { ++f; acc += (uint32_t)J[f]; }
                 const uint32_t pid = ictl[u].next_pid + q;
-                dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, err);
+                dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, er, err);
                 S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
             }
             c = dead ? -1 : cbase[u] + cell_of(cs, x, y);

@@ -42,6 +42,8 @@ std::string Param::derive(const dsmc_params &p)
     }
     if (p.wall_bc != DSMC_BC_DIFFUSE && p.wall_bc != DSMC_BC_SPECULAR) return "wall_bc must be DIFFUSE or SPECULAR";
     if (p.wall_bc == DSMC_BC_DIFFUSE && !(p.T_wall > 0)) return "diffuse body wall needs T_wall > 0";
+    if (p.zeta != 0 && p.zeta != 2) return "zeta must be 0 (monatomic) or 2 (diatomic, Larsen-Borgnakke)";
+    if (p.zeta == 2 && !(p.rot_prob > 0 && p.rot_prob <= 1)) return "rot_prob (1/Z_rot) must lie in (0, 1]";
 
     dx = p.L1 / p.N1;
     dy = p.L2 / p.N2;
@@ -83,6 +85,19 @@ std::string Param::derive(const dsmc_params &p)
         inj_e[f] = 0.0;
         if (p.face_bc[f] == DSMC_BC_STREAM)
             inj_e[f] = maxwell_flux(p.n_inf, inj_vmp, inj_a[f]) * len[f] * p.L3 * p.tau / Fn;
+    }
+    // Larsen-Borgnakke (zeta = 2): translational fraction Beta(5/2-omega, 2)
+    // as U1^(1/(5/2-omega)) U2^(1/(7/2-omega)); rotational energy per unit
+    // mass in equilibrium at T has mean kT/m
+    sample_blocks = 6;
+    if (p.zeta == 2) {
+        lb_ia = 1.0 / (2.5 - p.Vtl);
+        lb_ib = 1.0 / (3.5 - p.Vtl);
+        body_ekt = p.wall_bc == DSMC_BC_DIFFUSE ? p.boltz * p.T_wall / p.mass : 0.0;
+        for (int f = 0; f < 4; ++f)
+            face_ekt[f] = p.face_bc[f] == DSMC_BC_DIFFUSE ? p.boltz * p.face_T[f] / p.mass : 0.0;
+        inj_ekt = p.boltz * p.T / p.mass;
+        sample_blocks = 7;
     }
     seed32 = (uint32_t)(p.seed ^ (p.seed >> 32));
     return "";

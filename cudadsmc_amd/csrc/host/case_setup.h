@@ -30,6 +30,11 @@ struct Param {
     double face_vmp[4]{}, face_sig[4]{};
     double inj_e[4]{}, inj_a[4]{};
     double inj_vmp = 0, inj_sig = 0;
+    // rotational energy (zeta = 2): Larsen-Borgnakke exponents 1/(5/2-omega),
+    // 1/(7/2-omega) and kT/m of the diffuse walls and the freestream
+    double lb_ia = 0, lb_ib = 0;
+    double body_ekt = 0, face_ekt[4]{}, inj_ekt = 0;
+    int sample_blocks = 6;
     uint32_t seed32 = 0;
 
     // Validates the input and computes the derived constants.

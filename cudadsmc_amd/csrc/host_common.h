@@ -78,6 +78,12 @@ inline DevCase make_devcase(const Param &P, const CutTables &T, int64_t cap)
     cs.inj_vmp = (float)P.inj_vmp; cs.inj_sig = (float)P.inj_sig;
     cs.inj_ux = (float)p->V_jet; cs.inj_uy = (float)p->V_y;
     cs.seed = P.seed32; cs.cap = (uint32_t)cap;
+    cs.zeta = p->zeta;
+    if (p->zeta == 2) {
+        cs.rot_p = (float)p->rot_prob; cs.lb_ia = (float)P.lb_ia; cs.lb_ib = (float)P.lb_ib;
+        cs.body_ekt = (float)P.body_ekt; cs.inj_ekt = (float)P.inj_ekt;
+        for (int f = 0; f < 4; ++f) cs.face_ekt[f] = (float)P.face_ekt[f];
+    }
     // Box outside which no cell lists a segment: the segments' bounding box
     // grown by the list margin (case_setup.cpp), two cells for the distance
     // between a particle and its cell's far side, and a float safety band.
@@ -121,6 +127,18 @@ inline void fields_from_sums(const Param &P, const CutTables &T, const double *a
         if (uz) uz[c] = w;
         if (Tt) Tt[c] = t;
         if (p) p[c] = nn * k * t;
+    }
+}
+
+// Rotational temperature of the C cells from the seventh sample block (sum of
+// the rotational energy per unit mass): T_rot = 2 m <e> / (zeta k).
+inline void rot_temperature_from_sums(const Param &P, const CutTables &T, const double *acc, size_t stride,
+                                      double *Tr)
+{
+    const double k = P.in.boltz, m = P.in.mass, z = (double)P.in.zeta;
+    for (int c = 0; c < T.ncell; ++c) {
+        const double sn = acc[c];
+        Tr[c] = sn > 0 ? 2.0 * m * (acc[6 * stride + c] / sn) / (z * k) : 0.0;
     }
 }
 

@@ -143,7 +143,8 @@ class Run:
 
     def download(self, moved=False):
         """Current state (n_live particles), or with moved=True the n_move
-        particles processed by the last particlemove()."""
+        particles processed by the last particlemove(); a diatomic gas
+        (zeta = 2) adds er, the rotational energy per unit mass (J/kg)."""
         s = self.status()
         n = s.n_move if moved else s.n_live
         out = [np.empty(max(n, 1), np.float32) for _ in range(5)]
@@ -151,7 +152,17 @@ class Run:
         nn = C.c_int64()
         fn = self.lib.dsmc_download_moved if moved else self.lib.dsmc_download_particles
         check(fn(self.h, n, *[ptr(v, C.c_float) for v in out], ptr(pid, C.c_uint32), C.byref(nn)))
-        return dict(x=out[0][:n], y=out[1][:n], vx=out[2][:n], vy=out[3][:n], vz=out[4][:n], pid=pid[:n])
+        d = dict(x=out[0][:n], y=out[1][:n], vx=out[2][:n], vy=out[3][:n], vz=out[4][:n], pid=pid[:n])
+        if self.info.zeta:
+            er = np.empty(max(n, 1), np.float32)
+            check(self.lib.dsmc_download_rotational(self.h, 1 if moved else 0, n, ptr(er, C.c_float), C.byref(nn)))
+            d["er"] = er[:n]
+        return d
+
+    def upload_rotational(self, er):
+        """Rotational energies per unit mass of the current state (zeta = 2)."""
+        e = abi.f32(er)
+        check(self.lib.dsmc_upload_rotational(self.h, e.shape[0], ptr(e, C.c_float)))
 
     def tables(self):
         i = self.info
@@ -221,6 +232,8 @@ class Run:
             if int(z["ncell"]) != self.ncell:
                 raise ValueError(f"checkpoint has {int(z['ncell'])} cells, this case {self.ncell}")
             self.upload(z["x"], z["y"], z["vx"], z["vy"], z["vz"], z["pid"])
+            if self.info.zeta:
+                self.upload_rotational(z["er"])
             self.set_cell_state(z["sigcmax"], z["mrem"])
             c = [int(v) for v in z["counters"]]
             s = abi.dsmc_status()
@@ -240,17 +253,29 @@ class Run:
         check(self.lib.dsmc_reset_samples(self.h))
 
     def samples(self):
-        acc = np.empty(6 * self.ncell, np.float64)
+        """Sample sums, (sample_blocks, ncell): n, vx, vy, vz, v^2, collisions
+        and, for zeta = 2, the rotational energy per unit mass."""
+        nb = self.info.sample_blocks
+        acc = np.empty(nb * self.ncell, np.float64)
         ns = C.c_uint64()
         check(self.lib.dsmc_download_samples(self.h, ptr(acc, C.c_double), C.byref(ns)))
-        return acc.reshape(6, self.ncell), ns.value
+        return acc.reshape(nb, self.ncell), ns.value
 
     def fields(self):
+        """n, ux, uy, uz, T (translational), p; with zeta = 2 also Trot and
+        Tov = (3 T + zeta Trot) / (3 + zeta)."""
         out = [np.empty(self.ncell, np.float64) for _ in range(6)]
         check(self.lib.dsmc_download_fields(self.h, *[ptr(v, C.c_double) for v in out]))
         shape = (self.params.N1, self.params.N2)
         names = ("n", "ux", "uy", "uz", "T", "p")
-        return {k: v.reshape(shape) for k, v in zip(names, out)}
+        f = {k: v.reshape(shape) for k, v in zip(names, out)}
+        z = self.info.zeta
+        if z:
+            tr = np.empty(self.ncell, np.float64)
+            check(self.lib.dsmc_download_rot_temperature(self.h, ptr(tr, C.c_double)))
+            f["Trot"] = tr.reshape(shape)
+            f["Tov"] = (3.0 * f["T"] + z * f["Trot"]) / (3.0 + z)
+        return f
 
     # -- launch settings (results never depend on them) --
     def get_launch(self):

@@ -62,6 +62,13 @@ typedef struct dsmc_params {
     uint64_t seed;       /* Philox key of this instance */
     int64_t capacity;    /* particle capacity; 0 = 4x the initial fill estimate */
     int    sample_start; /* first step whose state is sampled; < 0 = never */
+    /* Rotational energy (src/Param.h:14 zeta, the reference's N2): 0 for a
+     * monatomic gas, 2 for a diatomic one with Larsen-Borgnakke exchange in
+     * collisions.  rot_prob = 1/Z_rot, the fraction of collisions that
+     * exchange, in (0, 1] (1: every collision, as the reference's scheme);
+     * ignored for zeta = 0. */
+    int    zeta;
+    double rot_prob;
 } dsmc_params;
 
 /* Immersed body: closed polygon, counter-clockwise, n_points >= 3
@@ -90,6 +97,11 @@ typedef struct dsmc_case_info {
     double cell_volume;          /* full cell volume dx*dy*L3 */
     double ppc_eff;              /* particles per full fluid cell at n_inf */
     int64_t capacity;            /* particle capacity */
+    int    zeta;                 /* rotational degrees of freedom, 0 or 2 */
+    int    sample_blocks;        /* SoA blocks of the sample accumulators: 6, 7 with zeta = 2 */
+    float  rot_p;                /* Larsen-Borgnakke: fraction of collisions that exchange */
+    float  lb_ia, lb_ib;         /* 1/(5/2-omega), 1/(7/2-omega): the translational-fraction draw */
+    float  body_ekt, face_ekt[4], inj_ekt;  /* kT/m of diffuse walls and freestream, J/kg */
 } dsmc_case_info;
 
 typedef struct dsmc_status {
@@ -167,6 +179,15 @@ int dsmc_upload_particles(dsmc_handle h, int64_t n, const float *x, const float 
                           const uint32_t *pid);
 int dsmc_download_particles(dsmc_handle h, int64_t cap, float *x, float *y, float *vx,
                             float *vy, float *vz, uint32_t *pid, int64_t *n_out);
+/* Rotational energy of a diatomic gas (zeta = 2; the reference's d_Erot,
+ * Particle.cuh:30, GPUParticle.cu:19,44-53), as energy per unit mass e in
+ * J/kg, E_rot = mass * e, one float per particle in the order of the
+ * particle arrays.  dsmc_upload_particles sets e = 0; dsmc_upload_rotational
+ * then sets the n = n_live energies of the current state (finite, >= 0).
+ * download: the current state (moved = 0) or the n_move particles of the
+ * last dsmc_move (moved = 1).  DSMC_ERR_ARG for a monatomic case. */
+int dsmc_upload_rotational(dsmc_handle h, int64_t n, const float *er);
+int dsmc_download_rotational(dsmc_handle h, int moved, int64_t cap, float *er, int64_t *n_out);
 
 /* The three phases of Run::solver (Run.cpp:159-162,245-256):
  *  dsmc_move     = Run::particlemove (GPUParticles::Move + DeleteInvalid) plus
@@ -213,8 +234,12 @@ int dsmc_restore_counters(dsmc_handle h, const dsmc_status *s);
 int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples);
 
 /* Sampling (GPUCells::Sample + DownloadToHost, GPUCell.cu:83-91,147-167).
- * acc[6*ncell]: six SoA blocks sum n, sum vx, sum vy, sum vz, sum v^2,
- * collisions.  Fields: number density n, velocity, temperature, pressure. */
+ * acc[S*ncell], S = dsmc_case_info.sample_blocks: SoA blocks sum n, sum vx,
+ * sum vy, sum vz, sum v^2, collisions and, for zeta = 2, the sum of the
+ * rotational energy per unit mass.  Fields: number density n, velocity,
+ * translational temperature, pressure; dsmc_download_rot_temperature gives
+ * T_rot = 2 m <e> / (zeta k) per cell (DSMC_ERR_ARG for a monatomic case),
+ * and the overall temperature is (3 T + zeta T_rot) / (3 + zeta). */
 /* One sample of the current state into the accumulators (GPUCells::Sample,
  * src/cell/Cell.cuh:22, GPUCell.cu:147-167,336-402, which Run::solver calls
  * every 100 steps, src/Run.cpp:294-296): per cell, the particle count, the
@@ -228,6 +253,7 @@ int dsmc_reset_samples(dsmc_handle h);
 int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples);
 int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,
                          double *uz, double *T, double *p);
+int dsmc_download_rot_temperature(dsmc_handle h, double *Trot);
 
 /* ---- Binning on caller device arrays -----------------------------------
  * The reference's GPUCells::CalparticleNum -> CalparticleStartIndex ->

@@ -142,13 +142,20 @@ static inline int cell_of(const or_case *cs, float x, float y)
     return j + i * cs->ny;
 }
 
+/* Rotational energy per unit mass of a molecule with two rotational degrees
+ * of freedom in equilibrium at kT/m = ekt: exponential with mean ekt, as
+ * Random::RotationalenergySample (Random.cpp:48-51) draws -ln(U) k T. */
+static inline float rot_energy(uint32_t w, float ekt) { return -or_logf(u01o(w)) * ekt; }
+
 /* Diffuse re-emission from a wall with unit normal n (into the fluid).
  * Reference: GPUSegment.cuh:59-84 and WallBoundary.cpp:33-43.  D7 fix:
  * independent draws -- normal speed from the flux-weighted (Rayleigh) law
- * v_n = sqrt(-ln U) sqrt(2kTw/m), tangential and z components N(0, kTw/m). */
-static void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
+ * v_n = sqrt(-ln U) sqrt(2kTw/m), tangential and z components N(0, kTw/m).
+ * With er != NULL the rotational energy is accommodated to the wall too
+ * (drawn at kTw/m = ekt); the reference has no rotational wall model. */
+static void diffuse_emit(float nx, float ny, float vmp, float sig, float ekt, uint32_t pid,
                          uint32_t seed, uint32_t step, uint32_t b,
-                         float *vx, float *vy, float *vz)
+                         float *vx, float *vy, float *vz, float *er)
 {
     uint32_t ctr[4] = { b, step, OR_SUB_MOVE, 0u }, w[4];
     or_philox(ctr, pid, seed, w);
@@ -160,6 +167,7 @@ static void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
     *vx = vn * nx + vt * (-ny);
     *vy = vn * ny + vt * nx;
     *vz = r * s;
+    if (er) *er = rot_energy(w[3], ekt);
 }
 
 /* One particle's flight of duration t.  Reference: moveParticles
@@ -171,7 +179,7 @@ static void diffuse_emit(float nx, float ny, float vmp, float sig, uint32_t pid,
  * specular faces mirror as WallBoundary::apply in Boundary.cuh:40-62, diffuse
  * faces re-emit). */
 static void move_one(const or_case *cs, float *px, float *py, float *pvx, float *pvy,
-                     float *pvz, uint32_t pid, uint32_t step, float t,
+                     float *pvz, float *per, uint32_t pid, uint32_t step, float t,
                      uint8_t *dead, uint32_t *err)
 {
     float x = *px, y = *py, vx = *pvx, vy = *pvy, vz = *pvz;
@@ -267,8 +275,8 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
             } else {
                 float nx = kind == 0 ? 1.0f : (kind == 1 ? -1.0f : 0.0f);
                 float ny = kind == 2 ? 1.0f : (kind == 3 ? -1.0f : 0.0f);
-                diffuse_emit(nx, ny, cs->face_vmp[kind], cs->face_sig[kind], pid,
-                             cs->seed, step, (uint32_t)b, &vx, &vy, &vz);
+                diffuse_emit(nx, ny, cs->face_vmp[kind], cs->face_sig[kind], cs->face_ekt[kind], pid,
+                             cs->seed, step, (uint32_t)b, &vx, &vy, &vz, per);
             }
         } else {
             const float *S = cs->seg + 6 * (kind - 4);
@@ -279,8 +287,8 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
                 vx = vx - 2.0f * vn * S[4];
                 vy = vy - 2.0f * vn * S[5];
             } else {
-                diffuse_emit(S[4], S[5], cs->body_vmp, cs->body_sig, pid, cs->seed,
-                             step, (uint32_t)b, &vx, &vy, &vz);
+                diffuse_emit(S[4], S[5], cs->body_vmp, cs->body_sig, cs->body_ekt, pid, cs->seed,
+                             step, (uint32_t)b, &vx, &vy, &vz, per);
             }
         }
     }
@@ -289,13 +297,13 @@ static void move_one(const or_case *cs, float *px, float *py, float *pvx, float 
 }
 
 void or_move(const or_case *cs, int n, float *x, float *y, float *vx, float *vy,
-             float *vz, const uint32_t *pid, uint32_t step, uint8_t *dead,
+             float *vz, float *er, const uint32_t *pid, uint32_t step, uint8_t *dead,
              uint32_t *err)
 {
     uint32_t e = 0;
     for (int i = 0; i < n; ++i)
-        move_one(cs, &x[i], &y[i], &vx[i], &vy[i], &vz[i], pid[i], step, cs->dt,
-                 &dead[i], &e);
+        move_one(cs, &x[i], &y[i], &vx[i], &vy[i], &vz[i], er ? &er[i] : NULL, pid[i], step,
+                 cs->dt, &dead[i], &e);
     *err |= e;
 }
 
@@ -318,7 +326,7 @@ void or_inject_counts(const or_case *cs, uint32_t step, int J[4])
 }
 
 static void inject_one(const or_case *cs, int f, uint32_t pid, uint32_t step,
-                       float *ox, float *oy, float *ovx, float *ovy, float *ovz,
+                       float *ox, float *oy, float *ovx, float *ovy, float *ovz, float *oer,
                        uint8_t *dead, uint32_t *err)
 {
     uint32_t ctr[4] = { 0u, step, OR_SUB_INJECT, 0u }, w[4];
@@ -366,12 +374,21 @@ static void inject_one(const or_case *cs, int f, uint32_t pid, uint32_t step,
     else if (f == 1) { x = cs->L1;        y = upos * cs->L2; vx = -vn; vy = cs->inj_uy + cs->inj_sig * g0; }
     else if (f == 2) { x = upos * cs->L1; y = 0.0f;          vy = vn;  vx = cs->inj_ux + cs->inj_sig * g0; }
     else             { x = upos * cs->L1; y = cs->L2;        vy = -vn; vx = cs->inj_ux + cs->inj_sig * g0; }
-    move_one(cs, &x, &y, &vx, &vy, &vz, pid, step, ufrac * cs->dt, dead, err);
+    /* rotational energy of the freestream (the draw GPUParticle.cu:409-420
+     * makes and leaves unstored, D13) */
+    float er = 0.0f;
+    if (oer) {
+        uint32_t c3[4] = { 0u, step, OR_SUB_INJECT, 1u };
+        or_philox(c3, pid, cs->seed, w);
+        er = rot_energy(w[0], cs->inj_ekt);
+    }
+    move_one(cs, &x, &y, &vx, &vy, &vz, oer ? &er : NULL, pid, step, ufrac * cs->dt, dead, err);
     *ox = x; *oy = y; *ovx = vx; *ovy = vy; *ovz = vz;
+    if (oer) *oer = er;
 }
 
 void or_inject(const or_case *cs, uint32_t step, uint32_t next_pid, int J_total,
-               float *x, float *y, float *vx, float *vy, float *vz,
+               float *x, float *y, float *vx, float *vy, float *vz, float *er,
                uint32_t *pid, uint8_t *dead, uint32_t *err)
 {
     int J[4];
@@ -382,7 +399,8 @@ void or_inject(const or_case *cs, uint32_t step, uint32_t next_pid, int J_total,
         while (f < 3 && q >= acc) { ++f; acc += J[f]; }
         uint32_t p = next_pid + (uint32_t)q;
         pid[q] = p;
-        inject_one(cs, f, p, step, &x[q], &y[q], &vx[q], &vy[q], &vz[q], &dead[q], &e);
+        inject_one(cs, f, p, step, &x[q], &y[q], &vx[q], &vy[q], &vz[q], er ? &er[q] : NULL,
+                   &dead[q], &e);
     }
     *err |= e;
 }
@@ -428,9 +446,21 @@ void or_scatter(int n, const int *cell, const int *slot, const int *start, int *
 /*  - D5: Gamma(5/2 - omega) from tgamma, folded into K1 by the host;        */
 /*  - candidate count uses the pair number np(np-1) (D24; np^2 over-counts   */
 /*    collisions by (1 + 1/<np>)), T_ref in the VHS cross-section.          */
+/* Rotational exchange (zeta = 2): the Larsen-Borgnakke scheme the reference */
+/* writes out and leaves commented (GPUCell.cu:256-275,316-328, D1), for the */
+/* pair at once -- the collision energy E_c = m_r g^2/2 + E_r1 + E_r2 is     */
+/* split into translation, fraction x ~ Beta(5/2 - omega, zeta), and         */
+/* rotation, shared between the two by Beta(zeta/2, zeta/2) (uniform for    */
+/* zeta = 2).  Deviations: the translational energy uses the reduced mass    */
+/* m_r = m/2 (the reference takes m, which breaks energy conservation), a    */
+/* fraction rot_p = 1/Z_rot of the collisions exchanges (1 = every one, the  */
+/* reference), and x = U1^(1/(5/2-omega)) U2^(1/(7/2-omega)) -- the product  */
+/* of Beta(a,1) and Beta(a+1,1), which is Beta(a,2) -- in place of the       */
+/* acceptance-rejection loop: the same law with a fixed number of draws.     */
+/* Energies are per unit mass: E_c/m = g^2/4 + e1 + e2, g' = 2 sqrt(x E_c/m).*/
 /* ------------------------------------------------------------------------- */
 static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
-                        const int *perm, float *vx, float *vy, float *vz,
+                        const int *perm, float *vx, float *vy, float *vz, float *er,
                         float *sigcmax, float *mrem)
 {
     if (np < 2) return 0;
@@ -463,6 +493,20 @@ static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
             float sp, cp;
             or_sincos2pi(u01(w2[0]), &sp, &cp);
             float g = sqrtf(g2);
+            if (er) {
+                uint32_t c3[4] = { (uint32_t)m, step, OR_SUB_COLLIDE, 2u }, w3[4];
+                or_philox(c3, (uint32_t)c, cs->seed, w3);
+                if (u01(w3[0]) < cs->rot_p) {
+                    float ec = 0.25f * g2 + (er[p1] + er[p2]);
+                    float x = or_expf(cs->lb_ia * or_logf(u01o(w3[1])) + cs->lb_ib * or_logf(u01o(w3[2])));
+                    float et = x * ec;
+                    float erot = ec - et;
+                    float e1 = erot * u01(w3[3]);
+                    er[p1] = e1;
+                    er[p2] = erot - e1;
+                    g = 2.0f * sqrtf(et);
+                }
+            }
             float rx = g * cosr, ry = g * (sinr * cp), rz = g * (sinr * sp);
             float mx = 0.5f * (vx[p1] + vx[p2]);
             float my = 0.5f * (vy[p1] + vy[p2]);
@@ -478,7 +522,7 @@ static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
 }
 
 void or_collide(const or_case *cs, uint32_t step, const int *start, const int *perm,
-                float *vx, float *vy, float *vz, float *sigcmax, float *mrem,
+                float *vx, float *vy, float *vz, float *er, float *sigcmax, float *mrem,
                 int *ncoll_cell, int nthreads)
 {
     int C = cs->ncell;
@@ -487,7 +531,7 @@ void or_collide(const or_case *cs, uint32_t step, const int *start, const int *p
 #endif
     for (int c = 0; c < C; ++c)
         ncoll_cell[c] = collide_cell(cs, c, step, start[c], start[c + 1] - start[c], perm,
-                                     vx, vy, vz, sigcmax, mrem);
+                                     vx, vy, vz, er, sigcmax, mrem);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -496,17 +540,20 @@ void or_collide(const or_case *cs, uint32_t step, const int *start, const int *p
 /* accumulators (D15): counts, sum v, sum v^2, collisions per cell.          */
 /* ------------------------------------------------------------------------- */
 void or_sample(int ncell, const int *start, const int *perm, const float *vx,
-               const float *vy, const float *vz, const int *ncoll_cell, double *acc)
+               const float *vy, const float *vz, const float *er, const int *ncoll_cell,
+               double *acc)
 {
     for (int c = 0; c < ncell; ++c) {
         int s0 = start[c], np = start[c + 1] - s0;
-        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0;
+        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
         for (int k = 0; k < np; ++k) {
             int p = perm[s0 + k];
             double a = (double)vx[p], b = (double)vy[p], d = (double)vz[p];
             sx += a; sy += b; sz += d;
             s2 += a * a + b * b + d * d;
+            if (er) se += (double)er[p];
         }
+        if (er) acc[6 * ncell + c] += se;
         acc[0 * ncell + c] += (double)np;
         acc[1 * ncell + c] += sx;
         acc[2 * ncell + c] += sy;
@@ -518,12 +565,14 @@ void or_sample(int ncell, const int *start, const int *perm, const float *vx,
 
 void or_reorder(int n, const int *perm,
                 const float *x, const float *y, const float *vx, const float *vy,
-                const float *vz, const uint32_t *pid,
-                float *ox, float *oy, float *ovx, float *ovy, float *ovz, uint32_t *opid)
+                const float *vz, const float *er, const uint32_t *pid,
+                float *ox, float *oy, float *ovx, float *ovy, float *ovz, float *oer,
+                uint32_t *opid)
 {
     for (int k = 0; k < n; ++k) {
         int p = perm[k];
         ox[k] = x[p]; oy[k] = y[p]; ovx[k] = vx[p]; ovy[k] = vy[p]; ovz[k] = vz[p];
+        if (er) oer[k] = er[p];
         opid[k] = pid[p];
     }
 }
@@ -551,8 +600,8 @@ static int inside_body(const or_case *cs, float px, float py)
 
 int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
                  const float *ux, const float *uy, const float *sig, int cap,
-                 float *x, float *y, float *vx, float *vy, float *vz, uint32_t *pid,
-                 int *count, uint32_t *err)
+                 float *x, float *y, float *vx, float *vy, float *vz, float *er,
+                 uint32_t *pid, int *count, uint32_t *err)
 {
     int C = cs->ncell;
     long long n = 0;
@@ -612,6 +661,12 @@ int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
             vy[i] = uy[c] + sig[c] * (r1 * s1);
             vz[i] = sig[c] * (r2 * c2);
             pid[i] = (uint32_t)i;
+            /* rotational energy at the cell's kT/m = sig^2 (Run.cpp:134) */
+            if (er) {
+                uint32_t cr[4] = { (uint32_t)k, 0u, OR_SUB_INIT, 0x80000000u };
+                or_philox(cr, (uint32_t)c, cs->seed, w);
+                er[i] = rot_energy(w[0], sig[c] * sig[c]);
+            }
         }
     }
     *err |= e;
@@ -635,7 +690,7 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
     for (int i = 0; i < n; ++i) {
         uint32_t e = 0;
         move_one(cs, &st->x[i], &st->y[i], &st->vx[i], &st->vy[i], &st->vz[i],
-                 st->pid[i], st->step, cs->dt, &st->dead[i], &e);
+                 st->er ? &st->er[i] : NULL, st->pid[i], st->step, cs->dt, &st->dead[i], &e);
         err |= e;
     }
     int J[4];
@@ -653,7 +708,7 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
         uint32_t e = 0;
         st->pid[n + q] = p;
         inject_one(cs, f, p, st->step, &st->x[n + q], &st->y[n + q], &st->vx[n + q],
-                   &st->vy[n + q], &st->vz[n + q], &st->dead[n + q], &e);
+                   &st->vy[n + q], &st->vz[n + q], st->er ? &st->er[n + q] : NULL, &st->dead[n + q], &e);
         einj |= e;
     }
     err |= einj;
@@ -702,13 +757,13 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
     for (int i = 0; i < ntot; ++i)
         if (st->cell[i] >= 0) st->perm[st->start[st->cell[i]] + st->slot[i]] = i;
 
-    or_collide(cs, st->step, st->start, st->perm, st->vx, st->vy, st->vz,
+    or_collide(cs, st->step, st->start, st->perm, st->vx, st->vy, st->vz, st->er,
                st->sigcmax, st->mrem, st->ncoll_cell, nt);
     long long tc = 0;
     for (int c = 0; c < C; ++c) tc += st->ncoll_cell[c];
     st->total_coll += tc;
     if (sample) {
-        or_sample(C, st->start, st->perm, st->vx, st->vy, st->vz, st->ncoll_cell, st->acc);
+        or_sample(C, st->start, st->perm, st->vx, st->vy, st->vz, st->er, st->ncoll_cell, st->acc);
         st->nsamples += 1;
     }
     int nn = st->start[C];
@@ -720,6 +775,7 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
         st->bx[k] = st->x[p]; st->by[k] = st->y[p];
         st->bvx[k] = st->vx[p]; st->bvy[k] = st->vy[p]; st->bvz[k] = st->vz[p];
         st->bpid[k] = st->pid[p];
+        if (st->er) st->ber[k] = st->er[p];
     }
     float *t;
     uint32_t *tp;
@@ -729,6 +785,7 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
     t = st->vy; st->vy = st->bvy; st->bvy = t;
     t = st->vz; st->vz = st->bvz; st->bvz = t;
     tp = st->pid; st->pid = st->bpid; st->bpid = tp;
+    t = st->er; st->er = st->ber; st->ber = t;
     st->n = nn;
     st->next_pid += (uint32_t)Jt;
     st->step += 1;

@@ -74,6 +74,12 @@ typedef struct or_case {
     const float *fbox;                 /* 4 per cell: fluid-part bounding box of cut cells */
     const uint64_t *fmask;             /* per cell: 8x8 sub-boxes of fbox that touch fluid */
     uint32_t seed;
+    /* rotational energy (Larsen-Borgnakke), zeta = 2 rotational degrees of
+     * freedom or 0 for a monatomic gas; energies are per unit mass, J/kg */
+    int   zeta;
+    float rot_p;                       /* fraction of collisions that exchange (1/Z_rot) */
+    float lb_ia, lb_ib;                /* 1/(5/2-omega), 1/(7/2-omega) */
+    float body_ekt, face_ekt[4], inj_ekt;  /* kT/m of the diffuse walls and the freestream */
 } or_case;
 
 /* ---- primitives ---------------------------------------------------------- */
@@ -83,40 +89,45 @@ float or_expf(float x);
 void  or_sincos2pi(float u, float *s, float *c);
 
 /* ---- hot-path phases ----------------------------------------------------- */
-/* Free flight + walls + body + outflow for particles [0,n). */
+/* Free flight + walls + body + outflow for particles [0,n).  er: rotational
+ * energy per unit mass (re-drawn at diffuse walls), NULL for zeta = 0. */
 void or_move(const or_case *cs, int n, float *x, float *y, float *vx, float *vy,
-             float *vz, const uint32_t *pid, uint32_t step, uint8_t *dead,
+             float *vz, float *er, const uint32_t *pid, uint32_t step, uint8_t *dead,
              uint32_t *err);
 /* Number of inflow particles created at `step` on each face. */
 void or_inject_counts(const or_case *cs, uint32_t step, int J[4]);
 /* Create this step's inflow particles at slots [0,J) of the output arrays. */
 void or_inject(const or_case *cs, uint32_t step, uint32_t next_pid, int J_total,
-               float *x, float *y, float *vx, float *vy, float *vz,
+               float *x, float *y, float *vx, float *vy, float *vz, float *er,
                uint32_t *pid, uint8_t *dead, uint32_t *err);
 /* Binning: cell id and stable rank (serial atomicAdd order). */
 void or_bin(const or_case *cs, int n, const float *x, const float *y,
             const uint8_t *dead, int *cell, int *count, int *slot);
 void or_scan(const int *count, int ncell, int *start /* ncell+1 */);
 void or_scatter(int n, const int *cell, const int *slot, const int *start, int *perm);
-/* NTC/VHS collisions of every cell, through perm; updates velocities in place. */
+/* NTC/VHS collisions of every cell, through perm; updates velocities (and,
+ * zeta = 2, rotational energies er) in place. */
 void or_collide(const or_case *cs, uint32_t step, const int *start, const int *perm,
-                float *vx, float *vy, float *vz, float *sigcmax, float *mrem,
+                float *vx, float *vy, float *vz, float *er, float *sigcmax, float *mrem,
                 int *ncoll_cell, int nthreads);
-/* FP64 sampling accumulators: acc[6*ncell] as 6 SoA blocks n,vx,vy,vz,v2,coll. */
+/* FP64 sampling accumulators: acc as SoA blocks of ncell: n, vx, vy, vz, v^2,
+ * collisions, and with er != NULL a seventh, the rotational energy sum. */
 void or_sample(int ncell, const int *start, const int *perm, const float *vx,
-               const float *vy, const float *vz, const int *ncoll_cell, double *acc);
+               const float *vy, const float *vz, const float *er, const int *ncoll_cell,
+               double *acc);
 void or_reorder(int n, const int *perm,
                 const float *x, const float *y, const float *vx, const float *vy,
-                const float *vz, const uint32_t *pid,
-                float *ox, float *oy, float *ovx, float *ovy, float *ovz, uint32_t *opid);
+                const float *vz, const float *er, const uint32_t *pid,
+                float *ox, float *oy, float *ovx, float *ovy, float *ovz, float *oer,
+                uint32_t *opid);
 
 /* Initial fill (Run::assignParticle): expected[c] = mean particle count of
  * cell c; cut[c] != 0 marks a partially solid cell (rejection sampling).
  * Writes into arrays of capacity cap; returns particle count (or -1). */
 int or_init_fill(const or_case *cs, const double *expected, const uint8_t *cut,
                  const float *ux, const float *uy, const float *sig, int cap,
-                 float *x, float *y, float *vx, float *vy, float *vz, uint32_t *pid,
-                 int *count, uint32_t *err);
+                 float *x, float *y, float *vx, float *vy, float *vz, float *er,
+                 uint32_t *pid, int *count, uint32_t *err);
 
 /* ---- batch helpers for the tests ------------------------------------------ */
 void or_logf_n(int n, const float *in, float *out);
@@ -135,6 +146,7 @@ typedef struct or_state {
     float *sigcmax, *mrem;
     double *acc; long long nsamples;
     long long total_coll;
+    float *er, *ber;                                 /* rotational energy (zeta = 2), else NULL */
 } or_state;
 
 /* One timestep; sample != 0 accumulates sampling sums.  nthreads <= 1 runs

@@ -43,6 +43,8 @@ class or_case(C.Structure):
         ("fbox", C.POINTER(C.c_float)),
         ("fmask", C.POINTER(C.c_uint64)),
         ("seed", C.c_uint32),
+        ("zeta", C.c_int), ("rot_p", C.c_float), ("lb_ia", C.c_float), ("lb_ib", C.c_float),
+        ("body_ekt", C.c_float), ("face_ekt", C.c_float * 4), ("inj_ekt", C.c_float),
     ]
 
 
@@ -64,6 +66,7 @@ class or_state(C.Structure):
         ("sigcmax", _F), ("mrem", _F),
         ("acc", _D), ("nsamples", C.c_longlong),
         ("total_coll", C.c_longlong),
+        ("er", _F), ("ber", _F),
     ]
 
 
@@ -91,16 +94,16 @@ def load():
         "or_logf_n": [C.c_int, _F, _F],
         "or_expf_n": [C.c_int, _F, _F],
         "or_sincos2pi_n": [C.c_int, _F, _F, _F],
-        "or_move": [P, C.c_int, _F, _F, _F, _F, _F, _U, C.c_uint32, _B, _U],
+        "or_move": [P, C.c_int, _F, _F, _F, _F, _F, _F, _U, C.c_uint32, _B, _U],
         "or_inject_counts": [P, C.c_uint32, _I],
-        "or_inject": [P, C.c_uint32, C.c_uint32, C.c_int, _F, _F, _F, _F, _F, _U, _B, _U],
+        "or_inject": [P, C.c_uint32, C.c_uint32, C.c_int, _F, _F, _F, _F, _F, _F, _U, _B, _U],
         "or_bin": [P, C.c_int, _F, _F, _B, _I, _I, _I],
         "or_scan": [_I, C.c_int, _I],
         "or_scatter": [C.c_int, _I, _I, _I, _I],
-        "or_collide": [P, C.c_uint32, _I, _I, _F, _F, _F, _F, _F, _I, C.c_int],
-        "or_sample": [C.c_int, _I, _I, _F, _F, _F, _I, _D],
-        "or_reorder": [C.c_int, _I, _F, _F, _F, _F, _F, _U, _F, _F, _F, _F, _F, _U],
-        "or_init_fill": [P, _D, _B, _F, _F, _F, C.c_int, _F, _F, _F, _F, _F, _U, _I, _U],
+        "or_collide": [P, C.c_uint32, _I, _I, _F, _F, _F, _F, _F, _F, _I, C.c_int],
+        "or_sample": [C.c_int, _I, _I, _F, _F, _F, _F, _I, _D],
+        "or_reorder": [C.c_int, _I, _F, _F, _F, _F, _F, _F, _U, _F, _F, _F, _F, _F, _F, _U],
+        "or_init_fill": [P, _D, _B, _F, _F, _F, C.c_int, _F, _F, _F, _F, _F, _F, _U, _I, _U],
         "or_step": [P, C.POINTER(or_state), C.c_int, C.c_int],
     }
     for name, args in sig.items():
@@ -163,7 +166,10 @@ class OracleCase:
                   "inj_vmp", "inj_sig", "inj_ux", "inj_uy", "margin", "body_bc", "seed"):
             setattr(c, k, getattr(info, k))
         c.dt = np.float32(info.dt)
+        for k in ("zeta", "rot_p", "lb_ia", "lb_ib", "body_ekt", "inj_ekt"):
+            setattr(c, k, getattr(info, k))
         for f in range(4):
+            c.face_ekt[f] = info.face_ekt[f]
             c.face_bc[f] = info.face_bc[f]
             c.face_vmp[f] = info.face_vmp[f]
             c.face_sig[f] = info.face_sig[f]
@@ -183,14 +189,20 @@ class OracleCase:
     def ncell(self):
         return self.info.ncell
 
+    @property
+    def zeta(self):
+        return self.info.zeta
+
     # --- phases ---
     def move(self, s, step):
-        """s: dict of x,y,vx,vy,vz (float32), pid; moved in place; returns dead, err."""
+        """s: dict of x,y,vx,vy,vz (float32), pid and, zeta = 2, er; moved in
+        place; returns dead, err."""
         n = s["x"].shape[0]
         dead = np.zeros(n, np.uint8)
         err = C.c_uint32(0)
+        er = s["er"] if self.zeta else None
         load().or_move(self.ptr, n, *[_p(s[k], C.c_float) for k in ("x", "y", "vx", "vy", "vz")],
-                       _p(s["pid"], C.c_uint32), step, _p(dead, C.c_uint8), C.byref(err))
+                       _p(er, C.c_float), _p(s["pid"], C.c_uint32), step, _p(dead, C.c_uint8), C.byref(err))
         return dead, err.value
 
     def inject_counts(self, step):
@@ -199,12 +211,14 @@ class OracleCase:
         return J
 
     def inject(self, step, next_pid, Jt):
-        out = {k: np.zeros(Jt, np.float32) for k in ("x", "y", "vx", "vy", "vz")}
+        names = ("x", "y", "vx", "vy", "vz", "er") if self.zeta else ("x", "y", "vx", "vy", "vz")
+        out = {k: np.zeros(Jt, np.float32) for k in names}
         out["pid"] = np.zeros(Jt, np.uint32)
         dead = np.zeros(Jt, np.uint8)
         err = C.c_uint32(0)
         load().or_inject(self.ptr, step, next_pid, Jt, *[_p(out[k], C.c_float) for k in ("x", "y", "vx", "vy", "vz")],
-                         _p(out["pid"], C.c_uint32), _p(dead, C.c_uint8), C.byref(err))
+                         _p(out.get("er"), C.c_float), _p(out["pid"], C.c_uint32), _p(dead, C.c_uint8),
+                         C.byref(err))
         return out, dead, err.value
 
     def bin(self, x, y, dead=None):
@@ -227,19 +241,24 @@ class OracleCase:
         load().or_scatter(cell.shape[0], _p(cell, C.c_int), _p(slot, C.c_int), _p(start, C.c_int), _p(perm, C.c_int))
         return perm
 
-    def collide(self, step, start, perm, vx, vy, vz, sigcmax, mrem, nthreads=1):
+    def collide(self, step, start, perm, vx, vy, vz, sigcmax, mrem, nthreads=1, er=None):
+        """er: rotational energies (zeta = 2), updated in place; required then."""
+        if self.zeta and er is None:
+            raise ValueError("a diatomic case collides with its rotational energies")
         ncoll = np.zeros(self.ncell, np.int32)
         load().or_collide(self.ptr, step, _p(start, C.c_int), _p(perm, C.c_int), _p(vx, C.c_float),
-                          _p(vy, C.c_float), _p(vz, C.c_float), _p(sigcmax, C.c_float), _p(mrem, C.c_float),
-                          _p(ncoll, C.c_int), nthreads)
+                          _p(vy, C.c_float), _p(vz, C.c_float), _p(er if self.zeta else None, C.c_float),
+                          _p(sigcmax, C.c_float), _p(mrem, C.c_float), _p(ncoll, C.c_int), nthreads)
         return ncoll
 
-    def sample(self, start, perm, vx, vy, vz, ncoll, acc):
+    def sample(self, start, perm, vx, vy, vz, ncoll, acc, er=None):
         load().or_sample(self.ncell, _p(start, C.c_int), _p(perm, C.c_int), _p(vx, C.c_float), _p(vy, C.c_float),
-                         _p(vz, C.c_float), _p(ncoll, C.c_int), _p(acc, C.c_double))
+                         _p(vz, C.c_float), _p(er if self.zeta else None, C.c_float), _p(ncoll, C.c_int),
+                         _p(acc, C.c_double))
 
     def init_fill(self, expected, cut, ux, uy, sig, cap):
-        arrs = {k: np.zeros(cap, np.float32) for k in ("x", "y", "vx", "vy", "vz")}
+        names = ("x", "y", "vx", "vy", "vz", "er") if self.zeta else ("x", "y", "vx", "vy", "vz")
+        arrs = {k: np.zeros(cap, np.float32) for k in names}
         arrs["pid"] = np.zeros(cap, np.uint32)
         count = np.zeros(self.ncell, np.int32)
         err = C.c_uint32(0)
@@ -249,7 +268,8 @@ class OracleCase:
         n = load().or_init_fill(self.ptr, _p(e, C.c_double), _p(cut, C.c_uint8), _p(ux, C.c_float),
                                 _p(uy, C.c_float), _p(sig, C.c_float), cap,
                                 *[_p(arrs[k], C.c_float) for k in ("x", "y", "vx", "vy", "vz")],
-                                _p(arrs["pid"], C.c_uint32), _p(count, C.c_int), C.byref(err))
+                                _p(arrs.get("er"), C.c_float), _p(arrs["pid"], C.c_uint32), _p(count, C.c_int),
+                                C.byref(err))
         if n < 0:
             raise RuntimeError("oracle fill exceeds capacity")
         return {k: v[:n].copy() for k, v in arrs.items()}, count, err.value
@@ -266,7 +286,8 @@ class OracleState:
         st = or_state()
         st.n, st.cap, st.step = n, cap, step
         st.next_pid = int(parts["pid"].max()) + 1 if next_pid is None and n else (next_pid or 0)
-        for k in ("x", "y", "vx", "vy", "vz"):
+        self.keys = ("x", "y", "vx", "vy", "vz", "er") if case.zeta else ("x", "y", "vx", "vy", "vz")
+        for k in self.keys:
             a = np.zeros(cap, np.float32)
             a[:n] = parts[k]
             b = np.zeros(cap, np.float32)
@@ -287,7 +308,7 @@ class OracleState:
         self.ncoll = np.zeros(C_, np.int32)
         self.sigcmax = np.full(C_, np.float32(sigcmax0), np.float32)
         self.mrem = np.zeros(C_, np.float32)
-        self.acc = np.zeros(6 * C_, np.float64)
+        self.acc = np.zeros(case.info.sample_blocks * C_, np.float64)
         st.dead = _p(self.dead, C.c_uint8)
         st.cell, st.slot, st.perm = _p(self.cell, C.c_int), _p(self.slot, C.c_int), _p(self.perm, C.c_int)
         st.count, st.start, st.ncoll_cell = _p(self.count, C.c_int), _p(self.start, C.c_int), _p(self.ncoll, C.c_int)
@@ -307,7 +328,7 @@ class OracleState:
 
     def particles(self):
         n = self.st.n
-        return {k: self._cur(k)[:n].copy() for k in ("x", "y", "vx", "vy", "vz", "pid")}
+        return {k: self._cur(k)[:n].copy() for k in self.keys + ("pid",)}
 
 
 def case_from_params(params, body=None):

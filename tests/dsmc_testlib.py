@@ -13,9 +13,9 @@ def small_box(n_cells=16, ppc=8, seed=7):
     return cfg["params"], None
 
 
-def small_cylinder(nx=80, ny=40, ppc=10, n_points=64, seed=11, wall=abi.BC_DIFFUSE):
+def small_cylinder(nx=80, ny=40, ppc=10, n_points=64, seed=11, wall=abi.BC_DIFFUSE, gas=cases.ARGON, rot_prob=1.0):
     """The Mach-10 cylinder case on a coarse grid (same physics, fewer cells)."""
-    cfg = cases.cylinder_mach10(ppc=ppc, nx=nx, ny=ny, n_points=n_points, seed=seed)
+    cfg = cases.cylinder_mach10(ppc=ppc, nx=nx, ny=ny, n_points=n_points, seed=seed, gas=gas, rot_prob=rot_prob)
     p = cfg["params"]
     p.wall_bc = wall
     return p, cfg["body"]
@@ -31,8 +31,14 @@ def oracle_fill(oc, info, tables):
     return parts, count, err
 
 
+def fields_of(oc):
+    """Particle fields of a case: with rotational energy (zeta = 2), er too."""
+    return FIELDS + ("er",) if oc.zeta else FIELDS
+
+
 def oracle_phases(oc, parts, step, next_pid, sigcmax, mrem, cap):
     """One oracle step split like the GPU phases.  parts is not modified."""
+    FIELDS = fields_of(oc)
     s = {k: parts[k].copy() for k in FIELDS}
     n = s["x"].shape[0]
     dead, err = oc.move(s, step)
@@ -49,11 +55,14 @@ def oracle_phases(oc, parts, step, next_pid, sigcmax, mrem, cap):
     sm = sigcmax.copy()
     mr = mrem.copy()
     vx, vy, vz = allp["vx"].copy(), allp["vy"].copy(), allp["vz"].copy()
-    ncoll = oc.collide(step, start, perm, vx, vy, vz, sm, mr)
-    acc = np.zeros(6 * oc.ncell)
-    oc.sample(start, perm, vx, vy, vz, ncoll, acc)
+    er = allp["er"].copy() if oc.zeta else None
+    ncoll = oc.collide(step, start, perm, vx, vy, vz, sm, mr, er=er)
+    acc = np.zeros(oc.info.sample_blocks * oc.ncell)
+    oc.sample(start, perm, vx, vy, vz, ncoll, acc, er=er)
     new = {k: allp[k][perm] for k in ("x", "y", "pid")}
     new["vx"], new["vy"], new["vz"] = vx[perm], vy[perm], vz[perm]
+    if oc.zeta:
+        new["er"] = er[perm]
     cell_alive = cell.copy()
     return dict(moved=allp, dead=alld, err=err | ierr, cell=cell_alive, count=count, start=start,
                 perm=perm, new=new, sigcmax=sm, mrem=mr, ncoll=ncoll, acc=acc, Jt=Jt)

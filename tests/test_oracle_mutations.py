@@ -18,7 +18,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
-PIN_SUITES = ["tests/test_oracle_pins.py", "tests/test_oracle_walls.py", "tests/test_oracle_step.py"]
+PIN_SUITES = ["tests/test_oracle_pins.py", "tests/test_oracle_walls.py", "tests/test_oracle_step.py",
+              "tests/test_oracle_rotation.py"]
 
 # (id, what the mistake is, exact text of the oracle, its replacement)
 MUTATIONS = [
@@ -32,8 +33,8 @@ MUTATIONS = [
      "y = upos * cs->L2; vx = vn;  vy = cs->inj_uy + cs->inj_sig * g0; }",
      "y = upos * cs->L2; vx = vn;  vy = cs->inj_sig * g0; }"),
     ("inflow_full_step", "inflow particles fly the whole step instead of a random part",
-     "move_one(cs, &x, &y, &vx, &vy, &vz, pid, step, ufrac * cs->dt, dead, err);",
-     "move_one(cs, &x, &y, &vx, &vy, &vz, pid, step, cs->dt, dead, err);"),
+     "move_one(cs, &x, &y, &vx, &vy, &vz, oer ? &er : NULL, pid, step, ufrac * cs->dt, dead, err);",
+     "move_one(cs, &x, &y, &vx, &vy, &vz, oer ? &er : NULL, pid, step, cs->dt, dead, err);"),
     ("inflow_no_flux_weight", "inflow normal speed without the flux weight v_n",
      "float acc = (cand / ws) * or_expf(ref - d * d);", "float acc = or_expf(ref - d * d);"),
     ("specular_body_sign", "specular body reflection with the wrong sign",
@@ -78,6 +79,24 @@ MUTATIONS = [
      "for (int c = 0; c < ncell; ++c) { s += count[c]; start[c] = s; }"),
     ("fill_vy_cosine", "fill v_y from the cosine of the v_x pair (v_x, v_y correlated)",
      "vy[i] = uy[c] + sig[c] * (r1 * s1);", "vy[i] = uy[c] + sig[c] * (r1 * c1);"),
+    # rotational energy (zeta = 2)
+    ("lb_full_mass", "collision energy with the full mass in E_t (the reference's bookkeeping)",
+     "float ec = 0.25f * g2 + (er[p1] + er[p2]);", "float ec = 0.5f * g2 + (er[p1] + er[p2]);"),
+    ("lb_beta_one_factor", "translational share Beta(5/2 - omega, 1): one factor of the product dropped",
+     "float x = or_expf(cs->lb_ia * or_logf(u01o(w3[1])) + cs->lb_ib * or_logf(u01o(w3[2])));",
+     "float x = or_expf(cs->lb_ia * or_logf(u01o(w3[1])));"),
+    ("lb_split_half", "rotational energy halved between the two (the reference's live code)",
+     "float e1 = erot * u01(w3[3]);", "float e1 = erot * 0.5f;"),
+    ("lb_rot_prob_ignored", "every collision exchanges whatever rot_prob is",
+     "if (u01(w3[0]) < cs->rot_p) {", "if (1) {"),
+    ("lb_speed_kept", "relative speed not rescaled to the new translational energy",
+     "g = 2.0f * sqrtf(et);", "(void)et;"),
+    ("wall_rot_not_accommodated", "diffuse walls keep the rotational energy",
+     "if (er) *er = rot_energy(w[3], ekt);", "(void)ekt;"),
+    ("inflow_rot_wall_temperature", "inflow rotational energy at the wall temperature",
+     "er = rot_energy(w[0], cs->inj_ekt);", "er = rot_energy(w[0], cs->body_ekt);"),
+    ("fill_rot_per_dof", "fill rotational energy at kT/2 (one degree of freedom)",
+     "er[i] = rot_energy(w[0], sig[c] * sig[c]);", "er[i] = rot_energy(w[0], 0.5f * sig[c] * sig[c]);"),
 ]
 
 
