@@ -32,6 +32,8 @@ warm start against plain DSMC on 32 held-out cases (cudadsmc_amd.surrogate).
 
 Workload cylinder_mach10_400x200: the single Mach-10 cylinder (400x200
 cells, about 2M particles), 20,000 steps with sampling from 10,000.
+Workload cylinder_n2_mach10_400x200: the same case in nitrogen (zeta = 2,
+Larsen-Borgnakke exchange in every collision, Mach 10 at gamma = 7/5).
 """
 import argparse
 import json
@@ -305,7 +307,12 @@ def bench_cylinder(args):
     from cudadsmc_amd import cases
     from cudadsmc_amd.run import Run
 
-    cfg = cases.cylinder_mach10()
+    if args.workload == "cylinder_n2_mach10_400x200":
+        # the reference's gas: N2 with Larsen-Borgnakke exchange in every collision
+        cfg = cases.cylinder_mach10(gas=cases.NITROGEN)
+        cfg["name"] = args.workload
+    else:
+        cfg = cases.cylinder_mach10()
     p = cfg["params"]
     steps = args.steps if args.steps is not None else 19990
     develop = args.develop if args.develop is not None else 0
@@ -334,6 +341,7 @@ def bench_cylinder(args):
             "timed_steps": [develop + args.warmup, develop + args.warmup + steps],
             "sample_from_step": cfg["sample_from"], "launch": "eager with per-kernel HIP events",
             "launch_settings": run.get_launch(), "collisions": s1.collisions - s0.collisions, "seed": int(p.seed),
+            "gas": "N2 (zeta = 2, rot_prob %g)" % p.rot_prob if p.zeta else "Ar",
         },
         "graph_replay": graph,
         "roofline": roof,
@@ -350,7 +358,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--develop", type=int, default=None,
                     help="untimed development steps before the warm-up (sweep64: 1,500; cylinder: 0)")
-    ap.add_argument("--workload", default="sweep64", choices=["sweep64", "cylinder_mach10_400x200"])
+    ap.add_argument("--workload", default="sweep64",
+                    choices=["sweep64", "cylinder_mach10_400x200", "cylinder_n2_mach10_400x200"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-steady", action="store_true", help="skip the warm-start time-to-steady evaluation")

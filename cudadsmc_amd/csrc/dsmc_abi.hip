@@ -522,21 +522,24 @@ int build_graph(dsmc_handle h)
 // of the counters -- what each must read and write at least, per particle,
 // candidate, collision and cell, as below -- added to the timing totals.
 // Sums over the steps: live at the start (the particle-steps), injected,
-// moved (live + injected), live at the end.
+// moved (live + injected), live at the end.  A diatomic gas adds its
+// rotational energy: written for every injected particle by the move,
+// gathered and written back by the cell kernel, one more sample block.
 void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
 {
     const double C = h->ncell, k = K;
+    const double zr = h->P.in.zeta ? 1.0 : 0.0;
     const double snl = (double)(after.psteps - before.psteps);
     const double sJ = (double)(after.next_pid - before.next_pid);
     const double snm = snl + sJ;
     const double snn = snl - (double)before.n_live + (double)after.n_live;
     const double S = (double)(after.nsamples - before.nsamples);      // sampled steps
-    h->t_bytes[0] += 40.0 * snl + 24.0 * sJ + 12.0 * snm + 4.0 * C * k;
+    h->t_bytes[0] += 40.0 * snl + (24.0 + 4.0 * zr) * sJ + 12.0 * snm + 4.0 * C * k;
     h->t_bytes[1] += 8.0 * C * k;
     h->t_bytes[2] += 12.0 * snm;
-    h->t_bytes[3] += 52.0 * snn + 32.0 * (double)(after.ncand - before.ncand) +
+    h->t_bytes[3] += (52.0 + 8.0 * zr) * snn + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * k +
-                     (S > 0 ? 16.0 * snn * (S / k) + 96.0 * C * S : 0.0);
+                     (S > 0 ? 16.0 * snn * (S / k) + (96.0 + 16.0 * zr) * C * S : 0.0);
 }
 
 }  // namespace

@@ -11,6 +11,8 @@ Datasets, shape [N1][N2], index i*N2 + j (the cell order of the solver):
   Numchild, Numgrandchild  adaptive sub-cell counts (int; 0: no sub-cells)
   MFP      mean free path m / (sqrt(2) rho pi d^2) (T/T_inf)^(omega - 1/2)
            (Cell.cpp:125)
+  Trot     rotational temperature (K), a diatomic gas (zeta = 2) only; T is
+           then the translational temperature
 """
 import os
 
@@ -28,8 +30,11 @@ def datasets(fields, params):
     with np.errstate(divide="ignore", invalid="ignore"):
         mfp = np.where(rho > 0, params.mass / (np.sqrt(2.0) * rho * np.pi * params.diam ** 2)
                        * np.power(np.where(g["T"] > 0, g["T"], 1.0) / params.T, params.Vtl - 0.5), 0.0)
-    return dict(U=g["ux"], V=g["uy"], W=g["uz"], P=g["p"], T=g["T"], Rho=rho,
-                Numchild=np.zeros(shape, np.int32), Numgrandchild=np.zeros(shape, np.int32), MFP=mfp)
+    ds = dict(U=g["ux"], V=g["uy"], W=g["uz"], P=g["p"], T=g["T"], Rho=rho,
+              Numchild=np.zeros(shape, np.int32), Numgrandchild=np.zeros(shape, np.int32), MFP=mfp)
+    if "Trot" in g:
+        ds["Trot"] = g["Trot"]
+    return ds
 
 
 def write_npz(path, ds):
@@ -49,7 +54,7 @@ def write_vts(path, ds, params):
             f.write(" ".join(f"{i * dx:.9g} {j * dy:.9g} 0" for i in range(n1 + 1)))
             f.write("\n")
         f.write("</DataArray>\n</Points>\n<CellData>\n")
-        for name in NAMES:
+        for name in NAMES + (("Trot",) if "Trot" in ds else ()):
             a = ds[name]
             typ = "Int32" if a.dtype.kind == "i" else "Float64"
             f.write(f'<DataArray type="{typ}" Name="{name}" format="ascii">\n')

@@ -85,3 +85,25 @@ def test_cpp_sample_every_equals_python_sample(gpu_lib, tmp_path):
     for k in ("U", "V", "W", "P", "T", "Rho"):
         np.testing.assert_array_equal(got[k], ds[k], err_msg=k)
     r.close()
+
+
+def test_cpp_nitrogen_equals_python_run(gpu_lib, tmp_path):
+    """dsmc_run --gas nitrogen (zeta = 2, Z_rot = 5): the fields written,
+    the rotational temperature among them, equal the Python Run of the case
+    it recorded."""
+    out = tmp_path / "n2"
+    _run(["--gas", "nitrogen", "--zrot", "5", "--output-every", "300"], out)
+    case = json.load(open(out / "case.json"))
+    assert case["params"]["zeta"] == 2 and abs(case["params"]["rot_prob"] - 0.2) < 1e-15
+    p = cases.make_params(**case["params"])
+    r = Run(p, np.array(case["body"], np.float64))
+    r.initialize()
+    r.solver(300)
+    ds = output.datasets(r.fields(), p)
+    got = _load(out / "result_300.npz")
+    for k in ("U", "V", "W", "P", "T", "Rho", "Trot"):
+        np.testing.assert_array_equal(got[k], ds[k], err_msg=k)
+    vts = output.read_vts(str(out / "result_300.vts"))
+    np.testing.assert_allclose(vts["Trot"], ds["Trot"], rtol=1e-8)
+    assert np.nanmax(ds["Trot"]) > 500.0          # rotational energy heated in the shock layer
+    r.close()
