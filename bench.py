@@ -194,7 +194,11 @@ def time_to_steady(args):
         max_field_error_warm=r["max_field_error"], median_field_error_warm=float(sorted(e("field_error"))[
             len(r["field_error"]) // 2]),
         rel_l2_warm={k: float(max(d[k] for d in r["field_error"])) for k in ("n", "u", "T")},
-        fallback_rate=r["fallback_rate"], device_err=r["device_err"])
+        fallback_rate=r["fallback_rate"], device_err=r["device_err"],
+        # cases after which the one-off data generation and fit are paid back
+        # by the time each warm-started case saves when run alone
+        break_even_cases=(r["data_s"] + r["fit_s"]) / max(
+            (sum(r["wall_alone_plain_s"]) - sum(r["wall_alone_warm_s"])) / len(r["wall_alone_plain_s"]), 1e-9))
 
 
 # ------------------------------------------------------------------ workloads
@@ -363,10 +367,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-steady", action="store_true", help="skip the warm-start time-to-steady evaluation")
-    ap.add_argument("--steady-runs", type=int, default=512, help="DSMC runs that train the surrogate")
+    ap.add_argument("--steady-runs", type=int, default=1024, help="DSMC runs that train the surrogate")
     ap.add_argument("--steady-eval", type=int, default=32, help="held-out cases")
-    ap.add_argument("--steady-fit-steps", type=int, default=4000)
-    ap.add_argument("--steady-average", type=int, default=8000,
+    ap.add_argument("--steady-fit-steps", type=int, default=8000)
+    ap.add_argument("--steady-average", type=int, default=12000,
                     help="steps the fields are averaged over for the field errors")
     ap.add_argument("--tune", action="store_true", help="run the launch-settings autotuner first (untimed)")
     ap.add_argument("--profile-run", action="store_true",
