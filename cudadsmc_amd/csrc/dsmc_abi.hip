@@ -39,8 +39,8 @@ struct dsmc_handle_s {
     int ncell = 0, ntile_scan = 0;
     int64_t cap = 0;
     SoA A{}, B{};
-    int2 *cellslot = nullptr;
-    int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;
+    Bins bn{};                   // the step's binning (k_move, k_scan_runs, k_place, k_scatter)
+    int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;   // + fill
     int *start = nullptr, *tile_first = nullptr, *ncoll_cell = nullptr;
     float *sigcmax = nullptr, *mrem = nullptr;
     double *acc = nullptr;
@@ -53,7 +53,7 @@ struct dsmc_handle_s {
     double *d_expected = nullptr;
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
     int tile_p = CELL_P, graph_len = 8;
-    int keep_perm = 0, timing = 0;
+    int timing = 0;
     bool cell_order = false;     // the current state is in cell order, start[] its (a step completed)
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -86,15 +86,15 @@ int set_device(dsmc_handle h)
 int launch_step(dsmc_handle h, int phase_mask)
 {
     hipStream_t st = h->stream;
-    if (phase_mask & 1) HIPCHK(launch_move(h->cs, h->A, h->B, h->cellslot, h->count, h->ctrl, h->grid_move, st));
+    if (phase_mask & 1) HIPCHK(launch_move(h->cs, h->A, h->B, h->bn, h->ctrl, h->grid_move, st));
     if (phase_mask & 2) {
-        HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
-        HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
-                              h->tile_p, h->ctrl, h->grid_scatter, st));
+        HIPCHK(launch_scan_runs(h->bn, h->ncell, h->ctrl, st));
+        HIPCHK(launch_scatter(h->bn, h->start, h->perm, h->tile_first, h->ncell, h->tile_p, h->ctrl,
+                              h->grid_scatter, st));
     }
     if (phase_mask & 4) {
         HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->tile_first, h->sigcmax, h->mrem,
-                           h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->keep_perm, h->tile_p,
+                           h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->tile_p,
                            h->grid_cell, st));
         h->cell_order = true;
     } else if (phase_mask & 1) {
@@ -126,6 +126,7 @@ int reset_cell_state(dsmc_handle h)
     HIPCHK(hipMemsetAsync(h->mrem, 0, sizeof(float) * h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * h->P.sample_blocks * (size_t)h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(h->count, 0, sizeof(int) * h->ncell, h->stream));
+    HIPCHK(hipMemsetAsync(h->bn.cnt, 0, sizeof(unsigned long long) * h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(h->ncoll_cell, 0, sizeof(int) * h->ncell, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return DSMC_OK;
@@ -263,7 +264,13 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
             return bad(e, "particle buffers");
     }
     const int ntiles_cell = (int)((N + CELL_P_MIN - 1) / CELL_P_MIN) + 1;
-    if ((e = h->alloc(&h->cellslot, N)) || (e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
+    const size_t NR = (N + 63) / 64 * 64 + 64;   // run records: 64 per chunk of moved particles
+    Bins &bn = h->bn;
+    if ((e = h->alloc(&bn.cnt, C)) || (e = h->alloc(&bn.lstart, C)) || (e = h->alloc(&bn.tsum, h->ntile_scan)) ||
+        (e = h->alloc(&bn.toff, h->ntile_scan)) || (e = h->alloc(&bn.runrec, NR)) ||
+        (e = h->alloc(&bn.chunk_runs, NR / 64)) || (e = h->alloc(&bn.runlist, N)))
+        return bad(e, "binning buffers");
+    if ((e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
         (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||
         (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, C + 1)) ||
         (e = h->alloc(&h->tile_first, 2 * (size_t)ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
@@ -518,13 +525,16 @@ int build_graph(dsmc_handle h)
     return DSMC_OK;
 }
 
-// Algorithmic bytes of the four kernels over the K steps between two reads
-// of the counters -- what each must read and write at least, per particle,
-// candidate, collision and cell, as below -- added to the timing totals.
-// Sums over the steps: live at the start (the particle-steps), injected,
-// moved (live + injected), live at the end.  A diatomic gas adds its
-// rotational energy: written for every injected particle by the move,
-// gathered and written back by the cell kernel, one more sample block.
+// Algorithmic bytes of the four timing slots (k_move; k_scan_runs; k_place +
+// k_scatter; k_cell) over the K steps between two reads of the counters --
+// what each must read and write at least, per particle, run, candidate,
+// collision and cell -- added to the timing totals.  Sums over the steps:
+// live at the start (the particle-steps), injected, moved (live + injected),
+// live at the end, runs.  A run record is 16 B (written by k_move, read and
+// written again by k_place, read by k_scatter), a packed cell counter or
+// start 8 B, a chunk's run count 4 B per 64 moved particles.  A diatomic gas
+// adds its rotational energy: written for every injected particle by the
+// move, gathered and written back by the cell kernel, one more sample block.
 void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
 {
     const double C = h->ncell, k = K;
@@ -534,9 +544,10 @@ void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
     const double snm = snl + sJ;
     const double snn = snl - (double)before.n_live + (double)after.n_live;
     const double S = (double)(after.nsamples - before.nsamples);      // sampled steps
-    h->t_bytes[0] += 40.0 * snl + (24.0 + 4.0 * zr) * sJ + 12.0 * snm + 4.0 * C * k;
-    h->t_bytes[1] += 8.0 * C * k;
-    h->t_bytes[2] += 12.0 * snm;
+    const double R = (double)(after.nruns - before.nruns);            // (cell, wave) runs binned
+    h->t_bytes[0] += 40.0 * snl + (24.0 + 4.0 * zr) * sJ + (4.0 + 1.0 / 16.0) * snm + 16.0 * R + 8.0 * C * k;
+    h->t_bytes[1] += 16.0 * C * k;
+    h->t_bytes[2] += 48.0 * R + snm / 16.0 + 12.0 * C * k + 4.0 * snn;
     h->t_bytes[3] += (52.0 + 8.0 * zr) * snn + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * k +
                      (S > 0 ? 16.0 * snn * (S / k) + (96.0 + 16.0 * zr) * C * S : 0.0);
@@ -588,10 +599,10 @@ int timed_steps(dsmc_handle h, int nsteps)
         HIPCHK(hipEventRecord(ev[0], st));
         if ((r = launch_step(h, 1))) return r;
         HIPCHK(hipEventRecord(ev[1], st));
-        HIPCHK(launch_scan(h->count, h->lstart, h->tsum, h->toff, h->ncell, h->ctrl, st));
+        HIPCHK(launch_scan_runs(h->bn, h->ncell, h->ctrl, st));
         HIPCHK(hipEventRecord(ev[2], st));
-        HIPCHK(launch_scatter(h->cellslot, h->lstart, h->toff, h->start, h->perm, h->tile_first, h->ncell,
-                              h->tile_p, h->ctrl, h->grid_scatter, st));
+        HIPCHK(launch_scatter(h->bn, h->start, h->perm, h->tile_first, h->ncell, h->tile_p, h->ctrl,
+                              h->grid_scatter, st));
         HIPCHK(hipEventRecord(ev[3], st));
         if ((r = launch_step(h, 4))) return r;
         HIPCHK(hipEventRecord(ev[4], st));
@@ -667,9 +678,10 @@ int dsmc_status_get(dsmc_handle h, dsmc_status *o)
 
 int dsmc_set_keep_perm(dsmc_handle h, int on)
 {
+    // k_scatter writes the stable permutation itself and k_cell leaves it as
+    // it is, so perm is always kept; the setting is accepted and unused
+    (void)on;
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
-    h->keep_perm = on ? 1 : 0;
-    drop_graph(h);
     return DSMC_OK;
 }
 
@@ -682,12 +694,29 @@ int dsmc_download_bins(dsmc_handle h, int64_t cap, int *cell, int *count)
     if ((r = read_ctrl(h, c))) return r;
     if (cell) {
         if ((int64_t)c.n_move > cap) return fail(DSMC_ERR_ARG, "cell buffer too small");
-        std::vector<int2> cs(c.n_move);
-        if (c.n_move)
-            HIPCHK(hipMemcpy(cs.data(), h->cellslot, sizeof(int2) * c.n_move, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < c.n_move; ++i) cell[i] = cs[i].x;
+        // the cell of every moved particle, from the run records of its chunk
+        const size_t nch = ((size_t)c.n_move + 63) / 64;
+        std::vector<int> nr(nch);
+        std::vector<int4> rec(nch * 64);
+        if (nch) {
+            HIPCHK(hipMemcpy(nr.data(), h->bn.chunk_runs, sizeof(int) * nch, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(rec.data(), h->bn.runrec, sizeof(int4) * nch * 64, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < c.n_move; ++i) cell[i] = -1;
+        for (size_t q = 0; q < nch; ++q)
+            for (int k = 0; k < nr[q]; ++k) {
+                const int4 r = rec[q * 64 + k];
+                const uint64_t m = ((uint64_t)(uint32_t)r.w << 32) | (uint32_t)r.z;
+                for (int b = 0; b < 64; ++b)
+                    if ((m >> b) & 1u) cell[q * 64 + b] = r.x;
+            }
     }
-    if (count) HIPCHK(hipMemcpy(count, h->count, sizeof(int) * h->ncell, hipMemcpyDeviceToHost));
+    if (count) {
+        // the per-cell counters of the last move (the scan clears them)
+        std::vector<unsigned long long> v(h->ncell);
+        HIPCHK(hipMemcpy(v.data(), h->bn.cnt, sizeof(unsigned long long) * h->ncell, hipMemcpyDeviceToHost));
+        for (int k = 0; k < h->ncell; ++k) count[k] = (int)(uint32_t)v[k];
+    }
     return DSMC_OK;
 }
 

@@ -34,8 +34,8 @@ struct dsmc_batch_s {
     int ncell = 0, ntile_scan = 0;
     int64_t cap = 0;
     SoA A{}, B{};
-    int2 *cellslot = nullptr;
-    int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;
+    Bins bn{};                          // the step's binning over all instances' cells
+    int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;   // + fill
     int *start = nullptr, *tile_first = nullptr, *ncoll_cell = nullptr;
     float *sigcmax = nullptr, *mrem = nullptr;
     double *acc = nullptr;
@@ -86,12 +86,12 @@ int launch_step(dsmc_batch b, int mask)
 {
     hipStream_t st = b->stream;
     if (mask & 1)
-        HIPCHK(launch_move_batch(b->d_cases, b->ninst, b->d_cbase, b->A, b->B, b->cellslot, b->count, b->ctrl,
-                                 b->ictl, b->grid_move, st));
-    if (mask & 2) HIPCHK(launch_scan(b->count, b->lstart, b->tsum, b->toff, b->ncell, b->ctrl, st));
+        HIPCHK(launch_move_batch(b->d_cases, b->ninst, b->d_cbase, b->A, b->B, b->bn, b->ctrl, b->ictl,
+                                 b->grid_move, st));
+    if (mask & 2) HIPCHK(launch_scan_runs(b->bn, b->ncell, b->ctrl, st));
     if (mask & 4)
-        HIPCHK(launch_scatter(b->cellslot, b->lstart, b->toff, b->start, b->perm, b->tile_first, b->ncell,
-                              b->tile_p, b->ctrl, b->grid_scatter, st));
+        HIPCHK(launch_scatter(b->bn, b->start, b->perm, b->tile_first, b->ncell, b->tile_p, b->ctrl,
+                              b->grid_scatter, st));
     if (mask & 8)
         HIPCHK(launch_cell_batch(b->all, b->d_cc, b->d_cbase, b->ninst, b->ictl, b->A, b->B, b->start, b->perm,
                                  b->tile_first, b->sigcmax, b->mrem, b->ncoll_cell, b->acc, b->ctrl, b->tile_p,
@@ -145,6 +145,7 @@ int reset_cell_state(dsmc_batch b)
     HIPCHK(hipMemsetAsync(b->mrem, 0, sizeof(float) * C, b->stream));
     HIPCHK(hipMemsetAsync(b->acc, 0, sizeof(double) * 6 * C, b->stream));
     HIPCHK(hipMemsetAsync(b->count, 0, sizeof(int) * C, b->stream));
+    HIPCHK(hipMemsetAsync(b->bn.cnt, 0, sizeof(unsigned long long) * C, b->stream));
     HIPCHK(hipMemsetAsync(b->ncoll_cell, 0, sizeof(int) * C, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return DSMC_OK;
@@ -299,7 +300,13 @@ int setup_device(dsmc_batch b, dsmc_batch *out)
             (e = b->alloc(&s->vy, N)) || (e = b->alloc(&s->vz, N)) || (e = b->alloc(&s->pid, N)))
             return bad(e, "particle buffers");
     const int ntiles_cell = (int)((N + CELL_P_MIN - 1) / CELL_P_MIN) + 1;
-    if ((e = b->alloc(&b->cellslot, N)) || (e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
+    const size_t NR = (N + 63) / 64 * 64 + 64;   // run records: 64 per chunk of moved particles
+    Bins &bn = b->bn;
+    if ((e = b->alloc(&bn.cnt, Cs)) || (e = b->alloc(&bn.lstart, Cs)) || (e = b->alloc(&bn.tsum, b->ntile_scan)) ||
+        (e = b->alloc(&bn.toff, b->ntile_scan)) || (e = b->alloc(&bn.runrec, NR)) ||
+        (e = b->alloc(&bn.chunk_runs, NR / 64)) || (e = b->alloc(&bn.runlist, N)))
+        return bad(e, "binning buffers");
+    if ((e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
         (e = b->alloc(&b->lstart, Cs)) || (e = b->alloc(&b->tsum, b->ntile_scan)) ||
         (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, Cs + 1)) ||
         (e = b->alloc(&b->tile_first, 2 * (size_t)ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
@@ -486,8 +493,10 @@ int timed_steps(dsmc_batch b, int nsteps)
         if ((r = harvest(b, k % kTimeRing))) return r;
     if ((r = read_ctrl(b, after, &ia))) return r;
     // sums over the steps: live at the start (the step's particle-steps),
-    // injected, moved (live + injected), live at the end
+    // injected, moved (live + injected), live at the end, (cell, wave) runs;
+    // the per-unit figures are those of the single-instance handle
     const double K = nsteps, C = b->ncell;
+    const double R = (double)(after.nruns - before.nruns);
     const double snl = (double)(after.psteps - before.psteps);
     const double sJ = (double)(after.next_pid - before.next_pid);
     const double snm = snl + sJ;
@@ -499,9 +508,9 @@ int timed_steps(dsmc_batch b, int nsteps)
         const double nn_i = (double)(ia[i].psteps - ib[i].psteps) - ib[i].n_live + ia[i].n_live;
         sampled += 16.0 * nn_i * (S / K) + 96.0 * (double)(b->cbase[i + 1] - b->cbase[i]) * S;
     }
-    b->t_bytes[0] += 40.0 * snl + 24.0 * sJ + 12.0 * snm + 4.0 * C * K;
-    b->t_bytes[1] += 8.0 * C * K;
-    b->t_bytes[2] += 12.0 * snm;
+    b->t_bytes[0] += 40.0 * snl + 24.0 * sJ + (4.0 + 1.0 / 16.0) * snm + 16.0 * R + 8.0 * C * K;
+    b->t_bytes[1] += 16.0 * C * K;
+    b->t_bytes[2] += 48.0 * R + snm / 16.0 + 12.0 * C * K + 4.0 * snn;
     b->t_bytes[3] += 52.0 * snn + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * K + sampled;
     b->t_launches += nsteps;

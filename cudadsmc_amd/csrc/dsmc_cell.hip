@@ -3,24 +3,19 @@
 // One block (4 waves of 64) takes a tile of consecutive cells whose particles
 // occupy [p0, p0+n) of the cell-sorted order; tiles of about CELL_P particles
 // are handed out by a device-wide queue (n <= CELL_CAP: staged in LDS;
-// larger tiles stage in the destination arrays in HBM).  Per tile:
-//   1. stable order inside each cell -- the scatter leaves a cell's
-//      permutation segment as a concatenation of ascending runs, one per
-//      k_move wave, so the segment is sorted by ranking its runs: 16 lanes
-//      per cell for segments of up to 64 with up to 16 runs, 32 lanes for
-//      up to 128 with up to 32 runs, one wave per cell for the rest (longer
-//      segments sorted by run key), and a block bitonic sort above 64 runs;
-//      the result equals the serial order;
-//   2. gather of the state into cell order -- the outflow compaction of
+// larger tiles stage in the destination arrays in HBM).  The permutation
+// already lists every cell's particles in ascending index order (k_scatter
+// ranks the k_move runs of each cell), so per tile:
+//   1. gather of the state into cell order -- the outflow compaction of
 //      GPUParticles::DeleteInvalid (cudaDSMC GPUParticle.cu:76-140);
-//   3. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
+//   2. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
 //      host semantics of Cell::collision, Cell.cpp:335-409): candidate counts
 //      one lane per cell; cells with up to 64 candidates run in groups of 8
 //      lanes, heavier cells by a whole wave, both in conflict-free rounds
 //      whose results equal the serial candidate loop bit for bit;
-//   4. FP64 sampling sums, 8 lanes per cell joined by a fixed butterfly
+//   3. FP64 sampling sums, 8 lanes per cell joined by a fixed butterfly
 //      (samplingInCells, GPUCell.cu:336-402, as time-averaged accumulators);
-//   5. velocities and cell state written back coalesced.
+//   4. velocities and cell state written back coalesced.
 // The last block to finish advances the device step control.
 #include <type_traits>
 
@@ -42,62 +37,10 @@ struct CellShared {
     float sm[CELL_MAXC], mr[CELL_MAXC], vol[CELL_MAXC];
     int nc[CELL_MAXC], M[CELL_MAXC];
     int big[CELL_MAXC], heavy[CELL_MAXC];
-    int rstart[CELL_WAVES][64], rdst[CELL_WAVES][64];   // per-wave run tables
     int inst[CELL_MAXC];             // batched runs: instance of each staged cell
-    int nbig, nheavy, nrest;
+    int nbig, nheavy;
     uint32_t coll, cand;
 };
-
-// Block-wide ascending sort of p[0..k) (flip bitonic network: every
-// comparator is ascending, so indices >= k act as +inf without storage).
-template <bool GLOBAL>
-__device__ void block_sort(int *p, int k)
-{
-    int P2 = 1;
-    while (P2 < k) P2 <<= 1;
-    const int half = P2 >> 1;
-    for (int size = 2; size <= P2; size <<= 1) {
-        for (int j = size >> 1; j > 0; j >>= 1) {
-            for (int ci = threadIdx.x; ci < half; ci += blockDim.x) {
-                int a, b;
-                if (j == (size >> 1)) {
-                    int hs = size >> 1, blk = ci / hs, pos = ci - blk * hs;
-                    a = blk * size + pos;
-                    b = blk * size + size - 1 - pos;
-                } else {
-                    a = (ci / j) * 2 * j + (ci % j);
-                    b = a + j;
-                }
-                if (b < k) {
-                    int va = p[a], vb = p[b];
-                    if (vb < va) { p[a] = vb; p[b] = va; }
-                }
-            }
-            if (GLOBAL) __threadfence();
-            __syncthreads();
-        }
-    }
-}
-
-// Ascending bitonic sort of one int per lane across the 64 lanes (flip
-// formulation: every comparator is ascending, so +inf padding is free).
-__device__ __forceinline__ int wave_sort64(int v)
-{
-    const int lane = lane_id();
-#pragma unroll
-    for (int size = 2; size <= 64; size <<= 1) {
-        int partner = lane ^ (size - 1);
-        int o = __shfl(v, partner, 64);
-        v = lane < partner ? min(v, o) : max(v, o);
-#pragma unroll
-        for (int s = size >> 2; s > 0; s >>= 1) {
-            partner = lane ^ s;
-            o = __shfl(v, partner, 64);
-            v = lane < partner ? min(v, o) : max(v, o);
-        }
-    }
-    return v;
-}
 
 template <bool GLOBAL>
 __device__ __forceinline__ void wave_sync()
@@ -105,186 +48,6 @@ __device__ __forceinline__ void wave_sync()
     if (GLOBAL) __threadfence_block();
     else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}
-
-// Run starts of a cell segment S[0..len): element e starts a run when its
-// k_move wave (index >> 6) differs from that of e-1.  Calls f(e, run) for
-// every element, with run = index of its run, chunk by chunk.
-template <class F>
-__device__ __forceinline__ int for_runs(const int *S, int len, F f)
-{
-    const int lane = lane_id();
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    int nseen = 0, prevkey = -1;
-    for (int b = 0; b < len; b += 64) {
-        const int e = b + lane;
-        const int key = e < len ? (S[e] >> 6) : -2;
-        int pk = __shfl_up(key, 1, 64);
-        if (lane == 0) pk = prevkey;
-        const bool st = e < len && key != pk;
-        const unsigned long long m = __ballot(st);
-        if (e < len) f(e, nseen + __popcll(m & upto) - 1, st);
-        nseen += __popcll(m);
-        prevkey = __shfl(key, 63, 64);
-    }
-    return nseen;
-}
-
-// Sorts S[0..len) (len > 64) ascending with one wave.  The segment is a
-// concatenation of runs, one per k_move wave that binned into this cell;
-// each run is ascending and holds one value of (index >> 6).  Sorting the
-// runs by that key sorts the segment.  T is scratch of len ints; rstart and
-// rdst hold 64 ints.  Returns false (nothing moved) above 64 runs.
-template <bool GLOBAL>
-__device__ bool wave_run_sort(int *S, int *T, int len, int *rstart, int *rdst)
-{
-    const int lane = lane_id();
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    // 1. run starts
-    int nr = 0, prevkey = -1;
-    for (int b = 0; b < len; b += 64) {
-        const int e = b + lane;
-        const int key = e < len ? (S[e] >> 6) : -2;
-        int pk = __shfl_up(key, 1, 64);
-        if (lane == 0) pk = prevkey;
-        const bool st = e < len && key != pk;
-        const unsigned long long m = __ballot(st);
-        const int cnt = __popcll(m);
-        if (nr + cnt > 64) return false;
-        if (st) rstart[nr + __popcll(m & lt)] = e;
-        nr += cnt;
-        prevkey = __shfl(key, 63, 64);
-    }
-    if (nr <= 1) return true;
-    wave_sync<GLOBAL>();
-    // 2. order the runs by key; destinations by an exclusive scan of lengths
-    const int rs = lane < nr ? rstart[lane] : len;
-    int re = __shfl_down(rs, 1, 64);
-    if (lane + 1 >= nr) re = len;
-    const int rlen = lane < nr ? re - rs : 0;
-    const int packed = lane < nr ? (((S[rs] >> 6) << 6) | lane) : 0x7fffffff;
-    const int sorted = wave_sort64(packed);
-    const int rj = sorted & 63;
-    int lj = __shfl(rlen, rj, 64);
-    if (lane >= nr) lj = 0;
-    int inc = lj;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int t = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += t;
-    }
-    if (lane < nr) rdst[rj] = inc - lj;
-    wave_sync<GLOBAL>();
-    // 3. move every element to its run's destination, then copy back
-    for_runs(S, len, [&](int e, int r, bool) { T[rdst[r] + e - rstart[r]] = S[e]; });
-    wave_sync<GLOBAL>();
-    for (int e = lane; e < len; e += 64) S[e] = T[e];
-    wave_sync<GLOBAL>();
-    return true;
-}
-
-// Sorts a segment of len <= 64 by one wave using its k_move runs: already
-// sorted segments (one run) return at once; otherwise each run is ranked
-// against the others (few runs) and every element moves to
-// (sum of lengths of runs with smaller key) + (position in its run).
-template <bool GLOBAL>
-__device__ __forceinline__ void wave_small_sort(int *S, int len, int *rtab)
-{
-    const int lane = lane_id();
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const bool in = lane < len;
-    const int v = in ? S[lane] : 0;
-    const int pv = (in && lane > 0) ? S[lane - 1] : 0;
-    const bool st = in && (lane == 0 || (v >> 6) != (pv >> 6));
-    const unsigned long long m = __ballot(st);
-    const int nr = __popcll(m);
-    if (nr <= 1) return;
-    const int run = __popcll(m & upto) - 1;
-    if (st) rtab[run] = lane;
-    wave_sync<GLOBAL>();
-    const int rs = lane < nr ? rtab[lane] : len;
-    int re = __shfl_down(rs, 1, 64);
-    if (lane + 1 >= nr) re = len;
-    // every lane takes part in the shuffle (its source lane must be active)
-    const int sv = __shfl(v, rs & 63, 64);
-    const int rkey = lane < nr ? (sv >> 6) : 0x7fffffff;
-    const int rlen = lane < nr ? re - rs : 0;
-    int dst = 0;
-    for (int r = 0; r < nr; ++r) {
-        const int k2 = __shfl(rkey, r, 64), l2 = __shfl(rlen, r, 64);
-        if (k2 < rkey) dst += l2;
-    }
-    const int mydst = __shfl(dst, run, 64) + lane - __shfl(rs, run, 64);
-    wave_sync<GLOBAL>();
-    if (in) S[mydst] = v;
-    wave_sync<GLOBAL>();
-}
-
-// The same ranking for many cells at once: a wave takes 64/G cells, G lanes
-// per cell, lane `sub` holding elements sub, sub + G, ... of a segment of up
-// to E*G elements.  Runs are numbered by ballots in element order; run r's
-// destination is the total length of the runs with smaller key.  The cells
-// are [0, count) or, FROMLIST, cells[0 .. count).  Cells with longer segments
-// or more than G runs are appended to out[] (count *nout) for the next pass;
-// rt is the wave's 64-int table.
-template <int G, int E, bool FROMLIST, bool GLOBAL>
-__device__ void group_sort(int *S, const int *st, const int *cells, int count, int *rt, int *out, int *nout,
-                           int wave, int nwaves)
-{
-    static_assert(G <= 32 && (64 % G) == 0, "group width");
-    constexpr int CPW = 64 / G, CAP = E * G;
-    const int lane = lane_id(), g = lane / G, sub = lane % G, gsh = g * G;
-    const unsigned long long gbits = (1ull << G) - 1ull;
-    const unsigned long long upto = (2ull << sub) - 1ull;
-    for (int base = wave * CPW; base < count; base += nwaves * CPW) {
-        const int k = base + g;
-        const int i = k < count ? (FROMLIST ? cells[k] : k) : 0;
-        const int o = k < count ? st[i] : 0;
-        const int len = k < count ? st[i + 1] - o : 0;
-        const bool take = len >= 2 && len <= CAP;
-        int v[E], run[E];
-        int nr = 0;
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            const int e = j * G + sub;
-            const bool in = take && e < len;
-            v[j] = in ? S[o + e] : 0;
-            const int pv = (in && e > 0) ? S[o + e - 1] : 0;
-            const bool s = in && (e == 0 || (v[j] >> 6) != (pv >> 6));
-            const unsigned long long gb = (__ballot(s) >> gsh) & gbits;
-            run[j] = nr + __popcll(gb & upto) - 1;
-            if (s && run[j] < G) rt[gsh + run[j]] = e;
-            nr += __popcll(gb);
-        }
-        if (sub == 0 && (len > CAP || (take && nr > G))) out[atomicAdd(nout, 1)] = i;
-        const bool work = take && nr > 1 && nr <= G;
-        if (!__ballot(work)) continue;
-        wave_sync<GLOBAL>();
-        const bool hr = work && sub < nr;
-        const int rs = hr ? rt[gsh + sub] : len;
-        int re = __shfl_down(rs, 1, G);
-        if (sub + 1 >= nr) re = len;
-        const int key = hr ? (S[o + rs] >> 6) : 0x7fffffff;
-        const int rl = hr ? re - rs : 0;
-        int dst = 0;
-        const int nrw = work ? nr : 0;
-        for (int r = 0; __ballot(r < nrw); ++r) {
-            const int k2 = __shfl(key, r, G), l2 = __shfl(rl, r, G);
-            if (k2 < key) dst += l2;
-        }
-        int d[E];
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            const int rr = min(max(run[j], 0), G - 1);
-            d[j] = __shfl(dst, rr, G) + j * G + sub - __shfl(rs, rr, G);
-        }
-        wave_sync<GLOBAL>();
-        if (work)
-#pragma unroll
-            for (int j = 0; j < E; ++j)
-                if (j * G + sub < len) S[o + d[j]] = v[j];
-        wave_sync<GLOBAL>();
-    }
 }
 
 // Collision constants of one cell: VHS sigma*c_r = K1 (c_r^2)^e_exp, the
@@ -547,7 +310,6 @@ struct CellArgs {
     double *acc;
     Ctrl *ctrl;
     int sample_start;
-    int keep_perm;
     // batched runs only
     const CellConst *cc;     // ninst
     const int *cbase;        // ninst + 1: first global cell of each instance
@@ -626,66 +388,24 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
     const int nwaves = bs >> 6;
     const DevCase &cs = a.cs;
-    // a tile of one chunk loads its permutation, cell starts and collision
-    // state in one round trip (the metadata stays staged until the
-    // collisions); larger tiles stage each chunk's starts for the ordering
-    // and everything again for the collisions
+    // a tile of one chunk of cells loads its permutation, cell starts and
+    // collision state in one round trip (the metadata stays staged until the
+    // collisions); larger tiles stage each chunk's metadata at the collisions
     const bool one = cb - ca <= CELL_MAXC;
     if (LDS)
         for (int k = tid; k < n; k += bs) Sp[k] = a.perm[p0 + k];
-
-    // 1. stable order within each cell, one wave per cell: up to 64 particles
-    //    sorted in registers, longer segments by their k_move runs; more than
-    //    64 runs fall back to the block network.  The velocity staging array
-    //    is free until the gather and serves as scratch.
-    int *R = reinterpret_cast<int *>(Vx);
-    for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
-        const int nc = min(cb - c0, CELL_MAXC);
-        for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[c0 + i] - p0;
-        if (one)
-            for (int i = tid; i < nc; i += bs) {
-                sh.sm[i] = a.sigcmax[c0 + i];
-                sh.mr[i] = a.mrem[c0 + i];
-                sh.vol[i] = cs.vol_ntc[c0 + i];
-            }
-        if (tid == 0) { sh.nbig = 0; sh.nheavy = 0; sh.nrest = 0; }
-        __syncthreads();
-        // segments up to 64 with up to 16 runs: 16-lane groups; up to 128
-        // with up to 32 runs: 32-lane groups; the rest one wave per cell
-        group_sort<16, 4, false, !LDS>(Sp, sh.st, nullptr, nc, sh.rstart[wave], sh.heavy, &sh.nheavy, wave,
-                                       nwaves);
-        if (!LDS) __threadfence();
-        __syncthreads();
-        const int n2 = sh.nheavy;
-        if (n2 > 0) {
-            group_sort<32, 4, true, !LDS>(Sp, sh.st, sh.heavy, n2, sh.rstart[wave], sh.M, &sh.nrest, wave, nwaves);
-            if (!LDS) __threadfence();
-            __syncthreads();
+    if (one) {
+        const int nc = cb - ca;
+        for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[ca + i] - p0;
+        for (int i = tid; i < nc; i += bs) {
+            sh.sm[i] = a.sigcmax[ca + i];
+            sh.mr[i] = a.mrem[ca + i];
+            sh.vol[i] = cs.vol_ntc[ca + i];
         }
-        const int nl = sh.nrest;
-        for (int h = wave; h < nl; h += nwaves) {
-            const int i = sh.M[h];
-            const int o = sh.st[i], len = sh.st[i + 1] - o;
-            if (len <= 1) continue;
-            if (len <= 64) {
-                wave_small_sort<!LDS>(Sp + o, len, sh.rstart[wave]);
-            } else if (!wave_run_sort<!LDS>(Sp + o, R + o, len, sh.rstart[wave], sh.rdst[wave])) {
-                if (lane == 0) sh.big[atomicAdd(&sh.nbig, 1)] = i;
-            }
-        }
-        if (!LDS) __threadfence();
-        __syncthreads();
-        const int nb = sh.nbig;
-        for (int b = 0; b < nb; ++b) {
-            const int i = sh.big[b];
-            block_sort<!LDS>(Sp + sh.st[i], sh.st[i + 1] - sh.st[i]);
-        }
-        __syncthreads();
     }
-    if (LDS && a.keep_perm)
-        for (int k = tid; k < n; k += bs) a.perm[p0 + k] = Sp[k];
+    __syncthreads();
 
-    // 2. gather into cell order, GATHER_UNROLL particles per thread in flight
+    // 1. gather into cell order, GATHER_UNROLL particles per thread in flight
     {
         const float *__restrict__ cx = cur.x, *__restrict__ cy = cur.y;
         const float *__restrict__ cvx = cur.vx, *__restrict__ cvy = cur.vy, *__restrict__ cvz = cur.vz;
@@ -723,7 +443,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     if (!LDS) __threadfence();
     __syncthreads();
 
-    // 3-4. collisions and sampling, chunk by chunk of cells
+    // 2-3. collisions and sampling, chunk by chunk of cells
     for (int c0 = ca; c0 < cb; c0 += CELL_MAXC) {
         const int nc = min(cb - c0, CELL_MAXC);
         if (!one) {
@@ -838,7 +558,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
         __syncthreads();
     }
 
-    // 5. velocities back to HBM
+    // 4. velocities back to HBM
     if (LDS)
         for (int k = tid; k < n; k += bs) {
             nxt.vx[p0 + k] = Vx[k];
@@ -1009,12 +729,11 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
 
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
                        const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
-                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int tile_p, int grid,
-                       hipStream_t st)
+                       double *acc, Ctrl *ctrl, int sample_start, int tile_p, int grid, hipStream_t st)
 {
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     CellArgs a{ cs, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
-                keep_perm, nullptr, nullptr, 0, nullptr, tile_p };
+                nullptr, nullptr, 0, nullptr, tile_p };
     if (cs.zeta != 0) {
         if (!A.er || !B.er) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_cell<false, true>), dim3(grid), dim3(256), 0, st, a);
@@ -1031,7 +750,7 @@ hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int 
 {
     if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
-    CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, 0, cc, cbase, ninst,
+    CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, cc, cbase, ninst,
                 ictl, tile_p };
     hipLaunchKernelGGL((k_cell<true, false>), dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();

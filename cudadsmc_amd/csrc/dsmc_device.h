@@ -60,8 +60,25 @@ struct Ctrl {
     uint32_t step, n_live, n_move, n_next;
     uint32_t next_pid, err, scan_done, cell_done;
     uint32_t step_coll, last_coll, step_cand, last_cand;
-    uint32_t tile_next, pad0;      // k_cell work queue
+    uint32_t tile_next;            // k_cell work queue
+    uint32_t n_runs;               // (cell, k_move wave) runs of the last binning
     unsigned long long psteps, ncoll, nsamples, ncand;
+    unsigned long long nruns;      // runs binned over all steps (algorithmic byte counts)
+};
+
+// Binning of one step.  k_move groups the lanes of each wave (64 consecutive
+// particles, a "chunk") by cell: each group is a run, counted with one 64-bit
+// atomic (runs << 32 | particles) per cell and recorded, compacted per chunk,
+// as (cell, its run slot in the cell, lane mask).  One scan covers both
+// counts; k_place lists every cell's runs contiguously, and k_scatter ranks
+// a cell's runs by chunk, which gives the stable (ascending index) order.
+struct Bins {
+    unsigned long long *cnt;       // per cell, zeroed again by the scan
+    unsigned long long *lstart;    // per cell: exclusive scan of cnt within its scan tile
+    unsigned long long *tsum, *toff;   // per scan tile: total, and exclusive scan of the totals
+    int4 *runrec;                  // chunk q's runs at [64q, 64q + chunk_runs[q]): cell, slot, mask lo, hi
+    int *chunk_runs;               // runs per chunk
+    int4 *runlist;                 // runs in cell order: run id (64q + k), unused, mask lo, hi
 };
 
 // Batched runs: every instance keeps its own counters (the global Ctrl holds

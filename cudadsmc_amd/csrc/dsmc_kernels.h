@@ -15,21 +15,24 @@ constexpr int CELL_BIG = 64;                 // cells with > 64 particles listed
 constexpr int CELL_WAVES = 4;                // waves per k_cell block
 constexpr int SAMPLE_G = 8;                  // lanes per cell in the FP64 sampling sums (k_cell, k_sample)
 
-hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, int2 *cellslot, int *count,
-                       Ctrl *ctrl, int grid, hipStream_t st);
+// The step's binning: k_move records runs into bn; launch_scan_runs scans
+// the packed counts; launch_scatter runs k_place and k_scatter (perm, start,
+// tile_first).
+hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, const Bins &bn, Ctrl *ctrl, int grid,
+                       hipStream_t st);
 // Many instances in one launch: cases[ninst] and cbase[ninst+1] in device
 // memory, per-instance counters in ictl[ninst], totals in ctrl.
 hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, const SoA &A, const SoA &B,
-                             int2 *cellslot, int *count, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st);
+                             const Bins &bn, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st);
+hipError_t launch_scan_runs(const Bins &bn, int ncell, Ctrl *ctrl, hipStream_t st);
+hipError_t launch_scatter(const Bins &bn, int *start, int *perm, int *tile_first, int ncell, int tile_p,
+                          const Ctrl *ctrl, int grid, hipStream_t st);
+// Plain exclusive scan of int counts (the fill and dsmc_bin_sort).
 hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell, Ctrl *ctrl,
                        hipStream_t st);
-hipError_t launch_scatter(const int2 *cellslot, const int *lstart, const int *toff, int *start,
-                          int *perm, int *tile_first, int ncell, int tile_p, const Ctrl *ctrl, int grid,
-                          hipStream_t st);
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
                        const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
-                       double *acc, Ctrl *ctrl, int sample_start, int keep_perm, int tile_p, int grid,
-                       hipStream_t st);
+                       double *acc, Ctrl *ctrl, int sample_start, int tile_p, int grid, hipStream_t st);
 hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
                              InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
                              const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,

@@ -1,18 +1,20 @@
 // HIP kernels of the DSMC hot path for gfx950 (wave64, 160 KB LDS per CU).
 //
-// One timestep = four launches, no host round trip:
-//   k_move    free flight + walls + body + outflow, inflow creation and the
-//             atomic per-cell count                       (cudaDSMC GPUParticle.cu:229-363,
-//                                                          :375-422, GPUCell.cu:169-198)
-//   k_scan    exclusive scan of the counts, two-level, last-block finish
-//                                                         (GPUCell.cu:104-120)
-//   k_scatter cell-sorted permutation + start table + tile boundaries
-//                                                         (GPUParticle.cu:365-373)
-//   k_cell    per tile of cells: stable (ascending index) order inside each
-//             cell, gather into cell order (this is also the outflow
-//             compaction), NTC/VHS collisions, FP64 sampling
-//                                                         (GPUCell.cu:200-402,
-//                                                          GPUParticle.cu:76-140)
+// One timestep = five launches, no host round trip:
+//   k_move       free flight + walls + body + outflow, inflow creation, and
+//                the binning: per wave (64 consecutive particles) one run per
+//                cell, one packed atomic per run  (cudaDSMC GPUParticle.cu:229-363,
+//                                                   :375-422, GPUCell.cu:169-198)
+//   k_scan_runs  exclusive scan of the per-cell particle and run counts,
+//                two-level, last-block finish      (GPUCell.cu:104-120)
+//   k_place      every cell's runs listed contiguously
+//   k_scatter    per cell, runs ranked by wave: the cell-sorted permutation
+//                in ascending index order (the stable order), the start
+//                table and the k_cell tile boundaries (GPUParticle.cu:365-373)
+//   k_cell       per tile of cells: gather into cell order (this is also the
+//                outflow compaction), NTC/VHS collisions, FP64 sampling
+//                                                  (GPUCell.cu:200-402,
+//                                                   GPUParticle.cu:76-140)
 // All launches have fixed grids (persistent, grid-stride over the
 // device-resident particle count), so a step can be captured in a hipGraph.
 #include <algorithm>
@@ -46,37 +48,41 @@ __device__ __forceinline__ int wave_sort64(int v)
     return v;
 }
 
-// Per-cell rank by one atomicAdd per distinct cell of the wave (the state is
-// cell-sorted, so a wave touches only a few cells): lanes sharing a cell get
-// consecutive slots in lane order, so each (wave, cell) group is one
-// ascending run of the cell's segment -- k_cell's sort relies on this.  The
-// groups are found in registers first and all their atomics go out together
-// (one memory round trip per wave).  Lanes with c < 0 take no slot.
-__device__ __forceinline__ int agg_slot(int *__restrict__ count, int c)
+// Binning of one wave, which holds the 64 consecutive particles of chunk
+// i >> 6 (lane = i & 63): the lanes are grouped by cell, each group being one
+// run of its cell (the state is cell-sorted, so a wave touches only a few
+// cells).  The lowest lane of a group adds (1 run, group size) to the cell's
+// packed counter -- one atomic per run, all of the wave's going out together
+// -- and records (cell, the slot the atomic returned, lane mask) as the
+// chunk's k-th run, k counting the groups in lane order; lane 0 stores the
+// chunk's run count.  Lanes with c < 0 (left the domain) are in no run.
+__device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i)
 {
     const int lane = lane_id();
     const unsigned long long lt = (1ull << lane) - 1ull;
     unsigned long long todo = __ballot(c >= 0);
-    int leader = lane, rank = 0, cnt = 0;
+    unsigned long long mine = 0ull;
     while (todo) {
         const int l = (int)__ffsll((long long)todo) - 1;
         const int lc = __shfl(c, l, 64);
         const unsigned long long m = __ballot(c == lc);
-        if (c == lc) { leader = l; rank = __popcll(m & lt); }
-        if (lane == l) cnt = __popcll(m);
+        if (lane == l) mine = m;
         todo &= ~m;
     }
-    int base = 0;
-    if (c >= 0 && lane == leader) base = atomicAdd(&count[c], cnt);
-    base = __shfl(base, leader, 64);
-    return c >= 0 ? base + rank : 0;
+    const unsigned long long leaders = __ballot(mine != 0ull);
+    const uint32_t q = i >> 6;
+    if (mine) {
+        const unsigned long long old = atomicAdd(&bn.cnt[c], (1ull << 32) | (unsigned long long)__popcll(mine));
+        bn.runrec[(size_t)q * 64 + __popcll(leaders & lt)] =
+            make_int4(c, (int)(uint32_t)(old >> 32), (int)(uint32_t)mine, (int)(uint32_t)(mine >> 32));
+    }
+    if (lane == 0) bn.chunk_runs[q] = __popcll(leaders);
 }
 
 // ---------------------------------------------------------------------------
 // k_move
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, int2 *__restrict__ cellslot,
-                                              int *__restrict__ count, Ctrl *__restrict__ ctrl)
+__global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, Bins bn, Ctrl *__restrict__ ctrl)
 {
     const uint32_t step = ctrl->step;
     const SoA S = (step & 1u) ? B : A;
@@ -115,8 +121,7 @@ __global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, int2 *__
             S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
             if (rot) S.er[i] = er;
         }
-        const int c = dead ? -1 : cell_of(cs, x, y);
-        cellslot[i] = make_int2(c, agg_slot(count, c));
+        agg_run(bn, dead ? -1 : cell_of(cs, x, y), i);
     }
     if (err) atomicOr(&ctrl->err, err);
 }
@@ -205,36 +210,168 @@ __global__ __launch_bounds__(256) void k_scan(int *__restrict__ count, int *__re
 }
 
 // ---------------------------------------------------------------------------
-// k_scatter: perm[start(cell)+slot] = i, materialise start[0..C], and record
-// for every tile of tile_p particles the first cell that starts in it and that
-// cell's start (tile_first holds int2 pairs).
+// k_scan_runs: k_scan over the packed per-cell counters (runs << 32 |
+// particles).  Neither half can carry into the other -- both totals stay
+// below 2^32 -- so one 64-bit scan gives the particle starts in the low half
+// and the run starts in the high half.  cnt is zeroed for the next step.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scatter(const int2 *__restrict__ cellslot,
-                                                 const int *__restrict__ lstart,
-                                                 const int *__restrict__ toff, int *__restrict__ start,
-                                                 int *__restrict__ perm, int *__restrict__ tile_first,
-                                                 int ncell, int tile_p, const Ctrl *__restrict__ ctrl)
+__device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long long v, unsigned long long &excl,
+                                                                unsigned long long *s_w)
 {
-    const uint32_t n_move = ctrl->n_move;
-    const int n_next = (int)ctrl->n_next;
-    const int ntiles = (n_next + tile_p - 1) / tile_p;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t i = gid; i < n_move; i += stride) {
-        int2 cs = cellslot[i];
-        if (cs.x >= 0) perm[lstart[cs.x] + toff[cs.x / SCAN_TILE] + cs.y] = (int)i;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    unsigned long long inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
     }
-    for (uint32_t c = gid; c <= (uint32_t)ncell; c += stride) {
-        int st = (int)c < ncell ? lstart[c] + toff[c / SCAN_TILE] : n_next;
-        start[c] = st;
-        int tlo = 0;
-        if (c > 0) {
-            int prev = lstart[c - 1] + toff[(c - 1) / SCAN_TILE];
-            tlo = prev / tile_p + 1;
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    unsigned long long wofs = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned long long t = s_w[k];
+        if (k < w) wofs += t;
+        total += t;
+    }
+    excl = wofs + inc - v;
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_runs(Bins bn, int ncell, int ntile, Ctrl *__restrict__ ctrl)
+{
+    __shared__ unsigned long long s_w[4];
+    __shared__ int s_last;
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_IPT;
+    unsigned long long v[SCAN_IPT];
+    unsigned long long run = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k) {
+        const int c = base + k;
+        v[k] = c < ncell ? bn.cnt[c] : 0ull;
+        run += v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k)
+        if (base + k < ncell) bn.cnt[base + k] = 0ull;
+    unsigned long long excl;
+    const unsigned long long total = block_excl_scan64(run, excl, s_w);
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; ++k) {
+        const int c = base + k;
+        if (c < ncell) bn.lstart[c] = excl;
+        excl += v[k];
+    }
+    if (threadIdx.x == 0) bn.tsum[blockIdx.x] = total;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(&ctrl->scan_done, 1u) == (uint32_t)(ntile - 1));
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    unsigned long long carry = 0;
+    for (int b0 = 0; b0 < ntile; b0 += 256) {
+        const int b = b0 + (int)threadIdx.x;
+        const unsigned long long t =
+            b < ntile ? __hip_atomic_load(&bn.tsum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        unsigned long long e;
+        const unsigned long long tot = block_excl_scan64(t, e, s_w);
+        if (b < ntile) bn.toff[b] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        ctrl->n_next = (uint32_t)carry;
+        ctrl->n_runs = (uint32_t)(carry >> 32);
+        ctrl->nruns += carry >> 32;
+        ctrl->scan_done = 0u;
+    }
+}
+
+// Packed starts of cell c (particles low, runs high); c == ncell gives the
+// totals.
+__device__ __forceinline__ unsigned long long packed_start(const Bins &bn, int c, int ncell, uint32_t n_next,
+                                                           uint32_t n_runs)
+{
+    return c < ncell ? bn.lstart[c] + bn.toff[c / SCAN_TILE]
+                     : (((unsigned long long)n_runs << 32) | (unsigned long long)n_next);
+}
+
+// ---------------------------------------------------------------------------
+// k_place: every cell's runs listed contiguously -- record k of chunk q goes
+// to run start(cell) + its slot.  One thread per record position.  Within a
+// cell the list is in the atomics' arrival order, which k_scatter ranks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__ ctrl)
+{
+    const uint32_t nrec = (ctrl->n_move + 63u) & ~63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += stride) {
+        if ((int)(i & 63u) >= bn.chunk_runs[i >> 6]) continue;
+        const int4 r = bn.runrec[i];
+        const unsigned long long v = bn.lstart[r.x] + bn.toff[r.x / SCAN_TILE];
+        bn.runlist[(uint32_t)(v >> 32) + (uint32_t)r.y] = make_int4((int)i, 0, r.z, r.w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter: SC_G lanes per cell.  A run's id is 64q + k for the k-th run of
+// chunk q; the runs of one cell lie in distinct chunks, and chunk q holds
+// particles [64q, 64q + 64), so ordering a cell's runs by id orders them by
+// particle index.  Run j's members go to start(c) + (the lengths of the
+// cell's runs with smaller id) + (their rank in the mask): perm lists each
+// cell's particles in ascending index order -- the order of the serial
+// binning, bit for bit.  Also materialises start[0..C] and, for every tile
+// of tile_p particles, the first cell that starts in it and that cell's start
+// (tile_first holds int2 pairs).
+// ---------------------------------------------------------------------------
+constexpr int SC_G = 16;
+
+__global__ __launch_bounds__(256) void k_scatter(Bins bn, int *__restrict__ start, int *__restrict__ perm,
+                                                 int *__restrict__ tile_first, int ncell, int tile_p,
+                                                 const Ctrl *__restrict__ ctrl)
+{
+    constexpr int CPW = 64 / SC_G;
+    const int lane = lane_id(), g = lane / SC_G, sub = lane % SC_G;
+    const uint32_t n_next = ctrl->n_next, n_runs = ctrl->n_runs;
+    const int ntiles = ((int)n_next + tile_p - 1) / tile_p;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int base = wave * CPW; base <= ncell; base += waves * CPW) {
+        const int c = base + g;
+        const unsigned long long v0 = c <= ncell ? packed_start(bn, c, ncell, n_next, n_runs) : 0ull;
+        const unsigned long long v1 = c < ncell ? packed_start(bn, c + 1, ncell, n_next, n_runs) : v0;
+        const int st = (int)(uint32_t)v0, rs = (int)(v0 >> 32);
+        const int L = (int)(v1 >> 32) - rs;
+        if (sub == 0 && c <= ncell) {
+            start[c] = st;
+            const int tlo = c > 0 ? (int)(uint32_t)packed_start(bn, c - 1, ncell, n_next, n_runs) / tile_p + 1 : 0;
+            const int thi = min(st / tile_p, ntiles - 1);
+            for (int t = tlo; t <= thi; ++t) reinterpret_cast<int2 *>(tile_first)[t] = make_int2(c, st);
         }
-        int thi = st / tile_p;
-        if (thi > ntiles - 1) thi = ntiles - 1;
-        for (int t = tlo; t <= thi; ++t) reinterpret_cast<int2 *>(tile_first)[t] = make_int2((int)c, st);
+        // wave-uniform passes (every lane takes part in the shuffles)
+        for (int b0 = 0; __ballot(b0 < L) != 0ull; b0 += SC_G) {
+            const bool have = b0 + sub < L;
+            const int4 e = have ? bn.runlist[rs + b0 + sub] : make_int4(0x7fffffff, 0, 0, 0);
+            const unsigned long long mask = ((unsigned long long)(uint32_t)e.w << 32) | (uint32_t)e.z;
+            int off = 0;
+            for (int p2 = 0; __ballot(p2 < L) != 0ull; p2 += SC_G) {
+                int4 f = e;
+                if (p2 != b0) f = p2 + sub < L ? bn.runlist[rs + p2 + sub] : make_int4(0x7fffffff, 0, 0, 0);
+                const int lk = __popc((uint32_t)f.z) + __popc((uint32_t)f.w);
+#pragma unroll
+                for (int t = 0; t < SC_G; ++t) {
+                    const int r2 = __shfl(f.x, t, SC_G), l2 = __shfl(lk, t, SC_G);
+                    if (r2 < e.x) off += l2;
+                }
+            }
+            if (have) {
+                const int q0 = e.x & ~63;
+                int *out = perm + st + off;
+                unsigned long long m = mask;
+                for (int r = 0; m; ++r, m &= m - 1ull) out[r] = q0 + (int)__builtin_ctzll(m);
+            }
+        }
     }
 }
 
@@ -435,8 +572,7 @@ __device__ __forceinline__ int find_inst(const uint32_t *tab, int ninst, uint32_
 
 __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ cases, int ninst,
                                                     const int *__restrict__ cbase, SoA A, SoA B,
-                                                    int2 *__restrict__ cellslot, int *__restrict__ count,
-                                                    Ctrl *__restrict__ ctrl, InstCtrl *__restrict__ ictl)
+                                                    Bins bn, Ctrl *__restrict__ ctrl, InstCtrl *__restrict__ ictl)
 {
     __shared__ uint32_t s_p[MAX_INST + 1], s_q[MAX_INST + 1];
     __shared__ int s_w[4];
@@ -502,17 +638,17 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
             c = dead ? -1 : cbase[u] + cell_of(cs, x, y);
             if (err) { atomicOr(&ictl[u].err, err); err = 0; }
         }
-        cellslot[i] = make_int2(c, agg_slot(count, c));
+        agg_run(bn, c, i);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, int2 *cellslot, int *count,
-                       Ctrl *ctrl, int grid, hipStream_t st)
+hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, const Bins &bn, Ctrl *ctrl, int grid,
+                       hipStream_t st)
 {
-    hipLaunchKernelGGL(k_move, dim3(grid), dim3(256), 0, st, cs, A, B, cellslot, count, ctrl);
+    hipLaunchKernelGGL(k_move, dim3(grid), dim3(256), 0, st, cs, A, B, bn, ctrl);
     return hipGetLastError();
 }
 
@@ -525,13 +661,21 @@ hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell,
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const int2 *cellslot, const int *lstart, const int *toff, int *start,
-                          int *perm, int *tile_first, int ncell, int tile_p, const Ctrl *ctrl, int grid,
-                          hipStream_t st)
+hipError_t launch_scan_runs(const Bins &bn, int ncell, Ctrl *ctrl, hipStream_t st)
+{
+    const int ntile = (ncell + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_runs, dim3(ntile), dim3(256), 0, st, bn, ncell, ntile, ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Bins &bn, int *start, int *perm, int *tile_first, int ncell, int tile_p,
+                          const Ctrl *ctrl, int grid, hipStream_t st)
 {
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, cellslot, lstart, toff, start, perm,
-                       tile_first, ncell, tile_p, ctrl);
+    hipLaunchKernelGGL(k_place, dim3(grid), dim3(256), 0, st, bn, ctrl);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, bn, start, perm, tile_first, ncell, tile_p, ctrl);
     return hipGetLastError();
 }
 
@@ -552,11 +696,10 @@ hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, 
 }
 
 hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, const SoA &A, const SoA &B,
-                             int2 *cellslot, int *count, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st)
+                             const Bins &bn, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st)
 {
     if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_move_batch, dim3(grid), dim3(256), 0, st, cases, ninst, cbase, A, B, cellslot, count,
-                       ctrl, ictl);
+    hipLaunchKernelGGL(k_move_batch, dim3(grid), dim3(256), 0, st, cases, ninst, cbase, A, B, bn, ctrl, ictl);
     return hipGetLastError();
 }
 

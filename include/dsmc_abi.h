@@ -208,10 +208,12 @@ int dsmc_run(dsmc_handle h, int nsteps, int graph);
 int dsmc_sync(dsmc_handle h);
 int dsmc_status_get(dsmc_handle h, dsmc_status *out);
 
-/* Intermediate buffers for parity checks.  cell/slot are valid after
- * dsmc_move for the n_move processed particles (cell -1 = deleted);
- * count after dsmc_move; start[ncell+1] after dsmc_reassign; perm after
- * dsmc_collide when keep_perm is set (ascending particle index per cell). */
+/* Intermediate buffers for parity checks.  cell is valid after dsmc_move
+ * for the n_move processed particles (cell -1 = deleted); count after
+ * dsmc_move; start[ncell+1] and perm (ascending particle index per cell)
+ * after dsmc_reassign, perm until the next dsmc_reassign.  perm is always
+ * kept: dsmc_set_keep_perm is accepted for callers that set it and changes
+ * nothing. */
 int dsmc_set_keep_perm(dsmc_handle h, int on);
 /* The n_move particles processed by the last dsmc_move (moved + inflow, in
  * processing order, deleted ones included); valid until dsmc_collide. */
