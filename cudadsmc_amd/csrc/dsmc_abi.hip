@@ -268,7 +268,8 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
     Bins &bn = h->bn;
     if ((e = h->alloc(&bn.cnt, C)) || (e = h->alloc(&bn.lstart, C)) || (e = h->alloc(&bn.tsum, h->ntile_scan)) ||
         (e = h->alloc(&bn.toff, h->ntile_scan)) || (e = h->alloc(&bn.runrec, NR)) ||
-        (e = h->alloc(&bn.chunk_runs, NR / 64)) || (e = h->alloc(&bn.runlist, N)))
+        (e = h->alloc(&bn.chunk_runs, NR / 64)) ||
+        (e = h->alloc(&bn.runlist, N)))
         return bad(e, "binning buffers");
     if ((e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
         (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||

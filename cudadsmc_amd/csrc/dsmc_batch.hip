@@ -304,7 +304,8 @@ int setup_device(dsmc_batch b, dsmc_batch *out)
     Bins &bn = b->bn;
     if ((e = b->alloc(&bn.cnt, Cs)) || (e = b->alloc(&bn.lstart, Cs)) || (e = b->alloc(&bn.tsum, b->ntile_scan)) ||
         (e = b->alloc(&bn.toff, b->ntile_scan)) || (e = b->alloc(&bn.runrec, NR)) ||
-        (e = b->alloc(&bn.chunk_runs, NR / 64)) || (e = b->alloc(&bn.runlist, N)))
+        (e = b->alloc(&bn.chunk_runs, NR / 64)) ||
+        (e = b->alloc(&bn.runlist, N)))
         return bad(e, "binning buffers");
     if ((e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
         (e = b->alloc(&b->lstart, Cs)) || (e = b->alloc(&b->tsum, b->ntile_scan)) ||

@@ -420,10 +420,16 @@ __device__ __forceinline__ bool inject_one(const DevCase &cs, int f, uint32_t pi
     if (!ok) err |= ERR_INFLOW;
     float vn = wn * cs.inj_vmp;
     vz = cs.inj_sig * g1;
-    if (f == 0)      { x = 0.0f;         y = upos * cs.L2; vx = vn;  vy = cs.inj_uy + cs.inj_sig * g0; }
-    else if (f == 1) { x = cs.L1;        y = upos * cs.L2; vx = -vn; vy = cs.inj_uy + cs.inj_sig * g0; }
-    else if (f == 2) { x = upos * cs.L1; y = 0.0f;         vy = vn;  vx = cs.inj_ux + cs.inj_sig * g0; }
-    else             { x = upos * cs.L1; y = cs.L2;        vy = -vn; vx = cs.inj_ux + cs.inj_sig * g0; }
+    // faces 0, 1 (x = 0, L1): normal along x; faces 2, 3 (y = 0, L2): along y;
+    // odd faces point the normal speed inward as -vn (selects, not branches)
+    const bool xf = f < 2;
+    const float vin = (f & 1) ? -vn : vn;
+    const float vtan = (xf ? cs.inj_uy : cs.inj_ux) + cs.inj_sig * g0;
+    const float wall = (f & 1) ? (xf ? cs.L1 : cs.L2) : 0.0f;
+    x = xf ? wall : upos * cs.L1;
+    y = xf ? upos * cs.L2 : wall;
+    vx = xf ? vin : vtan;
+    vy = xf ? vtan : vin;
     // rotational energy of the freestream (GPUParticle.cu:409 draws it and
     // leaves it unstored, D13)
     er = 0.0f;

@@ -53,7 +53,7 @@ __device__ __forceinline__ int wave_sort64(int v)
 // run of its cell (the state is cell-sorted, so a wave touches only a few
 // cells).  The lowest lane of a group adds (1 run, group size) to the cell's
 // packed counter -- one atomic per run, all of the wave's going out together
-// -- and records (cell, the slot the atomic returned, lane mask) as the
+// -- and records (cell, the run slot the atomic returned, lane mask) as the
 // chunk's k-th run, k counting the groups in lane order; lane 0 stores the
 // chunk's run count.  Lanes with c < 0 (left the domain) are in no run.
 __device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i)
@@ -79,6 +79,15 @@ __device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i)
     if (lane == 0) bn.chunk_runs[q] = __popcll(leaders);
 }
 
+// The face of the q-th inflow particle of a step (the faces' particles
+// follow one another, J[f] of face f): the number of faces whose cumulative
+// count is <= q.
+__device__ __forceinline__ int inflow_face(const int J[4], uint32_t q)
+{
+    const uint32_t a0 = (uint32_t)J[0], a1 = a0 + (uint32_t)J[1], a2 = a1 + (uint32_t)J[2];
+    return (q >= a0 ? 1 : 0) + (q >= a1 ? 1 : 0) + (q >= a2 ? 1 : 0);
+}
+
 // ---------------------------------------------------------------------------
 // k_move
 // ---------------------------------------------------------------------------
@@ -101,23 +110,26 @@ __global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, Bins bn,
     uint32_t err = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool rot = cs.zeta != 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntot; i += stride) {
-        float x, y, vx, vy, vz, er = 0.0f;
+    // the live state of a thread's next particle is loaded one iteration
+    // ahead, so those loads are in flight while the current particle moves
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
+    if (i < n_live) { px = S.x[i]; py = S.y[i]; pvx = S.vx[i]; pvy = S.vy[i]; pvz = S.vz[i]; }
+    for (; i < ntot; i += stride) {
+        float x = px, y = py, vx = pvx, vy = pvy, vz = pvz, er = 0.0f;
+        const uint32_t j = i + stride;
+        if (j < n_live) { px = S.x[j]; py = S.y[j]; pvx = S.vx[j]; pvy = S.vy[j]; pvz = S.vz[j]; }
         bool dead;
         if (i < n_live) {
-            x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
             bool vchg = false, echg = false;
             dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
             S.x[i] = x; S.y[i] = y;
             if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
             if (echg) S.er[i] = er;     // re-drawn at a diffuse wall (zeta = 2 only)
         } else {
-            uint32_t q = i - n_live;
-            int f = 0;
-            uint32_t acc = (uint32_t)J[0];
-            while (f < 3 && q >= acc) { ++f; acc += (uint32_t)J[f]; }
-            uint32_t pid = next_pid + q;
-            dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, er, err);
+            const uint32_t q = i - n_live;
+            const uint32_t pid = next_pid + q;
+            dead = inject_one(cs, inflow_face(J, q), pid, step, x, y, vx, vy, vz, er, err);
             S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
             if (rot) S.er[i] = er;
         }
@@ -297,21 +309,92 @@ __device__ __forceinline__ unsigned long long packed_start(const Bins &bn, int c
                      : (((unsigned long long)n_runs << 32) | (unsigned long long)n_next);
 }
 
+// XCD-aware split of `ntask` wave tasks: workgroups are dispatched to the
+// eight XCDs in turn (block b on XCD b % 8), each XCD with its own L2, so XCD
+// x takes the x-th eighth of the tasks and its waves stride through that
+// range -- neighbouring tasks (adjacent cells, and the runs that land in
+// them) then share one L2.  Sets [first, end) and the stride; a wave
+// starts at first and steps by stride while below end.
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ void xcd_range(int ntask, int &first, int &end, int &stride)
+{
+    const int nx = min(NXCD, (int)gridDim.x);                    // every part has a block
+    const int x = (int)blockIdx.x % nx, lb = (int)blockIdx.x / nx;
+    const int nb = ((int)gridDim.x - x + nx - 1) / nx;          // blocks on part x
+    const int wpb = (int)blockDim.x >> 6;
+    const int lo = (int)((long long)ntask * x / nx), hi = (int)((long long)ntask * (x + 1) / nx);
+    first = lo + lb * wpb + ((int)threadIdx.x >> 6);
+    end = hi;
+    stride = nb * wpb;
+}
+
 // ---------------------------------------------------------------------------
 // k_place: every cell's runs listed contiguously -- record k of chunk q goes
-// to run start(cell) + its slot.  One thread per record position.  Within a
-// cell the list is in the atomics' arrival order, which k_scatter ranks.
+// to run start(cell) + its slot.  A wave takes 64 chunks, scans their run
+// counts and deals the runs out to its lanes, so every lane places a run.
+// Within a cell the list is in the atomics' arrival order, which k_scatter
+// ranks away.
 // ---------------------------------------------------------------------------
+constexpr int PLACE_U = 4;
+
 __global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__ ctrl)
 {
-    const uint32_t nrec = (ctrl->n_move + 63u) & ~63u;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += stride) {
-        if ((int)(i & 63u) >= bn.chunk_runs[i >> 6]) continue;
-        const int4 r = bn.runrec[i];
-        const unsigned long long v = bn.lstart[r.x] + bn.toff[r.x / SCAN_TILE];
-        bn.runlist[(uint32_t)(v >> 32) + (uint32_t)r.y] = make_int4((int)i, 0, r.z, r.w);
+    const int lane = lane_id();
+    const uint32_t nch = (ctrl->n_move + 63u) >> 6;
+    int first, end, stride;
+    xcd_range((int)((nch + 63u) >> 6), first, end, stride);
+    for (int task = first; task < end; task += stride) {
+        const uint32_t q0 = (uint32_t)task * 64u;
+        const int nr = q0 + lane < nch ? bn.chunk_runs[q0 + lane] : 0;
+        const int inc = wave_incl_scan(nr), excl = inc - nr;
+        const int total = __shfl(inc, 63, 64);
+        // PLACE_U runs per lane per pass, their record loads in flight together
+        for (int t0 = 0; t0 < total; t0 += 64 * PLACE_U) {
+            size_t ri[PLACE_U];
+            int4 r[PLACE_U];
+#pragma unroll
+            for (int u = 0; u < PLACE_U; ++u) {
+                const int t = t0 + 64 * u + lane;
+                // the chunk holding run t: the last lane whose exclusive count is <= t
+                int j = 0;
+#pragma unroll
+                for (int d = 32; d > 0; d >>= 1) {
+                    const int ex = __shfl(excl, j + d, 64);
+                    if (ex <= t) j += d;
+                }
+                const int exj = __shfl(excl, j, 64);   // every lane takes part in the shuffles
+                ri[u] = t < total ? (size_t)(q0 + j) * 64 + (t - exj) : ~(size_t)0;
+            }
+#pragma unroll
+            for (int u = 0; u < PLACE_U; ++u)
+                if (ri[u] != ~(size_t)0) r[u] = bn.runrec[ri[u]];
+#pragma unroll
+            for (int u = 0; u < PLACE_U; ++u)
+                if (ri[u] != ~(size_t)0) {
+                    const unsigned long long v = bn.lstart[r[u].x] + bn.toff[r[u].x / SCAN_TILE];
+                    bn.runlist[(uint32_t)(v >> 32) + (uint32_t)r[u].y] = make_int4((int)ri[u], 0, r[u].z, r[u].w);
+                }
+        }
     }
+}
+
+// Position of the r-th set bit of m (r < popcount(m)), by halving.
+__device__ __forceinline__ int select_bit(unsigned long long m, int r)
+{
+    int pos = 0;
+    uint32_t w = (uint32_t)m;
+    int c = __popc(w);
+    if (r >= c) { r -= c; w = (uint32_t)(m >> 32); pos = 32; }
+    c = __popc(w & 0xffffu);
+    if (r >= c) { r -= c; w >>= 16; pos += 16; }
+    c = __popc(w & 0xffu);
+    if (r >= c) { r -= c; w >>= 8; pos += 8; }
+    c = __popc(w & 0xfu);
+    if (r >= c) { r -= c; w >>= 4; pos += 4; }
+    c = __popc(w & 0x3u);
+    if (r >= c) { r -= c; w >>= 2; pos += 2; }
+    return pos + (r >= (int)(w & 1u) ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,24 +404,29 @@ __global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__
 // particle index.  Run j's members go to start(c) + (the lengths of the
 // cell's runs with smaller id) + (their rank in the mask): perm lists each
 // cell's particles in ascending index order -- the order of the serial
-// binning, bit for bit.  Also materialises start[0..C] and, for every tile
-// of tile_p particles, the first cell that starts in it and that cell's start
-// (tile_first holds int2 pairs).
+// binning, bit for bit.  The runs are taken SC_G at a time: each lane ranks
+// one, then the group writes the SC_G runs one after the other, every lane
+// one member (consecutive addresses).  Also materialises start[0..C] and, for
+// every tile of tile_p particles, the first cell that starts in it and that
+// cell's start (tile_first holds int2 pairs).
 // ---------------------------------------------------------------------------
 constexpr int SC_G = 16;
+constexpr int SC_WIN = 1024;   // LDS window per wave: the permutation of its cells
 
 __global__ __launch_bounds__(256) void k_scatter(Bins bn, int *__restrict__ start, int *__restrict__ perm,
                                                  int *__restrict__ tile_first, int ncell, int tile_p,
                                                  const Ctrl *__restrict__ ctrl)
 {
     constexpr int CPW = 64 / SC_G;
+    __shared__ int s_win[4][SC_WIN];
     const int lane = lane_id(), g = lane / SC_G, sub = lane % SC_G;
+    int *const win = s_win[threadIdx.x >> 6];
     const uint32_t n_next = ctrl->n_next, n_runs = ctrl->n_runs;
     const int ntiles = ((int)n_next + tile_p - 1) / tile_p;
-    const int waves = gridDim.x * (blockDim.x >> 6);
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    for (int base = wave * CPW; base <= ncell; base += waves * CPW) {
-        const int c = base + g;
+    int first, end, stride;
+    xcd_range(ncell / CPW + 1, first, end, stride);
+    for (int task = first; task < end; task += stride) {
+        const int c = task * CPW + g;
         const unsigned long long v0 = c <= ncell ? packed_start(bn, c, ncell, n_next, n_runs) : 0ull;
         const unsigned long long v1 = c < ncell ? packed_start(bn, c + 1, ncell, n_next, n_runs) : v0;
         const int st = (int)(uint32_t)v0, rs = (int)(v0 >> 32);
@@ -349,28 +437,60 @@ __global__ __launch_bounds__(256) void k_scatter(Bins bn, int *__restrict__ star
             const int thi = min(st / tile_p, ntiles - 1);
             for (int t = tlo; t <= thi; ++t) reinterpret_cast<int2 *>(tile_first)[t] = make_int2(c, st);
         }
-        // wave-uniform passes (every lane takes part in the shuffles)
-        for (int b0 = 0; __ballot(b0 < L) != 0ull; b0 += SC_G) {
-            const bool have = b0 + sub < L;
-            const int4 e = have ? bn.runlist[rs + b0 + sub] : make_int4(0x7fffffff, 0, 0, 0);
-            const unsigned long long mask = ((unsigned long long)(uint32_t)e.w << 32) | (uint32_t)e.z;
+        // Lw: the wave's largest run count (the trip count of the wave-uniform
+        // loops); its cells hold perm[P0, P1), staged in LDS when that fits
+        int Lw = L, P1 = c < ncell ? (int)(uint32_t)v1 : 0;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            Lw = max(Lw, __shfl_xor(Lw, d, 64));
+            P1 = max(P1, __shfl_xor(P1, d, 64));
+        }
+        const int P0 = __shfl(st, 0, 64);
+        const bool staged = P1 - P0 <= SC_WIN;
+        for (int b0 = 0; b0 < Lw; b0 += SC_G) {
+            // this lane's run: id, mask (none: id INT_MAX, empty mask)
+            int rid = 0x7fffffff, mlo = 0, mhi = 0;
+            if (b0 + sub < L) {
+                const int4 e = bn.runlist[rs + b0 + sub];
+                rid = e.x; mlo = e.z; mhi = e.w;
+            }
             int off = 0;
-            for (int p2 = 0; __ballot(p2 < L) != 0ull; p2 += SC_G) {
-                int4 f = e;
-                if (p2 != b0) f = p2 + sub < L ? bn.runlist[rs + p2 + sub] : make_int4(0x7fffffff, 0, 0, 0);
-                const int lk = __popc((uint32_t)f.z) + __popc((uint32_t)f.w);
+            for (int p2 = 0; p2 < Lw; p2 += SC_G) {
+                int fid = rid, flen = __popc((uint32_t)mlo) + __popc((uint32_t)mhi);
+                if (p2 != b0) {
+                    fid = 0x7fffffff;
+                    flen = 0;
+                    if (p2 + sub < L) {
+                        const int4 f = bn.runlist[rs + p2 + sub];
+                        fid = f.x;
+                        flen = __popc((uint32_t)f.z) + __popc((uint32_t)f.w);
+                    }
+                }
 #pragma unroll
                 for (int t = 0; t < SC_G; ++t) {
-                    const int r2 = __shfl(f.x, t, SC_G), l2 = __shfl(lk, t, SC_G);
-                    if (r2 < e.x) off += l2;
+                    const int r2 = __shfl(fid, t, SC_G), l2 = __shfl(flen, t, SC_G);
+                    if (r2 < rid) off += l2;
                 }
             }
-            if (have) {
-                const int q0 = e.x & ~63;
-                int *out = perm + st + off;
-                unsigned long long m = mask;
-                for (int r = 0; m; ++r, m &= m - 1ull) out[r] = q0 + (int)__builtin_ctzll(m);
+            // every lane writes its own run's members, in ascending index order
+            if (b0 + sub < L) {
+                const int q0 = rid & ~63;
+                unsigned long long m = ((unsigned long long)(uint32_t)mhi << 32) | (uint32_t)mlo;
+                if (staged) {
+                    int *o = win + (st - P0) + off;
+                    for (int r = 0; m; ++r, m &= m - 1ull) o[r] = q0 + (int)__builtin_ctzll(m);
+                } else {
+                    int *o = perm + st + off;
+                    for (int r = 0; m; ++r, m &= m - 1ull) o[r] = q0 + (int)__builtin_ctzll(m);
+                }
             }
+        }
+        if (staged) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int k = lane; k < P1 - P0; k += 64) perm[P0 + k] = win[k];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -605,10 +725,23 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
     const uint32_t n_live = (uint32_t)tot_p, ntot = n_live + (uint32_t)tot_q;
     uint32_t err = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + t; i < ntot; i += stride) {
+    uint32_t i = blockIdx.x * blockDim.x + t;
+    // next live particle's state loaded one iteration ahead (as in k_move)
+    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
+    if (i < n_live) { px = S.x[i]; py = S.y[i]; pvx = S.vx[i]; pvy = S.vy[i]; pvz = S.vz[i]; }
+    int last = -1;          // the previous particle's instance: kept while its range holds the key
+    bool last_inflow = false;
+    for (; i < ntot; i += stride) {
+        float lx = px, ly = py, lvx = pvx, lvy = pvy, lvz = pvz;
+        const uint32_t jn = i + stride;
+        if (jn < n_live) { px = S.x[jn]; py = S.y[jn]; pvx = S.vx[jn]; pvy = S.vy[jn]; pvz = S.vz[jn]; }
         const bool inflow = i >= n_live;
         const uint32_t key = inflow ? i - n_live : i;
-        const int inst = find_inst(inflow ? s_q : s_p, ninst, key);
+        const uint32_t *tab = inflow ? s_q : s_p;
+        const int inst = (last >= 0 && inflow == last_inflow && tab[last] <= key && key < tab[last + 1])
+                             ? last : find_inst(tab, ninst, key);
+        last = inst;
+        last_inflow = inflow;
         int c = -1;
         for (bool todo = true; todo;) {
             const int u = __builtin_amdgcn_readfirstlane(inst);
@@ -619,7 +752,7 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
             float x, y, vx, vy, vz, er = 0.0f;
             bool dead;
             if (!inflow) {
-                x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
+                x = lx; y = ly; vx = lvx; vy = lvy; vz = lvz;
                 bool vchg = false, echg = false;
                 dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
                 S.x[i] = x; S.y[i] = y;
@@ -628,11 +761,8 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
                 const uint32_t q = key - s_q[u];
                 int J[4];
                 inject_counts(cs, step, J);
-                int f = 0;
-                uint32_t acc = (uint32_t)J[0];
-                while (f < 3 && q >= acc) { ++f; acc += (uint32_t)J[f]; }
                 const uint32_t pid = ictl[u].next_pid + q;
-                dead = inject_one(cs, f, pid, step, x, y, vx, vy, vz, er, err);
+                dead = inject_one(cs, inflow_face(J, q), pid, step, x, y, vx, vy, vz, er, err);
                 S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
             }
             c = dead ? -1 : cbase[u] + cell_of(cs, x, y);
