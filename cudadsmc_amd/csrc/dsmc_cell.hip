@@ -341,12 +341,19 @@ struct KSingle {
     __device__ void add(int, uint32_t, uint32_t) const {}
 };
 
-// Batched instances: LDS copies of the instance table and of the first cell
-// of every instance; sums of collisions and candidates per instance.
+// Batched instances: an LDS copy of the instance table (first cell of every
+// instance), the constants of the instances that the current chunk of cells
+// spans (at most CC_WIN of them in LDS, the rest read from the global table)
+// and their sums of collisions and candidates, added to the instance
+// counters when the chunk is done.  Staging only those keeps the block's LDS
+// small enough for four blocks per CU.
+constexpr int CC_WIN = 8;
+
 struct BatchShared {
     int cbase[MAX_INST + 1];
-    CellConst cc[MAX_INST];
-    uint32_t icoll[MAX_INST], icand[MAX_INST];
+    CellConst cc[CC_WIN];            // instances w0 .. w0 + wn - 1
+    uint32_t icoll[CC_WIN], icand[CC_WIN];
+    int w0, wn;
     int last;
 };
 
@@ -355,25 +362,65 @@ struct KBatch {
     const int *inst;     // instance of each staged cell (CellShared::inst)
     int ninst;
     uint32_t step;
+    const CellConst *gcc;    // every instance's constants (global)
+    InstCtrl *ictl;
+    // Adds the window's sums to the instance counters and clears them.
+    // Called by wave 0 only, after a block barrier that ended every add.
+    __device__ void flush() const
+    {
+        const int t = (int)threadIdx.x;
+        if (t < b.wn) {
+            if (b.icoll[t]) atomicAdd(&ictl[b.w0 + t].step_coll, b.icoll[t]);
+            if (b.icand[t]) atomicAdd(&ictl[b.w0 + t].step_cand, b.icand[t]);
+        }
+    }
+    // The previous chunk ended in a block barrier.  Wave 0 flushes the old
+    // window and then stages the new one: one wave runs in order, so its
+    // reads of the old window precede its writes of the new; the caller's
+    // barrier publishes it.
     __device__ void prepare(CellShared &sh, int c0, int nc) const
     {
         for (int i = threadIdx.x; i < nc; i += blockDim.x) sh.inst[i] = find_cell_inst(b.cbase, ninst, c0 + i);
+        if (threadIdx.x < 64) {
+            flush();
+            const int w0 = find_cell_inst(b.cbase, ninst, c0);
+            const int wn = min(find_cell_inst(b.cbase, ninst, c0 + nc - 1) - w0 + 1, CC_WIN);
+            const int t = (int)threadIdx.x;
+            if (t < wn) {
+                b.cc[t] = gcc[w0 + t];
+                b.icoll[t] = 0u;
+                b.icand[t] = 0u;
+            }
+            if (t == 0) { b.w0 = w0; b.wn = wn; }
+        }
+    }
+    __device__ CellConst cc(int u) const
+    {
+        const int k = u - b.w0;
+        return (k >= 0 && k < b.wn) ? b.cc[k] : gcc[u];
     }
     __device__ CK key(int i, int c) const
     {
         const int u = inst[i];
-        return CK{ b.cc[u].K1, b.cc[u].e_exp, b.cc[u].seed, c - b.cbase[u], 0.f, 0.f, 0.f };   // monatomic
+        const CellConst q = cc(u);
+        return CK{ q.K1, q.e_exp, q.seed, c - b.cbase[u], 0.f, 0.f, 0.f };   // monatomic
     }
-    __device__ float ntc_coef(int i) const { return b.cc[inst[i]].ntc_coef; }
+    __device__ float ntc_coef(int i) const { return cc(inst[i]).ntc_coef; }
     __device__ bool sample(int i) const
     {
-        const int s0 = b.cc[inst[i]].sample_start;
+        const int s0 = cc(inst[i]).sample_start;
         return s0 >= 0 && step >= (uint32_t)s0;
     }
     __device__ void add(int i, uint32_t coll, uint32_t cand) const
     {
-        if (coll) atomicAdd(&b.icoll[inst[i]], coll);
-        if (cand) atomicAdd(&b.icand[inst[i]], cand);
+        const int u = inst[i], k = u - b.w0;
+        if (k >= 0 && k < b.wn) {
+            if (coll) atomicAdd(&b.icoll[k], coll);
+            if (cand) atomicAdd(&b.icand[k], cand);
+        } else {
+            if (coll) atomicAdd(&ictl[u].step_coll, coll);
+            if (cand) atomicAdd(&ictl[u].step_cand, cand);
+        }
     }
 };
 
@@ -572,7 +619,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
 struct NoBatch {};
 
 template <bool BATCH, bool ROT>
-__global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
+__global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
 {
     // LDS staging of a tile: permutation, velocities and (ROT) rotational
     // energies; for a single cell staged in HBM it holds the collision
@@ -596,11 +643,7 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
     if (tid == 0) { sh.coll = 0u; sh.cand = 0u; }
     if constexpr (BATCH) {
         for (int t = tid; t <= a.ninst; t += blockDim.x) bsh.cbase[t] = a.cbase[t];
-        for (int t = tid; t < a.ninst; t += blockDim.x) {
-            bsh.cc[t] = a.cc[t];
-            bsh.icoll[t] = 0u;
-            bsh.icand[t] = 0u;
-        }
+        if (tid == 0) { bsh.w0 = 0; bsh.wn = 0; }
     }
     uint32_t tcoll = 0, tcand = 0;
     const int2 *tiles = reinterpret_cast<const int2 *>(a.tile_first);
@@ -657,17 +700,14 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
                                       nxt.vz + p0, ROT ? nxt.er + p0 : nullptr, sh, tcoll, tcand, stage.i,
                                       4 * CELL_CAP);
         };
-        if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step });
+        if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step, a.cc, a.ictl });
         else run(KSingle{ a.cs, sampling });
     }
     if (tcoll) atomicAdd(&sh.coll, tcoll);
     if (tcand) atomicAdd(&sh.cand, tcand);
     if constexpr (BATCH) {
         __syncthreads();
-        for (int t = tid; t < a.ninst; t += blockDim.x) {
-            if (bsh.icoll[t]) atomicAdd(&a.ictl[t].step_coll, bsh.icoll[t]);
-            if (bsh.icand[t]) atomicAdd(&a.ictl[t].step_cand, bsh.icand[t]);
-        }
+        if (tid < 64) KBatch{ bsh, sh.inst, a.ninst, step, a.cc, a.ictl }.flush();
         __threadfence();
     }
     __syncthreads();
@@ -690,7 +730,7 @@ __global__ __launch_bounds__(256, 3) void k_cell(CellArgs a)
             const uint32_t nn = (uint32_t)(a.start[bsh.cbase[t + 1]] - a.start[bsh.cbase[t]]);
             const uint32_t sc = __hip_atomic_load(&ic.step_coll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t sm = __hip_atomic_load(&ic.step_cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int s0 = bsh.cc[t].sample_start;
+            const int s0 = a.cc[t].sample_start;
             ic.psteps += nl;
             ic.next_pid += ic.n_move - nl;
             ic.n_live = nn;
