@@ -26,6 +26,8 @@ namespace dsmc {
 namespace {
 
 constexpr int CELL_MAXC = 128;       // cells whose metadata is staged at once
+constexpr int CELL_THREADS = 64 * CELL_WAVES;   // k_cell's block
+static_assert(CELL_MAXC + 1 <= CELL_THREADS, "one thread per staged cell start");
 constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
 constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
 constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
@@ -387,7 +389,8 @@ struct KBatch {
             const int wn = min(find_cell_inst(b.cbase, ninst, c0 + nc - 1) - w0 + 1, CC_WIN);
             const int t = (int)threadIdx.x;
             if (t < wn) {
-                b.cc[t] = gcc[w0 + t];
+                const CellConst q = gcc[w0 + t];
+                b.cc[t] = q;
                 b.icoll[t] = 0u;
                 b.icand[t] = 0u;
             }
@@ -438,16 +441,49 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     // a tile of one chunk of cells loads its permutation, cell starts and
     // collision state in one round trip (the metadata stays staged until the
     // collisions); larger tiles stage each chunk's metadata at the collisions
+    // The loads of this stage are issued before any is waited for (a loop
+    // that stores each value to LDS as it arrives waits once per value):
+    // the permutation in rounds of PV per thread, the first round together
+    // with the cell metadata.
     const bool one = cb - ca <= CELL_MAXC;
-    if (LDS)
-        for (int k = tid; k < n; k += bs) Sp[k] = a.perm[p0 + k];
-    if (one) {
+    {
+        constexpr int PV = 4;
         const int nc = cb - ca;
-        for (int i = tid; i <= nc; i += bs) sh.st[i] = a.start[ca + i] - p0;
-        for (int i = tid; i < nc; i += bs) {
-            sh.sm[i] = a.sigcmax[ca + i];
-            sh.mr[i] = a.mrem[ca + i];
-            sh.vol[i] = cs.vol_ntc[ca + i];
+        for (int k0 = 0; k0 < (LDS ? n : 0) || k0 == 0; k0 += PV * CELL_THREADS) {
+            int pv[PV];
+            if (LDS) {
+#pragma unroll
+                for (int j = 0; j < PV; ++j) {
+                    const int k = k0 + tid + j * CELL_THREADS;
+                    pv[j] = k < n ? a.perm[p0 + k] : 0;
+                }
+            }
+            int st0 = 0;
+            float sm0 = 0.0f, mr0 = 0.0f, vol0 = 0.0f;
+            if (one && k0 == 0) {
+                if (tid <= nc) st0 = a.start[ca + tid];
+                if (tid < nc) {
+                    sm0 = a.sigcmax[ca + tid];
+                    mr0 = a.mrem[ca + tid];
+                    vol0 = cs.vol_ntc[ca + tid];
+                }
+                kf.prepare(sh, ca, nc);
+            }
+            if (LDS) {
+#pragma unroll
+                for (int j = 0; j < PV; ++j) {
+                    const int k = k0 + tid + j * CELL_THREADS;
+                    if (k < n) Sp[k] = pv[j];
+                }
+            }
+            if (one && k0 == 0) {
+                if (tid <= nc) sh.st[tid] = st0 - p0;
+                if (tid < nc) {
+                    sh.sm[tid] = sm0;
+                    sh.mr[tid] = mr0;
+                    sh.vol[tid] = vol0;
+                }
+            }
         }
     }
     __syncthreads();
@@ -500,8 +536,8 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 sh.mr[i] = a.mrem[c0 + i];
                 sh.vol[i] = cs.vol_ntc[c0 + i];
             }
+            kf.prepare(sh, c0, nc);
         }
-        kf.prepare(sh, c0, nc);
         if (tid == 0) { sh.nheavy = 0; sh.nbig = 0; }
         __syncthreads();
         // candidate counts; cells go to the grouped rounds or, with many
@@ -776,9 +812,9 @@ hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int 
                 nullptr, nullptr, 0, nullptr, tile_p };
     if (cs.zeta != 0) {
         if (!A.er || !B.er) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_cell<false, true>), dim3(grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_cell<false, true>), dim3(grid), dim3(CELL_THREADS), 0, st, a);
     } else {
-        hipLaunchKernelGGL((k_cell<false, false>), dim3(grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_cell<false, false>), dim3(grid), dim3(CELL_THREADS), 0, st, a);
     }
     return hipGetLastError();
 }
@@ -792,7 +828,7 @@ hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int 
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, cc, cbase, ninst,
                 ictl, tile_p };
-    hipLaunchKernelGGL((k_cell<true, false>), dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_cell<true, false>), dim3(grid), dim3(CELL_THREADS), 0, st, a);
     return hipGetLastError();
 }
 
