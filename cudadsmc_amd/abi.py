@@ -120,6 +120,9 @@ SIGNATURES = {
     "dsmc_sync": [_h],
     "dsmc_status_get": [_h, _P(dsmc_status)],
     "dsmc_set_keep_perm": [_h, C.c_int],
+    "dsmc_set_subcells": [_h, _u32p],
+    "dsmc_amr_update": [_h],
+    "dsmc_download_amr": [_h, _u32p, _f32p, C.POINTER(C.c_double)],
     "dsmc_download_moved": [_h, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, _P(C.c_int64)],
     "dsmc_download_bins": [_h, C.c_int64, _i32p, _i32p],
     "dsmc_download_start": [_h, _i32p],

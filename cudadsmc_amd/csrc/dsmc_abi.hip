@@ -51,6 +51,11 @@ struct dsmc_handle_s {
     float *d_fbox = nullptr;
     uint64_t *d_fmask = nullptr;
     double *d_expected = nullptr;
+    // adaptive collision sub-cells (allocated on first use)
+    uint32_t *d_sub = nullptr;
+    float *d_lam = nullptr;
+    double *d_dtv = nullptr, *d_vol = nullptr;
+    int grid_amr = 0;
     int grid_move = 0, grid_scatter = 0, grid_cell = 0, grid_fill = 0;
     int tile_p = CELL_P, graph_len = 8;
     int timing = 0;
@@ -96,6 +101,9 @@ int launch_step(dsmc_handle h, int phase_mask)
         HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->tile_first, h->sigcmax, h->mrem,
                            h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->tile_p,
                            h->grid_cell, st));
+        if (h->cs.sub)
+            HIPCHK(launch_amr_cell(h->cs, h->A, h->B, h->start, h->sigcmax, h->mrem, h->ncoll_cell, h->acc,
+                                   h->ctrl, h->P.in.sample_start, h->grid_amr, st));
         h->cell_order = true;
     } else if (phase_mask & 1) {
         h->cell_order = false;
@@ -832,6 +840,86 @@ int dsmc_reset_samples(dsmc_handle h)
     HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(double) * h->P.sample_blocks * (size_t)h->ncell, h->stream));
     HIPCHK(hipMemsetAsync(&h->ctrl->nsamples, 0, sizeof(unsigned long long), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return DSMC_OK;
+}
+
+// Device buffers of the sub-cell grids, the fluid volumes (FP64) and the
+// launch shape of k_amr_cell, on first use.
+static int amr_alloc(dsmc_handle h)
+{
+    if (h->d_sub) return DSMC_OK;
+    const size_t C = (size_t)h->ncell;
+    hipError_t e;
+    if ((e = h->alloc(&h->d_sub, C)) || (e = h->alloc(&h->d_lam, C)) || (e = h->alloc(&h->d_dtv, C)) ||
+        (e = h->alloc(&h->d_vol, C)))
+        return fail(DSMC_ERR_HIP, std::string("sub-cell buffers: ") + hipGetErrorString(e));
+    HIPCHK(hipMemcpy(h->d_vol, h->T.vol.data(), sizeof(double) * C, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(h->d_sub, 0, sizeof(uint32_t) * C));
+    HIPCHK(hipMemset(h->d_lam, 0, sizeof(float) * C));
+    HIPCHK(hipMemset(h->d_dtv, 0, sizeof(double) * C));
+    int cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, h->device));
+    h->grid_amr = cu * occupancy_amr_cell(h->P.in.zeta != 0);
+    return DSMC_OK;
+}
+
+int dsmc_set_subcells(dsmc_handle h, const uint32_t *sub)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    drop_graph(h);
+    if (!sub) {
+        h->cs.sub = nullptr;
+        return DSMC_OK;
+    }
+    for (int c = 0; c < h->ncell; ++c) {
+        const uint32_t v = sub[c];
+        if (v && (!(v & 0xffu) || !((v >> 8) & 0xffu) || !((v >> 16) & 0xffu) || !((v >> 24) & 0xffu)))
+            return fail(DSMC_ERR_ARG, "dsmc_set_subcells: every level of a grid needs at least one sub-cell");
+    }
+    if ((r = amr_alloc(h))) return r;
+    HIPCHK(hipMemcpy(h->d_sub, sub, sizeof(uint32_t) * (size_t)h->ncell, hipMemcpyHostToDevice));
+    h->cs.sub = h->d_sub;
+    return DSMC_OK;
+}
+
+int dsmc_amr_update(dsmc_handle h)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    Ctrl c;
+    if ((r = read_ctrl(h, c))) return r;
+    if (c.nsamples == 0) return fail(DSMC_ERR_ARG, "dsmc_amr_update: no sampled step yet");
+    if ((r = amr_alloc(h))) return r;
+    const dsmc_params &p = h->P.in;
+    HIPCHK(launch_amr_update(h->cs, h->P.Fn, p.mass, p.diam, p.T, p.Vtl, p.boltz, h->acc, h->d_vol, h->ctrl,
+                             h->d_sub, h->d_lam, h->d_dtv, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!h->cs.sub) {
+        drop_graph(h);
+        h->cs.sub = h->d_sub;
+    }
+    return DSMC_OK;
+}
+
+int dsmc_download_amr(dsmc_handle h, uint32_t *sub, float *lam, double *dt)
+{
+    if (!h) return fail(DSMC_ERR_ARG, "null handle");
+    int r = set_device(h);
+    if (r) return r;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t C = (size_t)h->ncell;
+    if (!h->d_sub) {
+        if (sub) std::fill(sub, sub + C, 0u);
+        if (lam) std::fill(lam, lam + C, 0.0f);
+        if (dt) std::fill(dt, dt + C, 0.0);
+        return DSMC_OK;
+    }
+    if (sub) HIPCHK(hipMemcpy(sub, h->d_sub, sizeof(uint32_t) * C, hipMemcpyDeviceToHost));
+    if (lam) HIPCHK(hipMemcpy(lam, h->d_lam, sizeof(float) * C, hipMemcpyDeviceToHost));
+    if (dt) HIPCHK(hipMemcpy(dt, h->d_dtv, sizeof(double) * C, hipMemcpyDeviceToHost));
     return DSMC_OK;
 }
 

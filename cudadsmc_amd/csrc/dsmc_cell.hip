@@ -341,6 +341,8 @@ struct KSingle {
     __device__ float ntc_coef(int) const { return cs.ntc_coef; }
     __device__ bool sample(int) const { return sampling; }
     __device__ void add(int, uint32_t, uint32_t) const {}
+    // cells with adaptive sub-cells collide and sample in k_amr_cell
+    __device__ bool amr(int c, int k) const { return amr_cell(cs, c, k); }
 };
 
 // Batched instances: an LDS copy of the instance table (first cell of every
@@ -414,6 +416,7 @@ struct KBatch {
         const int s0 = cc(inst[i]).sample_start;
         return s0 >= 0 && step >= (uint32_t)s0;
     }
+    __device__ bool amr(int, int) const { return false; }   // batched instances: no sub-cells
     __device__ void add(int i, uint32_t coll, uint32_t cand) const
     {
         const int u = inst[i], k = u - b.w0;
@@ -546,7 +549,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             const int o = sh.st[i], k = sh.st[i + 1] - o;
             int M = 0;
             const float vol = sh.vol[i];
-            if (k >= 2 && vol > 0.0f) {
+            if (k >= 2 && vol > 0.0f && !kf.amr(c0 + i, k)) {
                 float mr = sh.mr[i];
                 M = ntc_count(kf.ntc_coef(i), k, vol, sh.sm[i], mr);
                 sh.mr[i] = mr;
@@ -606,7 +609,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 const int i = base + g;
                 const int o = i < nc ? sh.st[i] : 0;
                 const int k = i < nc ? sh.st[i + 1] - o : 0;
-                const bool on = k > 0 && kf.sample(i);
+                const bool on = k > 0 && kf.sample(i) && !kf.amr(c0 + i, k);
                 double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
                 if (on)
                     for (int q = sub; q < k; q += SAMPLE_G) {
@@ -801,6 +804,284 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     ctrl->cell_done = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// Adaptive collision sub-cells (Cell::genAMRmesh, Cell.cpp:168-279;
+// Cell::sortParticle2children, Cell.cpp:280-295; Cell::collision over the
+// leaves, Cell.cpp:335-352).  After k_cell (which leaves these cells alone
+// but for the gather into cell order), one block per cell with sub-cells:
+//   1. leaf of every particle; a stable order by leaf (wave 0 walks the
+//      cell in chunks of 64, one LDS cursor per leaf);
+//   2. the candidate numbers leaf by leaf with the remainder carried in leaf
+//      order (thread 0), then one thread per leaf runs its candidates with
+//      its own running (sigma c)max from the cell's -- the candidates are
+//      numbered on through the leaves, one Philox stream per cell and step;
+//   3. the cell's state written back in leaf order, FP64 sampling sums.
+// The arithmetic is k_cell's (ntc_count, vhs_sigc, scatter_dir,
+// scatter_pair, lb_exchange).
+// ---------------------------------------------------------------------------
+struct AmrArgs {
+    DevCase cs;
+    SoA A, B;
+    const int *start;
+    float *sigcmax, *mrem;
+    int *ncoll_cell;
+    double *acc;
+    Ctrl *ctrl;
+    int sample_start;
+};
+
+constexpr int AMR_THREADS = 256;
+
+template <bool ROT>
+__global__ __launch_bounds__(AMR_THREADS) void k_amr_cell(AmrArgs a)
+{
+    __shared__ int s_idx[AMR_CAP];          // leaf order -> particle in the cell
+    __shared__ int s_leaf[AMR_CAP];         // leaf of each particle (cell order)
+    __shared__ float s_v[ROT ? 4 : 3][AMR_CAP];
+    __shared__ int s_ls[AMR_LEAVES + 1], s_pos[AMR_LEAVES], s_M[AMR_LEAVES], s_m0[AMR_LEAVES];
+    __shared__ int s_list[AMR_THREADS], s_nlist;
+    __shared__ uint32_t s_nc, s_best;
+    const DevCase &cs = a.cs;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t step = a.ctrl->step - 1u;          // k_cell has advanced the step
+    const SoA S = ((step + 1u) & 1u) ? a.B : a.A;     // the state k_cell wrote
+    const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
+    const size_t C = (size_t)cs.ncell;
+    uint32_t bcoll = 0, bcand = 0;
+    for (int c0 = blockIdx.x * AMR_THREADS; c0 < cs.ncell; c0 += gridDim.x * AMR_THREADS) {
+        // this block's cells with sub-cells, found AMR_THREADS at a time
+        if (tid == 0) s_nlist = 0;
+        __syncthreads();
+        {
+            const int c = c0 + tid;
+            if (c < cs.ncell && amr_cell(cs, c, a.start[c + 1] - a.start[c])) s_list[atomicAdd(&s_nlist, 1)] = c;
+        }
+        __syncthreads();
+        const int nlist = s_nlist;
+        for (int q = 0; q < nlist; ++q) {
+            const int c = s_list[q];
+            const int o = a.start[c], np = a.start[c + 1] - o;
+            int nsx, nsy;
+            const int nl = amr_leaves(cs.sub[c], nsx, nsy);
+            // 1. leaves and their counts
+            for (int l = tid; l < nl; l += AMR_THREADS) s_pos[l] = 0;
+            if (tid == 0) { s_nc = 0u; s_best = __float_as_uint(a.sigcmax[c]); }
+            __syncthreads();
+            for (int k = tid; k < np; k += AMR_THREADS) {
+                const int lf = amr_leaf(cs, c, S.x[o + k], S.y[o + k], nsx, nsy);
+                s_leaf[k] = lf;
+                atomicAdd(&s_pos[lf], 1);
+            }
+            __syncthreads();
+            // 2a. leaf starts, candidate numbers with the carried remainder
+            if (tid == 0) {
+                const float vol = cs.vol_ntc[c];
+                const float vleaf = vol / (float)nl;
+                const float smax0 = a.sigcmax[c];
+                float mr = a.mrem[c];
+                int st = 0, m0 = 0;
+                for (int l = 0; l < nl; ++l) {
+                    const int npl = s_pos[l];
+                    s_ls[l] = st;
+                    s_pos[l] = st;
+                    st += npl;
+                    int M = 0;
+                    if (vol > 0.0f && npl >= 2) M = ntc_count(cs.ntc_coef, npl, vleaf, smax0, mr);
+                    s_M[l] = M;
+                    s_m0[l] = m0;
+                    m0 += M;
+                }
+                s_ls[nl] = st;
+                a.mrem[c] = mr;
+                bcand += (uint32_t)m0;
+            }
+            __syncthreads();
+            // 1b. stable order by leaf: wave 0, chunk after chunk in index order
+            if (tid < 64) {
+                const unsigned long long lt = (1ull << lane) - 1ull;
+                for (int k0 = 0; k0 < np; k0 += 64) {
+                    const int k = k0 + lane;
+                    const int lf = k < np ? s_leaf[k] : -1;
+                    unsigned long long todo = __ballot(lf >= 0);
+                    while (todo) {
+                        const int l0 = (int)__ffsll((long long)todo) - 1;
+                        const int ll = __shfl(lf, l0, 64);
+                        const unsigned long long m = __ballot(lf == ll);
+                        int base = 0;
+                        if (lane == l0) base = s_pos[ll];
+                        base = __shfl(base, l0, 64);
+                        if (lf == ll) s_idx[base + __popcll(m & lt)] = k;
+                        if (lane == l0) s_pos[ll] = base + __popcll(m);
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        todo &= ~m;
+                    }
+                }
+            }
+            __syncthreads();
+            // velocities in leaf order
+            for (int p = tid; p < np; p += AMR_THREADS) {
+                const int k = o + s_idx[p];
+                s_v[0][p] = S.vx[k];
+                s_v[1][p] = S.vy[k];
+                s_v[2][p] = S.vz[k];
+                if constexpr (ROT) s_v[3][p] = S.er[k];
+            }
+            __syncthreads();
+            // 2b. one thread per leaf
+            const CK key{ cs.K1, cs.e_exp, cs.seed, c, cs.rot_p, cs.lb_ia, cs.lb_ib };
+            for (int l = tid; l < nl; l += AMR_THREADS) {
+                const int M = s_M[l];
+                if (M == 0) continue;
+                const int b = s_ls[l], npl = s_ls[l + 1] - b, m0 = s_m0[l];
+                float smax = a.sigcmax[c];              // the cell's value at the step's start
+                uint32_t nc = 0;
+                for (int m = m0; m < m0 + M; ++m) {
+                    const uint4 w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+                    const int i1 = mulhi_idx(w.x, npl);
+                    int i2 = mulhi_idx(w.y, npl - 1);
+                    if (i2 >= i1) i2 = i2 + 1;
+                    const int p1 = b + i1, p2 = b + i2;
+                    float v1x = s_v[0][p1], v1y = s_v[1][p1], v1z = s_v[2][p1];
+                    float v2x = s_v[0][p2], v2y = s_v[1][p2], v2z = s_v[2][p2];
+                    const float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
+                    const float g2 = gx * gx + gy * gy + gz * gz;
+                    const float sc = vhs_sigc(key, g2);
+                    if (sc > smax) smax = sc;
+                    if (u01(w.z) < sc / smax) {
+                        Scatter sd = scatter_dir(key, step, m, w.w, g2);
+                        if constexpr (ROT) lb_exchange(key, step, m, g2, s_v[3][p1], s_v[3][p2], sd);
+                        scatter_pair(sd, v1x, v1y, v1z, v2x, v2y, v2z);
+                        s_v[0][p1] = v1x; s_v[1][p1] = v1y; s_v[2][p1] = v1z;
+                        s_v[0][p2] = v2x; s_v[1][p2] = v2y; s_v[2][p2] = v2z;
+                        ++nc;
+                    }
+                }
+                atomicAdd(&s_nc, nc);
+                atomicMax(&s_best, __float_as_uint(smax));   // non-negative floats order as their bits
+            }
+            __syncthreads();
+            // 3. the cell in leaf order: positions and ids through registers
+            constexpr int PR = AMR_CAP / AMR_THREADS;
+            float rx[PR], ry[PR];
+            uint32_t rp[PR];
+#pragma unroll
+            for (int j = 0; j < PR; ++j) {
+                const int p = tid + j * AMR_THREADS;
+                if (p < np) {
+                    const int k = o + s_idx[p];
+                    rx[j] = S.x[k]; ry[j] = S.y[k]; rp[j] = S.pid[k];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < PR; ++j) {
+                const int p = tid + j * AMR_THREADS;
+                if (p < np) {
+                    S.x[o + p] = rx[j]; S.y[o + p] = ry[j]; S.pid[o + p] = rp[j];
+                    S.vx[o + p] = s_v[0][p]; S.vy[o + p] = s_v[1][p]; S.vz[o + p] = s_v[2][p];
+                    if constexpr (ROT) S.er[o + p] = s_v[3][p];
+                }
+            }
+            const uint32_t ncell_coll = s_nc;
+            if (tid == 0) {
+                a.sigcmax[c] = __uint_as_float(s_best);
+                a.ncoll_cell[c] = (int)ncell_coll;
+                bcoll += ncell_coll;
+            }
+            // FP64 sampling sums of the cell (wave 0)
+            if (sampling && tid < 64) {
+                double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
+                for (int p = lane; p < np; p += 64) {
+                    const double u = (double)s_v[0][p], v = (double)s_v[1][p], w = (double)s_v[2][p];
+                    sx += u; sy += v; sz += w;
+                    s2 += u * u + v * v + w * w;
+                    if constexpr (ROT) se += (double)s_v[3][p];
+                }
+#pragma unroll
+                for (int d = 32; d > 0; d >>= 1) {
+                    sx += __shfl_xor(sx, d, 64);
+                    sy += __shfl_xor(sy, d, 64);
+                    sz += __shfl_xor(sz, d, 64);
+                    s2 += __shfl_xor(s2, d, 64);
+                    if constexpr (ROT) se += __shfl_xor(se, d, 64);
+                }
+                if (lane == 0) {
+                    unsafeAtomicAdd(&a.acc[c], (double)np);
+                    unsafeAtomicAdd(&a.acc[C + c], sx);
+                    unsafeAtomicAdd(&a.acc[2 * C + c], sy);
+                    unsafeAtomicAdd(&a.acc[3 * C + c], sz);
+                    unsafeAtomicAdd(&a.acc[4 * C + c], s2);
+                    unsafeAtomicAdd(&a.acc[5 * C + c], (double)ncell_coll);
+                    if constexpr (ROT) unsafeAtomicAdd(&a.acc[6 * C + c], se);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0 && (bcoll || bcand)) {
+        atomicAdd(&a.ctrl->ncoll, (unsigned long long)bcoll);
+        atomicAdd(&a.ctrl->last_coll, bcoll);
+        atomicAdd(&a.ctrl->ncand, (unsigned long long)bcand);
+        atomicAdd(&a.ctrl->last_cand, bcand);
+    }
+}
+
+// Sub-cell grids, mean free paths and VTS time steps from the time-averaged
+// sums (Cell::VTS, Cell.cpp:117-136; Cell::genAMRmesh, Cell.cpp:168-243):
+// lam = m / (sqrt(2) rho pi d^2) (T/T_inf)^(omega - 1/2) (the power by the
+// FP32 log/exp above, the rest in FP64), dt = lam / (2 (|u| + sqrt(2kT/m))),
+// level 2 of side lam/2, n = max(1, ceil(L/side)) at most AMR_MAX; level 3
+// from the smallest old level-3 mean free path -- one per cell here, so
+// lam -- or from twice the level-2 x side when the cell had no grid.
+struct AmrGas {
+    double Fn, mass, diam, T_inf, omega, boltz;
+};
+
+__device__ __forceinline__ int amr_div(double L, double side)
+{
+    const double q = L / side;
+    if (!(q > 1.0)) return 1;
+    const double n = ceil(q);
+    return n > (double)AMR_MAX ? AMR_MAX : (int)n;
+}
+
+__device__ __forceinline__ uint32_t amr_grid(double dx, double dy, double lam_max, double lam_min, bool had)
+{
+    const int n2x = amr_div(dx, 0.5 * lam_max), n2y = amr_div(dy, 0.5 * lam_max);
+    const double l2x = dx / (double)n2x, l2y = dy / (double)n2y;
+    const double mn = had ? lam_min : 2.0 * l2x;
+    const int n3x = amr_div(l2x, 0.5 * mn), n3y = amr_div(l2y, 0.5 * mn);
+    return (uint32_t)n2x | ((uint32_t)n2y << 8) | ((uint32_t)n3x << 16) | ((uint32_t)n3y << 24);
+}
+
+__global__ __launch_bounds__(256) void k_amr_update(DevCase cs, AmrGas g, const double *__restrict__ acc,
+                                                    const double *__restrict__ vol, const Ctrl *__restrict__ ctrl,
+                                                    uint32_t *__restrict__ sub, float *__restrict__ lam,
+                                                    double *__restrict__ dt)
+{
+    const int C = cs.ncell;
+    const long long ns = (long long)ctrl->nsamples;
+    const double pi_d2 = sqrt(2.0) * 3.14159265358979323846 * g.diam * g.diam;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
+        const double S0 = acc[c];
+        lam[c] = 0.0f;
+        dt[c] = 0.0;
+        if (!(S0 > 0.0) || !(vol[c] > 0.0) || ns <= 0) { sub[c] = 0u; continue; }
+        const double n = (S0 / (double)ns) * g.Fn / vol[c];
+        const double ux = acc[C + c] / S0, uy = acc[2 * C + c] / S0, uz = acc[3 * C + c] / S0;
+        const double uu = ux * ux + uy * uy + uz * uz;
+        const double T = g.mass / (3.0 * g.boltz) * (acc[4 * C + c] / S0 - uu);
+        const float tr = (float)(T / g.T_inf);
+        if (!(tr > 0.0f)) { sub[c] = 0u; continue; }
+        const float f = dm_expf((float)(g.omega - 0.5) * dm_logf(tr));
+        const double l = g.mass / (pi_d2 * (n * g.mass)) * (double)f;
+        lam[c] = (float)l;
+        dt[c] = 0.5 * l / (sqrt(uu) + sqrt(2.0 * g.boltz * T / g.mass));
+        sub[c] = amr_grid((double)cs.dx, (double)cs.dy, l, l, sub[c] != 0u);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
@@ -847,6 +1128,39 @@ int occupancy_cell_batch()
         nb < 1)
         nb = 1;
     return nb;
+}
+
+hipError_t launch_amr_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, float *sigcmax,
+                           float *mrem, int *ncoll_cell, double *acc, Ctrl *ctrl, int sample_start, int grid,
+                           hipStream_t st)
+{
+    if (!cs.sub) return hipErrorInvalidValue;
+    AmrArgs a{ cs, A, B, start, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start };
+    if (cs.zeta != 0) {
+        if (!A.er || !B.er) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_amr_cell<true>), dim3(grid), dim3(AMR_THREADS), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_amr_cell<false>), dim3(grid), dim3(AMR_THREADS), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_amr_update(const DevCase &cs, double Fn, double mass, double diam, double T_inf, double omega,
+                             double boltz, const double *acc, const double *vol, const Ctrl *ctrl, uint32_t *sub,
+                             float *lam, double *dt, hipStream_t st)
+{
+    const AmrGas g{ Fn, mass, diam, T_inf, omega, boltz };
+    const int grid = std::max(1, std::min((cs.ncell + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_amr_update, dim3(grid), dim3(256), 0, st, cs, g, acc, vol, ctrl, sub, lam, dt);
+    return hipGetLastError();
+}
+
+int occupancy_amr_cell(bool rot)
+{
+    int n = 0;
+    if (rot) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_amr_cell<true>, AMR_THREADS, 0);
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_amr_cell<false>, AMR_THREADS, 0);
+    return n > 0 ? n : 1;
 }
 
 }  // namespace dsmc

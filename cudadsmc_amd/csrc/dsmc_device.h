@@ -51,7 +51,34 @@ struct DevCase {
     const float *vol_ntc;
     const float *fbox;      // 4 per cell: fluid-part bounding box of cut cells
     const uint64_t *fmask;  // per cell: 8x8 sub-boxes of fbox that touch fluid
+    const uint32_t *sub;    // per cell: adaptive collision sub-cells (packed grid), null: none
 };
+
+// Adaptive collision sub-cells (Cell::genAMRmesh, Cell.cpp:168-279): a cell's
+// grid is packed n2x | n2y << 8 | n3x << 16 | n3y << 24 (level 2, and level 3
+// in every level-2 cell; 0: no grid), its leaves an nsx x nsy uniform grid,
+// nsx = n2x n3x.  A cell with more than AMR_CAP particles or AMR_LEAVES
+// leaves collides without sub-cells.
+constexpr int AMR_MAX = 16;
+constexpr int AMR_CAP = 2048;
+constexpr int AMR_LEAVES = 256;
+
+__host__ __device__ inline int amr_leaves(uint32_t s, int &nsx, int &nsy)
+{
+    if (!s) { nsx = 1; nsy = 1; return 1; }
+    nsx = (int)(s & 0xffu) * (int)((s >> 16) & 0xffu);
+    nsy = (int)((s >> 8) & 0xffu) * (int)((s >> 24) & 0xffu);
+    return nsx * nsy;
+}
+
+// Whether cell c (k particles) collides in sub-cells.
+__device__ __forceinline__ bool amr_cell(const DevCase &cs, int c, int k)
+{
+    if (!cs.sub || k < 2 || k > AMR_CAP) return false;
+    int nsx, nsy;
+    const int nl = amr_leaves(cs.sub[c], nsx, nsy);
+    return nl > 1 && nl <= AMR_LEAVES;
+}
 
 // Device-resident step control: everything a step needs that changes
 // between steps lives here, so a step has no host round trip and can be
@@ -186,6 +213,18 @@ __device__ __forceinline__ int cell_of(const DevCase &cs, float x, float y)
     i = i < 0 ? 0 : (i > cs.nx - 1 ? cs.nx - 1 : i);
     j = j < 0 ? 0 : (j > cs.ny - 1 ? cs.ny - 1 : j);
     return j + i * cs.ny;
+}
+
+// Leaf of a particle at (x, y) in cell c with an nsx x nsy leaf grid: the
+// same FP32 division as cell_of, then the fraction of the cell scaled.
+__device__ __forceinline__ int amr_leaf(const DevCase &cs, int c, float x, float y, int nsx, int nsy)
+{
+    const int i = c / cs.ny, j = c % cs.ny;
+    const float fx = x / cs.dx - (float)i, fy = y / cs.dy - (float)j;
+    int lx = (int)(fx * (float)nsx), ly = (int)(fy * (float)nsy);
+    lx = lx < 0 ? 0 : (lx > nsx - 1 ? nsx - 1 : lx);
+    ly = ly < 0 ? 0 : (ly > nsy - 1 ? nsy - 1 : ly);
+    return ly + lx * nsy;
 }
 
 // Rotational energy per unit mass of a molecule with two rotational degrees

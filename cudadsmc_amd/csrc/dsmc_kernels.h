@@ -37,6 +37,16 @@ hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int 
                              InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
                              const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
                              Ctrl *ctrl, int tile_p, int grid, hipStream_t st);
+// Adaptive collision sub-cells: after launch_cell, the cells with sub-cells
+// (cs.sub) collide and sample in leaves; launch_amr_update sets the grids,
+// mean free paths and VTS time steps from the sampling sums.
+hipError_t launch_amr_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, float *sigcmax,
+                           float *mrem, int *ncoll_cell, double *acc, Ctrl *ctrl, int sample_start, int grid,
+                           hipStream_t st);
+hipError_t launch_amr_update(const DevCase &cs, double Fn, double mass, double diam, double T_inf, double omega,
+                             double boltz, const double *acc, const double *vol, const Ctrl *ctrl, uint32_t *sub,
+                             float *lam, double *dt, hipStream_t st);
+int occupancy_amr_cell(bool rot);
 hipError_t launch_fill(const DevCase &cs, const SoA &A, const double *expected, const uint8_t *cut,
                        const float *ux, const float *uy, const float *sig, int *count, int *lstart,
                        int *tsum, int *toff, Ctrl *ctrl, int grid, hipStream_t st);

@@ -8,7 +8,8 @@ Datasets, shape [N1][N2], index i*N2 + j (the cell order of the solver):
   P        pressure n k T (Pa)
   T        temperature (K)
   Rho      mass density n m (kg/m^3)
-  Numchild, Numgrandchild  adaptive sub-cell counts (int; 0: no sub-cells)
+  Numchild, Numgrandchild  adaptive sub-cell counts: level-2 cells and level-3
+           cells in all of them (int; 0: no sub-cells)
   MFP      mean free path m / (sqrt(2) rho pi d^2) (T/T_inf)^(omega - 1/2)
            (Cell.cpp:125)
   Trot     rotational temperature (K), a diatomic gas (zeta = 2) only; T is
@@ -21,17 +22,21 @@ import numpy as np
 NAMES = ("U", "V", "W", "P", "T", "Rho", "Numchild", "Numgrandchild", "MFP")
 
 
-def datasets(fields, params):
+def datasets(fields, params, counts=None):
     """fields: dict n, ux, uy, uz, T, p (arrays [N1][N2] or flat) from
-    Run.fields() / Batch.fields(i); params: the case's dsmc_params."""
+    Run.fields() / Batch.fields(i); params: the case's dsmc_params; counts:
+    Numchild and Numgrandchild of the adaptive sub-cells (Run.amr()), else
+    zeros."""
     shape = (params.N1, params.N2)
     g = {k: np.asarray(v, np.float64).reshape(shape) for k, v in fields.items()}
     rho = g["n"] * params.mass
     with np.errstate(divide="ignore", invalid="ignore"):
         mfp = np.where(rho > 0, params.mass / (np.sqrt(2.0) * rho * np.pi * params.diam ** 2)
                        * np.power(np.where(g["T"] > 0, g["T"], 1.0) / params.T, params.Vtl - 0.5), 0.0)
+    nc = np.zeros(shape, np.int32) if counts is None else np.asarray(counts["Numchild"], np.int32).reshape(shape)
+    ng = np.zeros(shape, np.int32) if counts is None else np.asarray(counts["Numgrandchild"], np.int32).reshape(shape)
     ds = dict(U=g["ux"], V=g["uy"], W=g["uz"], P=g["p"], T=g["T"], Rho=rho,
-              Numchild=np.zeros(shape, np.int32), Numgrandchild=np.zeros(shape, np.int32), MFP=mfp)
+              Numchild=nc, Numgrandchild=ng, MFP=mfp)
     if "Trot" in g:
         ds["Trot"] = g["Trot"]
     return ds
@@ -67,10 +72,10 @@ def write_vts(path, ds, params):
         f.write("</CellData>\n</Piece>\n</StructuredGrid>\n</VTKFile>\n")
 
 
-def write_result(directory, iteration, fields, params, vtk=True):
+def write_result(directory, iteration, fields, params, vtk=True, counts=None):
     """res/result_<iter>.npz (+ .vts), the reference's file naming."""
     os.makedirs(directory, exist_ok=True)
-    ds = datasets(fields, params)
+    ds = datasets(fields, params, counts)
     base = os.path.join(directory, f"result_{iteration}")
     write_npz(base + ".npz", ds)
     if vtk:

@@ -53,6 +53,16 @@ def outline_segments(body):
     return np.ascontiguousarray(np.stack([a[:, 0], a[:, 1], b[:, 0], b[:, 1], e[:, 1] / L, -e[:, 0] / L], 1).ravel())
 
 
+def subcell_counts(sub):
+    """Numchild (level-2 cells) and Numgrandchild (level-3 cells in all of
+    them) of packed grids, as the reference's Output counts them
+    (Output.cpp:45-63); 0 for a cell without a grid."""
+    s = np.asarray(sub, dtype=np.uint32)
+    n2 = (s & 0xff) * ((s >> 8) & 0xff)
+    n3 = ((s >> 16) & 0xff) * ((s >> 24) & 0xff)
+    return dict(Numchild=n2.astype(np.int32), Numgrandchild=(n2 * n3).astype(np.int32))
+
+
 class Run:
     def __init__(self, params, body=None, tables=None):
         """body: the body outline (x, y rows); or tables: host-built cut-cell
@@ -199,6 +209,33 @@ class Run:
 
     def set_keep_perm(self, on=True):
         check(self.lib.dsmc_set_keep_perm(self.h, 1 if on else 0))
+
+    # -- adaptive collision sub-cells and the variable time step
+    #    (Cell::genAMRmesh, Cell::VTS, Cell.cpp:117-279) --
+    def set_subcells(self, sub):
+        """Packed per-cell grids (n2x | n2y << 8 | n3x << 16 | n3y << 24; 0:
+        none), or None to switch sub-cells off."""
+        if sub is None:
+            check(self.lib.dsmc_set_subcells(self.h, None))
+            return
+        a = np.ascontiguousarray(sub, dtype=np.uint32)
+        if a.shape != (self.ncell,):
+            raise ValueError("one grid per cell")
+        check(self.lib.dsmc_set_subcells(self.h, ptr(a, C.c_uint32)))
+
+    def amr_update(self):
+        """Grids, mean free paths and VTS time steps from the sampling sums."""
+        check(self.lib.dsmc_amr_update(self.h))
+
+    def amr(self):
+        """dict sub (packed grids), lam (mean free path, m), dt (VTS, s),
+        Numchild and Numgrandchild (the output datasets' counts)."""
+        sub = np.empty(self.ncell, np.uint32)
+        lam = np.empty(self.ncell, np.float32)
+        dt = np.empty(self.ncell, np.float64)
+        check(self.lib.dsmc_download_amr(self.h, ptr(sub, C.c_uint32), ptr(lam, C.c_float),
+                                         ptr(dt, C.c_double)))
+        return dict(sub=sub, lam=lam, dt=dt, **subcell_counts(sub))
 
     def cell_state(self):
         sm = np.empty(self.ncell, np.float32)

@@ -252,6 +252,32 @@ int dsmc_upload_samples(dsmc_handle h, const double *acc, uint64_t nsamples);
  * DSMC_ERR_ARG.  Asynchronous on the handle's stream. */
 int dsmc_sample(dsmc_handle h);
 int dsmc_reset_samples(dsmc_handle h);
+
+/* Adaptive collision sub-cells and the variable time step: Cell::genAMRmesh
+ * (src/cell/Cell.h:55, Cell.cpp:168-279), Element::genAMRmesh
+ * (src/meshes/Element.h:44, Element.cpp:158-182), Cell::VTS (Cell.h:52,
+ * Cell.cpp:117-136) and Cell::collision in the leaves (Cell.cpp:335-352).
+ * A cell's grid is packed n2x | n2y << 8 | n3x << 16 | n3y << 24 -- n2x x n2y
+ * level-2 cells, each with n3x x n3y level-3 cells -- and 0 means none; its
+ * leaves form a uniform (n2x n3x) x (n2y n3y) grid.  From the next step on,
+ * a cell with more than one and at most 256 leaves and 2 .. 2048 particles
+ * orders its particles stably by leaf and collides leaf by leaf with the
+ * leaf volume (fluid volume / leaves).
+ * dsmc_set_subcells: ncell packed grids (NULL: none at all; every level of a
+ * non-zero grid needs at least one sub-cell, else DSMC_ERR_ARG).
+ * dsmc_amr_update: per cell from the sampling sums (time-averaged; at least
+ * one sampled step, else DSMC_ERR_ARG): mean free path
+ * lam = m / (sqrt(2) rho pi d^2) (T/T_inf)^(omega - 1/2), VTS time step
+ * dt = lam / (2 (|u| + sqrt(2kT/m))), and the grid by genAMRmesh's rule --
+ * level 2 of side lam/2, n = max(1, ceil(L/side)), at most 16 per axis;
+ * level 3 from twice the level-2 x side for a cell without a grid, else from
+ * lam -- a cell without gas gets none.  Switches the sub-cells on.
+ * dsmc_download_amr: the grids, mean free paths (m) and VTS steps (s); any
+ * pointer may be NULL.  Zeros before the first update.  Single handles only
+ * (a batched sweep has no sub-cells). */
+int dsmc_set_subcells(dsmc_handle h, const uint32_t *sub);
+int dsmc_amr_update(dsmc_handle h);
+int dsmc_download_amr(dsmc_handle h, uint32_t *sub, float *lam, double *dt);
 int dsmc_download_samples(dsmc_handle h, double *acc, uint64_t *nsamples);
 int dsmc_download_fields(dsmc_handle h, double *n, double *ux, double *uy,
                          double *uz, double *T, double *p);

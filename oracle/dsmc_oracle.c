@@ -459,22 +459,15 @@ void or_scatter(int n, const int *cell, const int *slot, const int *start, int *
 /* acceptance-rejection loop: the same law with a fixed number of draws.     */
 /* Energies are per unit mass: E_c/m = g^2/4 + e1 + e2, g' = 2 sqrt(x E_c/m).*/
 /* ------------------------------------------------------------------------- */
-static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
-                        const int *perm, float *vx, float *vy, float *vz, float *er,
-                        float *sigcmax, float *mrem)
+/* The candidate loop of one collision cell (or leaf): candidates m0 .. m0+M-1
+ * of cell c's Philox stream, pairs among perm[s0 .. s0+np), running
+ * (sigma c_r)max in *smax.  Returns the number of collisions. */
+static int ntc_candidates(const or_case *cs, int c, uint32_t step, int s0, int np, int m0, int M,
+                          const int *perm, float *vx, float *vy, float *vz, float *er, float *smax_io)
 {
-    if (np < 2) return 0;
-    float vol = cs->vol_ntc[c];
-    if (!(vol > 0.0f)) return 0;
-    float smax = sigcmax[c];
-    float e = (float)((long long)np * (long long)(np - 1)) * cs->ntc_coef;
-    e = e * smax;
-    e = e / vol;
-    float tot = e + mrem[c];
-    int M = (int)tot;
-    mrem[c] = tot - (float)M;
+    float smax = *smax_io;
     int nc = 0;
-    for (int m = 0; m < M; ++m) {
+    for (int m = m0; m < m0 + M; ++m) {
         uint32_t ctr[4] = { (uint32_t)m, step, OR_SUB_COLLIDE, 0u }, w[4];
         or_philox(ctr, (uint32_t)c, cs->seed, w);
         int i1 = mulhi_idx(w[0], np);
@@ -517,11 +510,167 @@ static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
             ++nc;
         }
     }
+    *smax_io = smax;
+    return nc;
+}
+
+/* NTC candidate number of np particles in volume vol with remainder carry. */
+static int ntc_count(const or_case *cs, int np, float vol, float smax, float *mrem)
+{
+    float e = (float)((long long)np * (long long)(np - 1)) * cs->ntc_coef;
+    e = e * smax;
+    e = e / vol;
+    float tot = e + *mrem;
+    int M = (int)tot;
+    *mrem = tot - (float)M;
+    return M;
+}
+
+static int collide_cell(const or_case *cs, int c, uint32_t step, int s0, int np,
+                        const int *perm, float *vx, float *vy, float *vz, float *er,
+                        float *sigcmax, float *mrem)
+{
+    if (np < 2) return 0;
+    float vol = cs->vol_ntc[c];
+    if (!(vol > 0.0f)) return 0;
+    float smax = sigcmax[c];
+    int M = ntc_count(cs, np, vol, smax, &mrem[c]);
+    int nc = ntc_candidates(cs, c, step, s0, np, 0, M, perm, vx, vy, vz, er, &smax);
     sigcmax[c] = smax;
     return nc;
 }
 
-void or_collide(const or_case *cs, uint32_t step, const int *start, const int *perm,
+/* ------------------------------------------------------------------------- */
+/* Adaptive collision sub-cells and the variable time step.  Reference:      */
+/* Cell::VTS (Cell.cpp:117-136), Cell::genAMRmesh (Cell.cpp:168-279),        */
+/* Element::genAMRmesh (Element.cpp:158-182), Cell::sortParticle2children    */
+/* (Cell.cpp:280-295), Cell::collision over the leaves (Cell.cpp:335-352)    */
+/* and the Numchild/Numgrandchild datasets (Output.cpp:45-63).  The          */
+/* reference defines these on the host and never calls them.  Restated:      */
+/*  - mean free path lam = m / (sqrt(2) rho pi d^2) (T/T_inf)^(omega-1/2)   */
+/*    (Cell.cpp:125) from the cell's time-averaged sums and its fluid        */
+/*    volume (the reference samples each sub-cell instantaneously, dividing  */
+/*    by the parent's volume); the power as exp/log in FP32 with this        */
+/*    build's own routines, everything else in FP64;                         */
+/*  - VTS: dt = lam / (2 (|u| + sqrt(2kT/m))) (Cell.cpp:128-129);            */
+/*  - level 2: side Ch lam_max, Ch = 1/2, n = max(1, ceil(L1/L2)), side      */
+/*    L1/n (Cell.cpp:174-197); level 3 in every level-2 cell: side           */
+/*    Ch lam_min with lam_min the smallest old level-3 mean free path, or    */
+/*    2 x (level-2 x side) when the cell had no grid (Cell.cpp:217-243).     */
+/*    isIntersecting is always true (Element.cpp), so every level-2 cell of  */
+/*    a cell gets the same level-3 grid and the leaves form one uniform      */
+/*    nsx x nsy grid, nsx = n2x n3x; with one mean free path per cell,       */
+/*    lam_max = lam_min = lam;                                               */
+/*  - leaves: a cell's particles ordered stably by leaf, then NTC in every   */
+/*    leaf with the leaf volume (cell fluid volume / leaves), a running      */
+/*    (sigma c_r)max per leaf from the cell's value (the cell keeps the      */
+/*    largest), the candidate remainder carried from leaf to leaf in leaf    */
+/*    order, and the candidates numbered on through the leaves (one Philox   */
+/*    stream per cell and step as without sub-cells);                        */
+/*  - at most OR_AMR_MAX sub-cells per level and axis; a cell with more      */
+/*    than OR_AMR_CAP particles or more than OR_AMR_LEAVES leaves collides   */
+/*    without sub-cells.                                                     */
+/* ------------------------------------------------------------------------- */
+int or_amr_leaves(uint32_t sub, int *nsx, int *nsy)
+{
+    if (!sub) { *nsx = 1; *nsy = 1; return 1; }
+    *nsx = (int)(sub & 0xffu) * (int)((sub >> 16) & 0xffu);
+    *nsy = (int)((sub >> 8) & 0xffu) * (int)((sub >> 24) & 0xffu);
+    return *nsx * *nsy;
+}
+
+static int amr_div(double L, double side)
+{
+    double q = L / side;
+    if (!(q > 1.0)) return 1;
+    double n = ceil(q);
+    return n > (double)OR_AMR_MAX ? OR_AMR_MAX : (int)n;
+}
+
+uint32_t or_amr_grid(double dx, double dy, double lam_max, double lam_min, int had)
+{
+    int n2x = amr_div(dx, 0.5 * lam_max), n2y = amr_div(dy, 0.5 * lam_max);
+    double l2x = dx / (double)n2x, l2y = dy / (double)n2y;
+    double mn = had ? lam_min : 2.0 * l2x;
+    int n3x = amr_div(l2x, 0.5 * mn), n3y = amr_div(l2y, 0.5 * mn);
+    return (uint32_t)n2x | ((uint32_t)n2y << 8) | ((uint32_t)n3x << 16) | ((uint32_t)n3y << 24);
+}
+
+void or_amr_update(const or_case *cs, const or_amr_gas *g, const double *acc, long long nsamples,
+                   const double *vol, uint32_t *sub, float *lam, double *dt)
+{
+    int C = cs->ncell;
+    const double pi_d2 = sqrt(2.0) * 3.14159265358979323846 * g->diam * g->diam;
+    for (int c = 0; c < C; ++c) {
+        double S0 = acc[c];
+        lam[c] = 0.0f;
+        dt[c] = 0.0;
+        if (!(S0 > 0.0) || !(vol[c] > 0.0) || nsamples <= 0) { sub[c] = 0u; continue; }
+        double n = (S0 / (double)nsamples) * g->Fn / vol[c];
+        double ux = acc[C + c] / S0, uy = acc[2 * C + c] / S0, uz = acc[3 * C + c] / S0;
+        double uu = ux * ux + uy * uy + uz * uz;
+        double T = g->mass / (3.0 * g->boltz) * (acc[4 * C + c] / S0 - uu);
+        float tr = (float)(T / g->T_inf);
+        if (!(tr > 0.0f)) { sub[c] = 0u; continue; }
+        float f = or_expf((float)(g->omega - 0.5) * or_logf(tr));
+        double l = g->mass / (pi_d2 * (n * g->mass)) * (double)f;
+        lam[c] = (float)l;
+        dt[c] = 0.5 * l / (sqrt(uu) + sqrt(2.0 * g->boltz * T / g->mass));
+        sub[c] = or_amr_grid((double)cs->dx, (double)cs->dy, l, l, sub[c] != 0u);
+    }
+}
+
+int or_amr_leaf(const or_case *cs, int c, float x, float y, int nsx, int nsy)
+{
+    int i = c / cs->ny, j = c % cs->ny;
+    float fx = x / cs->dx - (float)i, fy = y / cs->dy - (float)j;
+    int lx = (int)(fx * (float)nsx), ly = (int)(fy * (float)nsy);
+    lx = lx < 0 ? 0 : (lx > nsx - 1 ? nsx - 1 : lx);
+    ly = ly < 0 ? 0 : (ly > nsy - 1 ? nsy - 1 : ly);
+    return ly + lx * nsy;
+}
+
+/* One cell with nsx x nsy leaves: perm[s0 .. s0+np) ordered stably by leaf,
+ * then NTC leaf by leaf. */
+static int collide_cell_leaves(const or_case *cs, int c, uint32_t step, int s0, int np, int nsx, int nsy,
+                               int *perm, const float *x, const float *y, float *vx, float *vy,
+                               float *vz, float *er, float *sigcmax, float *mrem)
+{
+    int nl = nsx * nsy;
+    int *lf = (int *)malloc(sizeof(int) * (size_t)np);
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)np);
+    int *ls = (int *)calloc((size_t)nl + 1, sizeof(int));
+    for (int k = 0; k < np; ++k) {
+        int p = perm[s0 + k];
+        lf[k] = or_amr_leaf(cs, c, x[p], y[p], nsx, nsy);
+        ls[lf[k] + 1] += 1;
+    }
+    for (int l = 0; l < nl; ++l) ls[l + 1] += ls[l];
+    int *pos = (int *)malloc(sizeof(int) * (size_t)nl);
+    for (int l = 0; l < nl; ++l) pos[l] = ls[l];
+    for (int k = 0; k < np; ++k) tmp[pos[lf[k]]++] = perm[s0 + k];
+    memcpy(perm + s0, tmp, sizeof(int) * (size_t)np);
+    float vol = cs->vol_ntc[c];
+    float vleaf = vol / (float)nl;
+    float smax0 = sigcmax[c], best = smax0;
+    int nc = 0, m0 = 0;
+    if (vol > 0.0f)
+        for (int l = 0; l < nl; ++l) {
+            int npl = ls[l + 1] - ls[l];
+            if (npl < 2) continue;
+            int M = ntc_count(cs, npl, vleaf, smax0, &mrem[c]);
+            float sm = smax0;
+            nc += ntc_candidates(cs, c, step, s0 + ls[l], npl, m0, M, perm, vx, vy, vz, er, &sm);
+            m0 += M;
+            if (sm > best) best = sm;
+        }
+    sigcmax[c] = best;
+    free(lf); free(tmp); free(ls); free(pos);
+    return nc;
+}
+
+void or_collide(const or_case *cs, uint32_t step, const int *start, int *perm,
+                const float *x, const float *y,
                 float *vx, float *vy, float *vz, float *er, float *sigcmax, float *mrem,
                 int *ncoll_cell, int nthreads)
 {
@@ -529,9 +678,15 @@ void or_collide(const or_case *cs, uint32_t step, const int *start, const int *p
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 1 ? nthreads : 1) if (nthreads > 1)
 #endif
-    for (int c = 0; c < C; ++c)
-        ncoll_cell[c] = collide_cell(cs, c, step, start[c], start[c + 1] - start[c], perm,
-                                     vx, vy, vz, er, sigcmax, mrem);
+    for (int c = 0; c < C; ++c) {
+        int np = start[c + 1] - start[c], nsx = 1, nsy = 1;
+        int nl = cs->sub ? or_amr_leaves(cs->sub[c], &nsx, &nsy) : 1;
+        if (nl > 1 && nl <= OR_AMR_LEAVES && np >= 2 && np <= OR_AMR_CAP)
+            ncoll_cell[c] = collide_cell_leaves(cs, c, step, start[c], np, nsx, nsy, perm, x, y,
+                                                vx, vy, vz, er, sigcmax, mrem);
+        else
+            ncoll_cell[c] = collide_cell(cs, c, step, start[c], np, perm, vx, vy, vz, er, sigcmax, mrem);
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -757,7 +912,7 @@ int or_step(const or_case *cs, or_state *st, int sample, int nthreads)
     for (int i = 0; i < ntot; ++i)
         if (st->cell[i] >= 0) st->perm[st->start[st->cell[i]] + st->slot[i]] = i;
 
-    or_collide(cs, st->step, st->start, st->perm, st->vx, st->vy, st->vz, st->er,
+    or_collide(cs, st->step, st->start, st->perm, st->x, st->y, st->vx, st->vy, st->vz, st->er,
                st->sigcmax, st->mrem, st->ncoll_cell, nt);
     long long tc = 0;
     for (int c = 0; c < C; ++c) tc += st->ncoll_cell[c];

@@ -80,7 +80,31 @@ typedef struct or_case {
     float rot_p;                       /* fraction of collisions that exchange (1/Z_rot) */
     float lb_ia, lb_ib;                /* 1/(5/2-omega), 1/(7/2-omega) */
     float body_ekt, face_ekt[4], inj_ekt;  /* kT/m of the diffuse walls and the freestream */
+    /* adaptive collision sub-cells per cell, packed n2x | n2y << 8 |
+     * n3x << 16 | n3y << 24 (0: none); NULL: no cell has any */
+    const uint32_t *sub;
 } or_case;
+
+/* ---- adaptive collision sub-cells and the variable time step -------------- */
+#define OR_AMR_MAX 16          /* sub-cells per level and axis, at most */
+#define OR_AMR_CAP 2048        /* cells with more particles collide without sub-cells */
+#define OR_AMR_LEAVES 256      /* and so do cells whose grid has more leaves */
+typedef struct or_amr_gas {
+    double Fn, mass, diam, T_inf, omega, boltz;
+} or_amr_gas;
+/* Leaves of a packed grid (1 for 0). */
+int or_amr_leaves(uint32_t sub, int *nsx, int *nsy);
+/* The two-level grid of one cell of size dx x dy from its mean free path
+ * (lam_max sizes level 2, lam_min level 3) and whether it had a grid. */
+uint32_t or_amr_grid(double dx, double dy, double lam_max, double lam_min, int had);
+/* Per cell from the time-averaged sums acc (nsamples samples) and fluid
+ * volumes vol: mean free path lam, VTS time step dt (0 where the cell has no
+ * gas), and the new grid sub (updated in place: its old value says whether
+ * the cell had one). */
+void or_amr_update(const or_case *cs, const or_amr_gas *g, const double *acc, long long nsamples,
+                   const double *vol, uint32_t *sub, float *lam, double *dt);
+/* Leaf of a particle at (x, y) in cell c with an nsx x nsy leaf grid. */
+int or_amr_leaf(const or_case *cs, int c, float x, float y, int nsx, int nsy);
 
 /* ---- primitives ---------------------------------------------------------- */
 void  or_philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]);
@@ -107,7 +131,11 @@ void or_scan(const int *count, int ncell, int *start /* ncell+1 */);
 void or_scatter(int n, const int *cell, const int *slot, const int *start, int *perm);
 /* NTC/VHS collisions of every cell, through perm; updates velocities (and,
  * zeta = 2, rotational energies er) in place. */
-void or_collide(const or_case *cs, uint32_t step, const int *start, const int *perm,
+/* perm is reordered in place within cells that have sub-cells (cs->sub):
+ * stably by leaf, so each leaf's particles are contiguous; x, y locate the
+ * leaves. */
+void or_collide(const or_case *cs, uint32_t step, const int *start, int *perm,
+                const float *x, const float *y,
                 float *vx, float *vy, float *vz, float *er, float *sigcmax, float *mrem,
                 int *ncoll_cell, int nthreads);
 /* FP64 sampling accumulators: acc as SoA blocks of ncell: n, vx, vy, vz, v^2,

@@ -45,8 +45,16 @@ class or_case(C.Structure):
         ("seed", C.c_uint32),
         ("zeta", C.c_int), ("rot_p", C.c_float), ("lb_ia", C.c_float), ("lb_ib", C.c_float),
         ("body_ekt", C.c_float), ("face_ekt", C.c_float * 4), ("inj_ekt", C.c_float),
+        ("sub", C.POINTER(C.c_uint32)),
     ]
 
+
+class or_amr_gas(C.Structure):
+    _fields_ = [("Fn", C.c_double), ("mass", C.c_double), ("diam", C.c_double), ("T_inf", C.c_double),
+                ("omega", C.c_double), ("boltz", C.c_double)]
+
+
+AMR_MAX, AMR_CAP, AMR_LEAVES = 16, 2048, 256
 
 _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
@@ -100,7 +108,11 @@ def load():
         "or_bin": [P, C.c_int, _F, _F, _B, _I, _I, _I],
         "or_scan": [_I, C.c_int, _I],
         "or_scatter": [C.c_int, _I, _I, _I, _I],
-        "or_collide": [P, C.c_uint32, _I, _I, _F, _F, _F, _F, _F, _F, _I, C.c_int],
+        "or_collide": [P, C.c_uint32, _I, _I, _F, _F, _F, _F, _F, _F, _F, _F, _I, C.c_int],
+        "or_amr_leaves": [C.c_uint32, _I, _I],
+        "or_amr_grid": [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int],
+        "or_amr_update": [P, C.POINTER(or_amr_gas), _D, C.c_longlong, _D, _U, _F, _D],
+        "or_amr_leaf": [P, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int],
         "or_sample": [C.c_int, _I, _I, _F, _F, _F, _F, _I, _D],
         "or_reorder": [C.c_int, _I, _F, _F, _F, _F, _F, _F, _U, _F, _F, _F, _F, _F, _F, _U],
         "or_init_fill": [P, _D, _B, _F, _F, _F, C.c_int, _F, _F, _F, _F, _F, _F, _U, _I, _U],
@@ -109,7 +121,8 @@ def load():
     for name, args in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_int if name in ("or_init_fill", "or_step") else None
+        fn.restype = C.c_int if name in ("or_init_fill", "or_step", "or_amr_leaves", "or_amr_leaf") else None
+    lib.or_amr_grid.restype = C.c_uint32
     _lib = lib
     return lib
 
@@ -182,6 +195,8 @@ class OracleCase:
         c.vol_ntc = _p(self.t["vol_ntc"], C.c_float)
         c.fbox = _p(self.t["fbox"], C.c_float)
         c.fmask = _p(self.t["fmask"], C.c_uint64)
+        c.sub = None
+        self._sub = None
         self.c = c
         self.ptr = C.byref(c)
 
@@ -241,15 +256,44 @@ class OracleCase:
         load().or_scatter(cell.shape[0], _p(cell, C.c_int), _p(slot, C.c_int), _p(start, C.c_int), _p(perm, C.c_int))
         return perm
 
-    def collide(self, step, start, perm, vx, vy, vz, sigcmax, mrem, nthreads=1, er=None):
-        """er: rotational energies (zeta = 2), updated in place; required then."""
+    def collide(self, step, start, perm, vx, vy, vz, sigcmax, mrem, nthreads=1, er=None, x=None, y=None):
+        """er: rotational energies (zeta = 2), updated in place; required then.
+        x, y: positions, required once the case has sub-cells (set_subcells),
+        whose cells' perm segments are then reordered by leaf in place."""
         if self.zeta and er is None:
             raise ValueError("a diatomic case collides with its rotational energies")
+        if self._sub is not None and (x is None or y is None):
+            raise ValueError("collisions in sub-cells need the positions")
         ncoll = np.zeros(self.ncell, np.int32)
-        load().or_collide(self.ptr, step, _p(start, C.c_int), _p(perm, C.c_int), _p(vx, C.c_float),
+        load().or_collide(self.ptr, step, _p(start, C.c_int), _p(perm, C.c_int), _p(x, C.c_float),
+                          _p(y, C.c_float), _p(vx, C.c_float),
                           _p(vy, C.c_float), _p(vz, C.c_float), _p(er if self.zeta else None, C.c_float),
                           _p(sigcmax, C.c_float), _p(mrem, C.c_float), _p(ncoll, C.c_int), nthreads)
         return ncoll
+
+    # --- adaptive collision sub-cells (Cell::genAMRmesh, Cell::VTS) ---
+    def set_subcells(self, sub):
+        """Packed per-cell grids (n2x | n2y << 8 | n3x << 16 | n3y << 24;
+        0 = none), or None for none at all."""
+        self._sub = None if sub is None else np.ascontiguousarray(sub, dtype=np.uint32)
+        self.c.sub = _p(self._sub, C.c_uint32)
+
+    def amr_update(self, gas, acc, nsamples, vol, sub):
+        """gas: dict Fn, mass, diam, T_inf, omega, boltz.  Returns the new
+        packed grids (sub is not modified), mean free paths and VTS steps."""
+        g = or_amr_gas(*(float(gas[k]) for k in ("Fn", "mass", "diam", "T_inf", "omega", "boltz")))
+        acc = np.ascontiguousarray(acc, dtype=np.float64)
+        vol = np.ascontiguousarray(vol, dtype=np.float64)
+        out = np.array(sub, dtype=np.uint32, copy=True)
+        lam = np.zeros(self.ncell, np.float32)
+        dt = np.zeros(self.ncell, np.float64)
+        load().or_amr_update(self.ptr, C.byref(g), _p(acc, C.c_double), int(nsamples), _p(vol, C.c_double),
+                             _p(out, C.c_uint32), _p(lam, C.c_float), _p(dt, C.c_double))
+        return out, lam, dt
+
+    def amr_leaf(self, c, x, y, nsx, nsy):
+        return load().or_amr_leaf(self.ptr, int(c), float(x), float(y), int(nsx), int(nsy))
+
 
     def sample(self, start, perm, vx, vy, vz, ncoll, acc, er=None):
         load().or_sample(self.ncell, _p(start, C.c_int), _p(perm, C.c_int), _p(vx, C.c_float), _p(vy, C.c_float),
@@ -336,3 +380,15 @@ def case_from_params(params, body=None):
     import cudadsmc_amd.run as run
     info, tables = run.describe(params, body)
     return OracleCase(info, tables, (params.N1, params.N2)), info, tables
+
+
+def amr_grid(dx, dy, lam_max, lam_min, had):
+    """Packed two-level grid of one cell (or_amr_grid)."""
+    return int(load().or_amr_grid(float(dx), float(dy), float(lam_max), float(lam_min), 1 if had else 0))
+
+
+def amr_leaves(sub):
+    """(leaves, nsx, nsy) of a packed grid."""
+    a, b = C.c_int(), C.c_int()
+    n = load().or_amr_leaves(int(sub), C.byref(a), C.byref(b))
+    return n, a.value, b.value

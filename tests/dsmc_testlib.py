@@ -56,7 +56,7 @@ def oracle_phases(oc, parts, step, next_pid, sigcmax, mrem, cap):
     mr = mrem.copy()
     vx, vy, vz = allp["vx"].copy(), allp["vy"].copy(), allp["vz"].copy()
     er = allp["er"].copy() if oc.zeta else None
-    ncoll = oc.collide(step, start, perm, vx, vy, vz, sm, mr, er=er)
+    ncoll = oc.collide(step, start, perm, vx, vy, vz, sm, mr, er=er, x=allp["x"], y=allp["y"])
     acc = np.zeros(oc.info.sample_blocks * oc.ncell)
     oc.sample(start, perm, vx, vy, vz, ncoll, acc, er=er)
     new = {k: allp[k][perm] for k in ("x", "y", "pid")}

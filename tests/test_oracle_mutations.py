@@ -19,7 +19,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
 PIN_SUITES = ["tests/test_oracle_pins.py", "tests/test_oracle_walls.py", "tests/test_oracle_step.py",
-              "tests/test_oracle_rotation.py"]
+              "tests/test_oracle_rotation.py", "tests/test_oracle_amr.py"]
 
 # (id, what the mistake is, exact text of the oracle, its replacement)
 MUTATIONS = [
@@ -67,7 +67,7 @@ MUTATIONS = [
      "(float)((long long)np * (long long)(np - 1)) * cs->ntc_coef;",
      "(float)((long long)np * (long long)np) * cs->ntc_coef;"),
     ("ntc_no_remainder", "NTC candidate remainder dropped",
-     "float tot = e + mrem[c];", "float tot = e;"),
+     "float tot = e + *mrem;", "float tot = e;"),
     ("vhs_exponent_sign", "VHS cross-section exponent with the wrong sign",
      "cs->K1 * or_expf(cs->e_exp * or_logf(g2))", "cs->K1 * or_expf(-cs->e_exp * or_logf(g2))"),
     ("sample_no_vz2", "sampled v^2 without the vz term",
@@ -97,6 +97,25 @@ MUTATIONS = [
      "er = rot_energy(w[0], cs->inj_ekt);", "er = rot_energy(w[0], cs->body_ekt);"),
     ("fill_rot_per_dof", "fill rotational energy at kT/2 (one degree of freedom)",
      "er[i] = rot_energy(w[0], sig[c] * sig[c]);", "er[i] = rot_energy(w[0], 0.5f * sig[c] * sig[c]);"),
+    ("amr_leaf_volume_of_cell", "sub-cell candidates with the cell's volume instead of the leaf's",
+     "float vleaf = vol / (float)nl;", "float vleaf = vol;"),
+    ("amr_pairs_across_cell", "sub-cell pairs drawn from the whole cell",
+     "nc += ntc_candidates(cs, c, step, s0 + ls[l], npl, m0, M, perm, vx, vy, vz, er, &sm);",
+     "nc += ntc_candidates(cs, c, step, s0, np, m0, M, perm, vx, vy, vz, er, &sm);"),
+    ("amr_no_leaf_remainder", "sub-cell candidate remainder not carried from leaf to leaf",
+     "int M = ntc_count(cs, npl, vleaf, smax0, &mrem[c]);",
+     "float r0 = 0.0f; int M = ntc_count(cs, npl, vleaf, smax0, &r0);"),
+    ("amr_grid_floor", "sub-cell count rounded down instead of up", "double n = ceil(q);", "double n = floor(q);"),
+    ("amr_level3_from_largest", "level 3 sized by the largest mean free path",
+     "double mn = had ? lam_min : 2.0 * l2x;", "double mn = had ? lam_max : 2.0 * l2x;"),
+    ("amr_mfp_exponent_sign", "mean free path temperature exponent with the wrong sign",
+     "float f = or_expf((float)(g->omega - 0.5) * or_logf(tr));",
+     "float f = or_expf((float)(0.5 - g->omega) * or_logf(tr));"),
+    ("amr_leaf_transposed", "leaf index with the axes transposed",
+     "    return ly + lx * nsy;\n}", "    return lx + ly * nsx;\n}"),
+    ("amr_vts_no_flow_speed", "VTS time step without the flow speed",
+     "dt[c] = 0.5 * l / (sqrt(uu) + sqrt(2.0 * g->boltz * T / g->mass));",
+     "dt[c] = 0.5 * l / sqrt(2.0 * g->boltz * T / g->mass);"),
 ]
 
 
