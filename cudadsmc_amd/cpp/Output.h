@@ -21,7 +21,7 @@ struct FieldSet {
     int N1 = 0, N2 = 0;
     double L1 = 0, L2 = 0;
     std::vector<double> U, V, W, P, T, Rho, MFP;
-    std::vector<int> Numchild, Numgrandchild;   // adaptive sub-cells: none here, all zero
+    std::vector<int> Numchild, Numgrandchild;   // adaptive sub-cells (0: no grid)
     std::vector<double> Trot;                   // rotational temperature (diatomic gas only, else empty)
 };
 

@@ -53,6 +53,8 @@ struct Param {
     int sample_start = 10000;      // first sampled step (< 0: never)
     int sample_every = 0;          // > 0: GPUCells::Sample after every this many steps instead
                                    // (Run.cpp:294-296 samples after steps 0, 100, 200, ...)
+    int amr_every = 0;             // > 0: adaptive collision sub-cells (Cell::genAMRmesh) set from
+                                   // the sampling sums after every this many steps once sampled
     int output_every = 0;          // write result_<iter> every this many steps (0: at the end only)
     std::string out_dir = "./res"; // the reference writes ./res/result_<iter>
     bool graph = true;             // replay captured steps; false: eager phase launches

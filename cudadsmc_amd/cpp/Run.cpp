@@ -86,7 +86,7 @@ Param parse_args(int argc, char **argv)
     std::string geometry = "cylinder";
     double mach = 10.0, kn = 0.01, tw = 2.5, half = 20.0, ppc = 30.0;
     int nx = 400, ny = 200;
-    long steps = -1, sample_from = -2, every = 0, seed = -1, sample_every = 0;
+    long steps = -1, sample_from = -2, every = 0, seed = -1, sample_every = 0, amr = 0;
     std::string out = "./res", gas = "argon";
     double zrot = 1.0;
     bool eager = false, phases = false;
@@ -109,6 +109,7 @@ Param parse_args(int argc, char **argv)
         else if (a == "--sample-from") sample_from = arg_i(val(), "--sample-from");
         else if (a == "--output-every") every = arg_i(val(), "--output-every");
         else if (a == "--sample-every") sample_every = arg_i(val(), "--sample-every");
+        else if (a == "--amr") amr = arg_i(val(), "--amr");
         else if (a == "--out") out = val();
         else if (a == "--eager") eager = true;
         else if (a == "--phases") phases = true;
@@ -134,6 +135,8 @@ Param parse_args(int argc, char **argv)
     p.geometry = geometry;
     if (steps >= 0) p.steps = (int)steps;
     if (sample_every < 0) throw std::invalid_argument("--sample-every must be >= 0");
+    if (amr < 0) throw std::invalid_argument("--amr must be >= 0");
+    p.amr_every = (int)amr;
     p.sample_every = (int)sample_every;
     if (sample_every > 0) p.sample_start = -1;   // the steps' own sampling off: Sample() instead
     if (sample_from >= -1) p.sample_start = (int)sample_from;
@@ -198,6 +201,10 @@ void Run::TransferCellsFromHostToDevice()
 
 void Run::Sample() { check(dsmc_sample(m_h), "dsmc_sample"); }
 
+// The sub-cell grids of every cell from its sampled state (Cell::VTS and
+// Cell::genAMRmesh, Cell.cpp:117-279, which the reference leaves uncalled).
+void Run::genAMRmesh() { check(dsmc_amr_update(m_h), "dsmc_amr_update"); }
+
 void Run::particlemove() { check(dsmc_move(m_h), "dsmc_move"); }
 void Run::ressignParticle() { check(dsmc_reassign(m_h), "dsmc_reassign"); }
 void Run::collision() { check(dsmc_collide(m_h), "dsmc_collide"); }
@@ -230,6 +237,18 @@ void Run::TransferCellsFromDeviceToHost()
     f.MFP.assign(C, 0.0);
     f.Numchild.assign(C, 0);
     f.Numgrandchild.assign(C, 0);
+    {
+        // adaptive sub-cells: level-2 cells and level-3 cells in all of them
+        // (Output.cpp:45-63); zeros without grids
+        std::vector<uint32_t> sub(C);
+        check(dsmc_download_amr(m_h, sub.data(), nullptr, nullptr), "dsmc_download_amr");
+        for (size_t c = 0; c < C; ++c) {
+            const int n2 = (int)(sub[c] & 0xffu) * (int)((sub[c] >> 8) & 0xffu);
+            const int n3 = (int)((sub[c] >> 16) & 0xffu) * (int)((sub[c] >> 24) & 0xffu);
+            f.Numchild[c] = n2;
+            f.Numgrandchild[c] = n2 * n3;
+        }
+    }
     f.Trot.clear();
     if (m_param.zeta) {
         f.Trot.resize(C);
@@ -296,6 +315,14 @@ void Run::solver()
                 check(dsmc_run(m_h, target - iter, p.graph ? 1 : 0), "dsmc_run");
                 iter = target;
                 if ((iter - 1) % se == 0) Sample();
+            }
+        } else if (p.amr_every > 0) {
+            // the grids renewed after every amr_every-th step, once sampled
+            while (iter < next) {
+                const int target = std::min(next, (iter / p.amr_every + 1) * p.amr_every);
+                check(dsmc_run(m_h, target - iter, p.graph ? 1 : 0), "dsmc_run");
+                iter = target;
+                if (iter % p.amr_every == 0 && status().nsamples > 0) genAMRmesh();
             }
         } else {
             check(dsmc_run(m_h, next - iter, p.graph ? 1 : 0), "dsmc_run");

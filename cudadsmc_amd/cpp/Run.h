@@ -37,7 +37,7 @@ public:
     // Options (all optional; the default is the Mach-10 cylinder benchmark):
     //   --case cylinder|wedge|none  --mach M --kn KN --tw RATIO --half-angle DEG
     //   --nx N1 --ny N2 --ppc P --seed S --steps K --sample-from STEP
-    //   --sample-every K --output-every K --out DIR --eager --phases
+    //   --sample-every K --amr K --output-every K --out DIR --eager --phases
     void initialize(int argc, char **argv);
     void initialize(const Param &p);
     /*============ Main loop ============*/
@@ -50,6 +50,7 @@ public:
     void ressignParticle();
     void collision();
     void Sample();
+    void genAMRmesh();
     void TransferCellsFromHostToDevice();
     void TransferCellsFromDeviceToHost();
     const CartesianMesh &mesh() const { return *m_mesh; }

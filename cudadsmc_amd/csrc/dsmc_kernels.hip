@@ -331,8 +331,8 @@ __device__ __forceinline__ void xcd_range(int ntask, int &first, int &end, int &
 
 // ---------------------------------------------------------------------------
 // k_place: every cell's runs listed contiguously -- record k of chunk q goes
-// to run start(cell) + its slot.  A wave takes 64 chunks, scans their run
-// counts and deals the runs out to its lanes, so every lane places a run.
+// to run start(cell) + its slot.  A wave takes up to 64 chunks, scans their
+// run counts and deals the runs out to its lanes, so every lane places a run.
 // Within a cell the list is in the atomics' arrival order, which k_scatter
 // ranks away.
 // ---------------------------------------------------------------------------
@@ -342,11 +342,17 @@ __global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__
 {
     const int lane = lane_id();
     const uint32_t nch = (ctrl->n_move + 63u) >> 6;
+    // chunks per task: 64, halved (down to 4) while that leaves fewer than
+    // four tasks per wave of the grid -- a small case then keeps every wave
+    // busy instead of a few waves running long chains
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t cpw = 64u;
+    while (cpw > 4u && nch / cpw < 4u * waves) cpw >>= 1;
     int first, end, stride;
-    xcd_range((int)((nch + 63u) >> 6), first, end, stride);
+    xcd_range((int)((nch + cpw - 1u) / cpw), first, end, stride);
     for (int task = first; task < end; task += stride) {
-        const uint32_t q0 = (uint32_t)task * 64u;
-        const int nr = q0 + lane < nch ? bn.chunk_runs[q0 + lane] : 0;
+        const uint32_t q0 = (uint32_t)task * cpw;
+        const int nr = (uint32_t)lane < cpw && q0 + lane < nch ? bn.chunk_runs[q0 + lane] : 0;
         const int inc = wave_incl_scan(nr), excl = inc - nr;
         const int total = __shfl(inc, 63, 64);
         // PLACE_U runs per lane per pass, their record loads in flight together
@@ -405,10 +411,12 @@ __device__ __forceinline__ int select_bit(unsigned long long m, int r)
 // cell's runs with smaller id) + (their rank in the mask): perm lists each
 // cell's particles in ascending index order -- the order of the serial
 // binning, bit for bit.  The runs are taken SC_G at a time: each lane ranks
-// one, then the group writes the SC_G runs one after the other, every lane
-// one member (consecutive addresses).  Also materialises start[0..C] and, for
-// every tile of tile_p particles, the first cell that starts in it and that
-// cell's start (tile_first holds int2 pairs).
+// one against the cell's others (shuffles within the group) and writes its
+// members into an LDS window holding the permutation of the wave's cells,
+// which the wave then copies out with consecutive addresses (a window too
+// small for the cells: straight to perm).  Also materialises start[0..C]
+// and, for every tile of tile_p particles, the first cell that starts in it
+// and that cell's start (tile_first holds int2 pairs).
 // ---------------------------------------------------------------------------
 constexpr int SC_G = 16;
 constexpr int SC_WIN = 1024;   // LDS window per wave: the permutation of its cells
@@ -425,15 +433,25 @@ __global__ __launch_bounds__(256) void k_scatter(Bins bn, int *__restrict__ star
     const int ntiles = ((int)n_next + tile_p - 1) / tile_p;
     int first, end, stride;
     xcd_range(ncell / CPW + 1, first, end, stride);
+    // the packed starts of a task's cells (and, for the tile table, of the
+    // cell before each), loaded one task ahead: the next task's loads are in
+    // flight while this one waits for its runs
+    auto starts = [&](int task, unsigned long long &a0, unsigned long long &a1, unsigned long long &ap) {
+        const int cc = task * CPW + g;
+        a0 = cc <= ncell ? packed_start(bn, cc, ncell, n_next, n_runs) : 0ull;
+        a1 = cc < ncell ? packed_start(bn, cc + 1, ncell, n_next, n_runs) : 0ull;
+        ap = (sub == 0 && cc > 0 && cc <= ncell) ? packed_start(bn, cc - 1, ncell, n_next, n_runs) : 0ull;
+    };
+    unsigned long long n0 = 0ull, n1 = 0ull, np0 = 0ull;
+    if (first < end) starts(first, n0, n1, np0);
     for (int task = first; task < end; task += stride) {
         const int c = task * CPW + g;
-        const unsigned long long v0 = c <= ncell ? packed_start(bn, c, ncell, n_next, n_runs) : 0ull;
-        const unsigned long long v1 = c < ncell ? packed_start(bn, c + 1, ncell, n_next, n_runs) : v0;
+        const unsigned long long v0 = n0, v1 = c < ncell ? n1 : v0, vp = np0;
         const int st = (int)(uint32_t)v0, rs = (int)(v0 >> 32);
         const int L = (int)(v1 >> 32) - rs;
         if (sub == 0 && c <= ncell) {
             start[c] = st;
-            const int tlo = c > 0 ? (int)(uint32_t)packed_start(bn, c - 1, ncell, n_next, n_runs) / tile_p + 1 : 0;
+            const int tlo = c > 0 ? (int)(uint32_t)vp / tile_p + 1 : 0;
             const int thi = min(st / tile_p, ntiles - 1);
             for (int t = tlo; t <= thi; ++t) reinterpret_cast<int2 *>(tile_first)[t] = make_int2(c, st);
         }
@@ -445,6 +463,7 @@ __global__ __launch_bounds__(256) void k_scatter(Bins bn, int *__restrict__ star
             Lw = max(Lw, __shfl_xor(Lw, d, 64));
             P1 = max(P1, __shfl_xor(P1, d, 64));
         }
+        if (task + stride < end) starts(task + stride, n0, n1, np0);
         const int P0 = __shfl(st, 0, 64);
         const bool staged = P1 - P0 <= SC_WIN;
         for (int b0 = 0; b0 < Lw; b0 += SC_G) {

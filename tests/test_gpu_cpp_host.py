@@ -107,3 +107,30 @@ def test_cpp_nitrogen_equals_python_run(gpu_lib, tmp_path):
     np.testing.assert_allclose(vts["Trot"], ds["Trot"], rtol=1e-8)
     assert np.nanmax(ds["Trot"]) > 500.0          # rotational energy heated in the shock layer
     r.close()
+
+
+def test_cpp_subcells_equal_python_run(gpu_lib, tmp_path):
+    """--amr 50: the driver renews the sub-cell grids after every 50th step
+    once sampled (dsmc_amr_update, Cell::genAMRmesh); its fields and
+    Numchild/Numgrandchild equal a Python run doing the same."""
+    _run(["--amr", "50"], tmp_path / "amr")
+    case = json.load(open(tmp_path / "amr" / "case.json"))
+    p = cases.make_params(**case["params"])
+    body = np.array(case["body"], np.float64)
+    r = Run(p, body)
+    r.initialize()
+    step = 0
+    for it in (150, 300):
+        while step < it:
+            r.solver(50)
+            step += 50
+            if r.status().nsamples > 0:
+                r.amr_update()
+        a = r.amr()
+        ds = output.datasets(r.fields(), p, a)
+        got = _load(tmp_path / "amr" / f"result_{it}.npz")
+        for k in ("U", "V", "W", "P", "T", "Rho", "Numchild", "Numgrandchild"):
+            np.testing.assert_array_equal(got[k], ds[k], err_msg=f"step {it} {k}")
+    assert (ds["Numchild"] > 1).sum() > 100
+    r.close()
+
