@@ -9,8 +9,8 @@ state of every candidate must equal the first candidate's, bit for bit,
 or the tuner raises.
 
 Search: coordinate descent, one setting at a time in the order tile size
-(k_place + k_scatter + k_cell), k_cell blocks, k_move blocks, the blocks of
-k_place and k_scatter,
+(k_place + k_rank + k_cell), k_cell blocks, k_move blocks, the blocks of
+k_place and k_rank,
 graph length; each keeps the best value found before moving on.
 """
 import hashlib
@@ -142,10 +142,10 @@ def tune(target, cus, steps=8, graph_steps_measure=64, log=print):
                 log(f"[tune] {key}={v}: {c:.4f} ms")
         best[key] = min(scores)[1]
 
-    sweep("cell_tile", TILES, lambda t: t["k_cell"] + t["k_place+k_scatter"])
+    sweep("cell_tile", TILES, lambda t: t["k_cell"] + t["k_place+k_rank"])
     sweep("cell_blocks", [cus * k for k in CELL_BLOCKS_PER_CU], lambda t: t["k_cell"])
     sweep("move_blocks", [cus * k for k in MOVE_BLOCKS_PER_CU], lambda t: t["k_move"])
-    sweep("scatter_blocks", [cus * k for k in SCATTER_BLOCKS_PER_CU], lambda t: t["k_place+k_scatter"])
+    sweep("scatter_blocks", [cus * k for k in SCATTER_BLOCKS_PER_CU], lambda t: t["k_place+k_rank"])
     # graph length: wall time of replays; the number of timed steps is a
     # multiple of every candidate so that each replays whole graphs only
     gscores = []

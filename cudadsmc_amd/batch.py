@@ -183,5 +183,5 @@ class Batch:
         ms, by = np.zeros(4), np.zeros(4)
         n = C.c_int64()
         check(self.lib.dsmc_batch_kernel_times(self.h, ptr(ms, C.c_double), ptr(by, C.c_double), C.byref(n)))
-        names = ("k_move", "k_scan_runs", "k_place+k_scatter", "k_cell")
+        names = ("k_move", "k_scan_runs", "k_place+k_rank", "k_cell")
         return {k: dict(ms=float(m), bytes=float(b)) for k, m, b in zip(names, ms, by)}, n.value

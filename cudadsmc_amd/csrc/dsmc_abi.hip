@@ -39,7 +39,7 @@ struct dsmc_handle_s {
     int ncell = 0, ntile_scan = 0;
     int64_t cap = 0;
     SoA A{}, B{};
-    Bins bn{};                   // the step's binning (k_move, k_scan_runs, k_place, k_scatter)
+    Bins bn{};                   // the step's binning (k_move, k_scan_runs, k_place, k_rank)
     int *perm = nullptr, *count = nullptr, *lstart = nullptr, *tsum = nullptr, *toff = nullptr;   // + fill
     int *start = nullptr, *tile_first = nullptr, *ncoll_cell = nullptr;
     float *sigcmax = nullptr, *mrem = nullptr;
@@ -94,11 +94,11 @@ int launch_step(dsmc_handle h, int phase_mask)
     if (phase_mask & 1) HIPCHK(launch_move(h->cs, h->A, h->B, h->bn, h->ctrl, h->grid_move, st));
     if (phase_mask & 2) {
         HIPCHK(launch_scan_runs(h->bn, h->ncell, h->ctrl, st));
-        HIPCHK(launch_scatter(h->bn, h->start, h->perm, h->tile_first, h->ncell, h->tile_p, h->ctrl,
+        HIPCHK(launch_scatter(h->bn, h->start, h->tile_first, h->ncell, h->tile_p, h->ctrl,
                               h->grid_scatter, st));
     }
     if (phase_mask & 4) {
-        HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->tile_first, h->sigcmax, h->mrem,
+        HIPCHK(launch_cell(h->cs, h->A, h->B, h->start, h->perm, h->bn.runlist, h->tile_first, h->sigcmax, h->mrem,
                            h->ncoll_cell, h->acc, h->ctrl, h->P.in.sample_start, h->tile_p,
                            h->grid_cell, st));
         if (h->cs.sub)
@@ -281,8 +281,8 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
         return bad(e, "binning buffers");
     if ((e = h->alloc(&h->perm, N)) || (e = h->alloc(&h->count, C)) ||
         (e = h->alloc(&h->lstart, C)) || (e = h->alloc(&h->tsum, h->ntile_scan)) ||
-        (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, C + 1)) ||
-        (e = h->alloc(&h->tile_first, 2 * (size_t)ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
+        (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, 2 * ((size_t)C + 1))) ||
+        (e = h->alloc(&h->tile_first, 4 * (size_t)ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
         (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, (size_t)P.sample_blocks * C)) ||
         (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
         (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||
@@ -535,13 +535,15 @@ int build_graph(dsmc_handle h)
 }
 
 // Algorithmic bytes of the four timing slots (k_move; k_scan_runs; k_place +
-// k_scatter; k_cell) over the K steps between two reads of the counters --
+// k_rank; k_cell) over the K steps between two reads of the counters --
 // what each must read and write at least, per particle, run, candidate,
 // collision and cell -- added to the timing totals.  Sums over the steps:
 // live at the start (the particle-steps), injected, moved (live + injected),
 // live at the end, runs.  A run record is 16 B (written by k_move, read and
-// written again by k_place, read by k_scatter), a packed cell counter or
-// start 8 B, a chunk's run count 4 B per 64 moved particles.  A diatomic gas
+// written again by k_place, read by k_rank, its base written back 4 B, read
+// again by k_cell, which expands it into the tile's permutation), a packed
+// cell counter 8 B, a cell's particle and run start 4 B each, a chunk's run
+// count 4 B per 64 moved particles.  A diatomic gas
 // adds its rotational energy: written for every injected particle by the
 // move, gathered and written back by the cell kernel, one more sample block.
 void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
@@ -556,8 +558,8 @@ void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
     const double R = (double)(after.nruns - before.nruns);            // (cell, wave) runs binned
     h->t_bytes[0] += 40.0 * snl + (24.0 + 4.0 * zr) * sJ + (4.0 + 1.0 / 16.0) * snm + 16.0 * R + 8.0 * C * k;
     h->t_bytes[1] += 16.0 * C * k;
-    h->t_bytes[2] += 48.0 * R + snm / 16.0 + 12.0 * C * k + 4.0 * snn;
-    h->t_bytes[3] += (52.0 + 8.0 * zr) * snn + 32.0 * (double)(after.ncand - before.ncand) +
+    h->t_bytes[2] += 52.0 * R + snm / 16.0 + 16.0 * C * k;
+    h->t_bytes[3] += (48.0 + 8.0 * zr) * snn + 16.0 * R + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * k +
                      (S > 0 ? 16.0 * snn * (S / k) + (96.0 + 16.0 * zr) * C * S : 0.0);
 }
@@ -610,7 +612,7 @@ int timed_steps(dsmc_handle h, int nsteps)
         HIPCHK(hipEventRecord(ev[1], st));
         HIPCHK(launch_scan_runs(h->bn, h->ncell, h->ctrl, st));
         HIPCHK(hipEventRecord(ev[2], st));
-        HIPCHK(launch_scatter(h->bn, h->start, h->perm, h->tile_first, h->ncell, h->tile_p, h->ctrl,
+        HIPCHK(launch_scatter(h->bn, h->start, h->tile_first, h->ncell, h->tile_p, h->ctrl,
                               h->grid_scatter, st));
         HIPCHK(hipEventRecord(ev[3], st));
         if ((r = launch_step(h, 4))) return r;
@@ -687,8 +689,9 @@ int dsmc_status_get(dsmc_handle h, dsmc_status *o)
 
 int dsmc_set_keep_perm(dsmc_handle h, int on)
 {
-    // k_scatter writes the stable permutation itself and k_cell leaves it as
-    // it is, so perm is always kept; the setting is accepted and unused
+    // the stable permutation is expanded from the ranked runs, which stay
+    // until the next binning, so it is always available; the setting is
+    // accepted and unused
     (void)on;
     if (!h) return fail(DSMC_ERR_ARG, "null handle");
     return DSMC_OK;
@@ -746,8 +749,16 @@ int dsmc_download_perm(dsmc_handle h, int64_t cap, int *perm)
     if (r) return r;
     Ctrl c;
     if ((r = read_ctrl(h, c))) return r;
-    if ((int64_t)c.n_live > cap) return fail(DSMC_ERR_ARG, "perm buffer too small");
-    if (c.n_live) HIPCHK(hipMemcpy(perm, h->perm, sizeof(int) * c.n_live, hipMemcpyDeviceToHost));
+    if ((int64_t)c.n_next > cap) return fail(DSMC_ERR_ARG, "perm buffer too small");
+    // the permutation k_cell expands from the runs: run (id, base, lane mask)
+    // holds particles 64 (id / 64) + (the mask's set bits) at positions base, ...
+    std::vector<int4> runs(c.n_runs);
+    if (c.n_runs) HIPCHK(hipMemcpy(runs.data(), h->bn.runlist, sizeof(int4) * c.n_runs, hipMemcpyDeviceToHost));
+    for (const int4 &e : runs) {
+        uint64_t m = ((uint64_t)(uint32_t)e.w << 32) | (uint32_t)e.z;
+        for (int r = 0; m; ++r, m &= m - 1)
+            if (e.y + r < (int64_t)c.n_next) perm[e.y + r] = (e.x & ~63) + __builtin_ctzll(m);
+    }
     return DSMC_OK;
 }
 

@@ -90,11 +90,11 @@ int launch_step(dsmc_batch b, int mask)
                                  b->grid_move, st));
     if (mask & 2) HIPCHK(launch_scan_runs(b->bn, b->ncell, b->ctrl, st));
     if (mask & 4)
-        HIPCHK(launch_scatter(b->bn, b->start, b->perm, b->tile_first, b->ncell, b->tile_p, b->ctrl,
+        HIPCHK(launch_scatter(b->bn, b->start, b->tile_first, b->ncell, b->tile_p, b->ctrl,
                               b->grid_scatter, st));
     if (mask & 8)
         HIPCHK(launch_cell_batch(b->all, b->d_cc, b->d_cbase, b->ninst, b->ictl, b->A, b->B, b->start, b->perm,
-                                 b->tile_first, b->sigcmax, b->mrem, b->ncoll_cell, b->acc, b->ctrl, b->tile_p,
+                                 b->bn.runlist, b->tile_first, b->sigcmax, b->mrem, b->ncoll_cell, b->acc, b->ctrl, b->tile_p,
                                  b->grid_cell, st));
     return DSMC_OK;
 }
@@ -309,8 +309,8 @@ int setup_device(dsmc_batch b, dsmc_batch *out)
         return bad(e, "binning buffers");
     if ((e = b->alloc(&b->perm, N)) || (e = b->alloc(&b->count, Cs)) ||
         (e = b->alloc(&b->lstart, Cs)) || (e = b->alloc(&b->tsum, b->ntile_scan)) ||
-        (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, Cs + 1)) ||
-        (e = b->alloc(&b->tile_first, 2 * (size_t)ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
+        (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, 2 * ((size_t)Cs + 1))) ||
+        (e = b->alloc(&b->tile_first, 4 * (size_t)ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
         (e = b->alloc(&b->sigcmax, Cs)) || (e = b->alloc(&b->mrem, Cs)) || (e = b->alloc(&b->acc, 6 * Cs)) ||
         (e = b->alloc(&b->ctrl, 1)) || (e = b->alloc(&b->fill_ctrl, 1)) || (e = b->alloc(&b->ictl, n)) ||
         (e = b->alloc(&b->d_cases, n)) || (e = b->alloc(&b->d_cc, n)) || (e = b->alloc(&b->d_cbase, n + 1)) ||
@@ -511,8 +511,8 @@ int timed_steps(dsmc_batch b, int nsteps)
     }
     b->t_bytes[0] += 40.0 * snl + 24.0 * sJ + (4.0 + 1.0 / 16.0) * snm + 16.0 * R + 8.0 * C * K;
     b->t_bytes[1] += 16.0 * C * K;
-    b->t_bytes[2] += 48.0 * R + snm / 16.0 + 12.0 * C * K + 4.0 * snn;
-    b->t_bytes[3] += 52.0 * snn + 32.0 * (double)(after.ncand - before.ncand) +
+    b->t_bytes[2] += 52.0 * R + snm / 16.0 + 16.0 * C * K;
+    b->t_bytes[3] += 48.0 * snn + 16.0 * R + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * K + sampled;
     b->t_launches += nsteps;
     return DSMC_OK;

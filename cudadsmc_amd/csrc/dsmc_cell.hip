@@ -3,9 +3,11 @@
 // One block (4 waves of 64) takes a tile of consecutive cells whose particles
 // occupy [p0, p0+n) of the cell-sorted order; tiles of about CELL_P particles
 // are handed out by a device-wide queue (n <= CELL_CAP: staged in LDS;
-// larger tiles stage in the destination arrays in HBM).  The permutation
-// already lists every cell's particles in ascending index order (k_scatter
-// ranks the k_move runs of each cell), so per tile:
+// larger tiles stage in the destination arrays in HBM).  Per tile:
+//   0. the tile's k_move runs, ranked within their cells by k_rank (each
+//      run's destination base), expanded into the permutation: every cell's
+//      particles in ascending index order (the order of the serial binning,
+//      sortParticles, GPUParticle.cu:365-373);
 //   1. gather of the state into cell order -- the outflow compaction of
 //      GPUParticles::DeleteInvalid (cudaDSMC GPUParticle.cu:76-140);
 //   2. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
@@ -304,9 +306,10 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
 struct CellArgs {
     DevCase cs;              // batched runs: ncell = all cells, vol_ntc = all cells
     SoA A, B;
-    const int *start;
-    int *perm;
-    const int *tile_first;   // int2 pairs (first cell, its start) per tile
+    const int *start;        // [0, C]: particle starts; [C+1, 2C+1]: run starts
+    int *perm;               // a single cell above the LDS capacity: its permutation
+    const int4 *runlist;     // per run, in cell order: id (64 chunk + k), destination base, lane mask
+    const int *tile_first;   // int4 (first cell, its start, its run start, 0) per tile
     float *sigcmax, *mrem;
     int *ncoll_cell;
     double *acc;
@@ -430,40 +433,39 @@ struct KBatch {
     }
 };
 
-// One tile: cells [ca, cb), particles [p0, p0+n).  LDS selects where the
-// permutation, velocities and (ROT: diatomic gas) rotational energies are
-// staged (block LDS, or HBM in place).
+// One tile: cells [ca, cb), particles [p0, p0+n), runs [ra, rb).  LDS
+// selects where the permutation, velocities and (ROT: diatomic gas)
+// rotational energies are staged (block LDS, or HBM in place).
 template <bool LDS, bool ROT, class KF>
 __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const SoA &nxt, uint32_t step,
-                          int ca, int cb, int p0, int n, int *Sp, float *Vx, float *Vy, float *Vz, float *Er,
-                          CellShared &sh, uint32_t &tcoll, uint32_t &tcand, int *lmarks, int lmark_cap)
+                          int ca, int cb, int p0, int n, int ra, int rb, int *Sp, float *Vx, float *Vy, float *Vz,
+                          float *Er, CellShared &sh, uint32_t &tcoll, uint32_t &tcand, int *lmarks, int lmark_cap)
 {
     const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
     const int nwaves = bs >> 6;
     const DevCase &cs = a.cs;
-    // a tile of one chunk of cells loads its permutation, cell starts and
+    // 0. the permutation of the tile, from its runs: run (id, base, lane
+    // mask) holds the particles 64 (id / 64) + (the mask's set bits), in
+    // ascending order from position base.  Each run is expanded by P threads,
+    // one 64/P-bit piece of its mask each (P = 4, 2 or 1, as many as one pass
+    // of the block's threads allows), which bounds the serial writes per
+    // thread.  A tile of one chunk of cells loads its runs, cell starts and
     // collision state in one round trip (the metadata stays staged until the
-    // collisions); larger tiles stage each chunk's metadata at the collisions
-    // The loads of this stage are issued before any is waited for (a loop
-    // that stores each value to LDS as it arrives waits once per value):
-    // the permutation in rounds of PV per thread, the first round together
-    // with the cell metadata.
+    // collisions); larger tiles stage each chunk's metadata at the
+    // collisions.
     const bool one = cb - ca <= CELL_MAXC;
     {
-        constexpr int PV = 4;
-        const int nc = cb - ca;
-        for (int k0 = 0; k0 < (LDS ? n : 0) || k0 == 0; k0 += PV * CELL_THREADS) {
-            int pv[PV];
-            if (LDS) {
-#pragma unroll
-                for (int j = 0; j < PV; ++j) {
-                    const int k = k0 + tid + j * CELL_THREADS;
-                    pv[j] = k < n ? a.perm[p0 + k] : 0;
-                }
-            }
+        const int nc = cb - ca, nr = rb - ra;
+        const int lp = nr <= CELL_THREADS / 4 ? 2 : nr <= CELL_THREADS / 2 ? 1 : 0;   // log2 P
+        const int pbits = 64 >> lp;
+        for (int i0 = 0; i0 < (nr << lp) || i0 == 0; i0 += CELL_THREADS) {
+            const int item = i0 + tid;
+            const bool has = item < (nr << lp);
+            int4 e = make_int4(0, 0, 0, 0);
+            if (has) e = a.runlist[ra + (item >> lp)];
             int st0 = 0;
             float sm0 = 0.0f, mr0 = 0.0f, vol0 = 0.0f;
-            if (one && k0 == 0) {
+            if (one && i0 == 0) {
                 if (tid <= nc) st0 = a.start[ca + tid];
                 if (tid < nc) {
                     sm0 = a.sigcmax[ca + tid];
@@ -472,14 +474,20 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 }
                 kf.prepare(sh, ca, nc);
             }
-            if (LDS) {
-#pragma unroll
-                for (int j = 0; j < PV; ++j) {
-                    const int k = k0 + tid + j * CELL_THREADS;
-                    if (k < n) Sp[k] = pv[j];
-                }
+            if (has) {
+                const int q0 = e.x & ~63;
+                const unsigned long long m = ((unsigned long long)(uint32_t)e.w << 32) | (uint32_t)e.z;
+                const int lo = (item & ((1 << lp) - 1)) * pbits;          // this thread's piece: bits [lo, lo + pbits)
+                const unsigned long long below = (1ull << lo) - 1ull;     // lo < 64
+                unsigned long long mine = pbits == 64 ? m : m & (((1ull << pbits) - 1ull) << lo);
+                const int b = e.y - p0 + __popcll(m & below);
+                // a run always lies inside its tile; the check keeps a
+                // damaged run table from writing outside the staging
+                if (b < 0 || b + __popcll(mine) > n) mine = 0ull;
+                int *const o = Sp + b;
+                for (int r = 0; mine; ++r, mine &= mine - 1ull) o[r] = q0 + (int)__builtin_ctzll(mine);
             }
-            if (one && k0 == 0) {
+            if (one && i0 == 0) {
                 if (tid <= nc) sh.st[tid] = st0 - p0;
                 if (tid < nc) {
                     sh.sm[tid] = sm0;
@@ -489,6 +497,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             }
         }
     }
+    if (!LDS) __threadfence();
     __syncthreads();
 
     // 1. gather into cell order, GATHER_UNROLL particles per thread in flight
@@ -685,13 +694,15 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
         if (tid == 0) { bsh.w0 = 0; bsh.wn = 0; }
     }
     uint32_t tcoll = 0, tcand = 0;
-    const int2 *tiles = reinterpret_cast<const int2 *>(a.tile_first);
+    const int4 *tiles = reinterpret_cast<const int4 *>(a.tile_first);
     int rest_c = 0, rest_cb = 0;      // the part of a cut tile still to do
     for (int round = 0;; ++round) {
-        int ca, cb, p0, n;
+        int ca, cb, p0, n, ra, rb;
+        const int *const rstart = a.start + a.cs.ncell + 1;
         if (rest_c < rest_cb) {
             ca = rest_c; cb = rest_cb;
             p0 = a.start[ca]; n = a.start[cb] - p0;
+            ra = rstart[ca]; rb = rstart[cb];
         } else {
             // dynamic tile queue: dense tiles cost more, blocks take the
             // next free one (each block's first tile is its own index, no
@@ -707,12 +718,13 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
             }
             if (q >= ntiles) break;
             const int t = ntiles - 1 - q;
-            // tile t: cells [ca, cb), particles [p0, p0 + n), from the
-            // scatter's table of (first cell, its start) pairs
-            const int2 tf = tiles[t];
-            const int2 tn = (t + 1 < ntiles) ? tiles[t + 1] : make_int2(a.cs.ncell, n_next);
+            // tile t: cells [ca, cb), particles [p0, p0 + n), runs [ra, rb),
+            // from the table of (first cell, its start, its run start)
+            const int4 tf = tiles[t];
+            const int4 tn = (t + 1 < ntiles) ? tiles[t + 1] : make_int4(a.cs.ncell, n_next, (int)ctrl->n_runs, 0);
             ca = tf.x; cb = tn.x;
             p0 = tf.y; n = tn.y - p0;
+            ra = tf.z; rb = tn.z;
         }
         rest_c = rest_cb = 0;
         if (ca >= cb) continue;
@@ -726,17 +738,18 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
             const int fit = __syncthreads_count(tid >= 1 && tid <= m && sh.st[tid] - p0 <= CELL_CAP);
             const int c2 = ca + (fit > 0 ? fit : 1);
             n = sh.st[c2 - ca] - p0;
+            rb = rstart[c2];
             __syncthreads();
             rest_c = c2; rest_cb = cb;
             cb = c2;
         }
         auto run = [&](const auto &kf) {
             if (n <= CELL_CAP)
-                cell_tile<true, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, s_perm, s_vx, s_vy, s_vz, s_er, sh, tcoll,
-                                     tcand, nullptr, 0);
+                cell_tile<true, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, ra, rb, s_perm, s_vx, s_vy, s_vz, s_er, sh,
+                                     tcoll, tcand, nullptr, 0);
             else    // a single cell: its collision rounds alone use the idle staging LDS for marks
-                cell_tile<false, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, a.perm + p0, nxt.vx + p0, nxt.vy + p0,
-                                      nxt.vz + p0, ROT ? nxt.er + p0 : nullptr, sh, tcoll, tcand, stage.i,
+                cell_tile<false, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, ra, rb, a.perm + p0, nxt.vx + p0,
+                                      nxt.vy + p0, nxt.vz + p0, ROT ? nxt.er + p0 : nullptr, sh, tcoll, tcand, stage.i,
                                       4 * CELL_CAP);
         };
         if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step, a.cc, a.ictl });
@@ -1085,11 +1098,11 @@ __global__ __launch_bounds__(256) void k_amr_update(DevCase cs, AmrGas g, const 
 }  // namespace
 
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
-                       const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
+                       const int4 *runlist, const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
                        double *acc, Ctrl *ctrl, int sample_start, int tile_p, int grid, hipStream_t st)
 {
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
-    CellArgs a{ cs, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
+    CellArgs a{ cs, A, B, start, perm, runlist, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, sample_start,
                 nullptr, nullptr, 0, nullptr, tile_p };
     if (cs.zeta != 0) {
         if (!A.er || !B.er) return hipErrorInvalidValue;
@@ -1102,12 +1115,12 @@ hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int 
 
 hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
                              InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
-                             const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
+                             const int4 *runlist, const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
                              Ctrl *ctrl, int tile_p, int grid, hipStream_t st)
 {
     if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
-    CellArgs a{ all, A, B, start, perm, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, cc, cbase, ninst,
+    CellArgs a{ all, A, B, start, perm, runlist, tile_first, sigcmax, mrem, ncoll_cell, acc, ctrl, -1, cc, cbase, ninst,
                 ictl, tile_p };
     hipLaunchKernelGGL((k_cell<true, false>), dim3(grid), dim3(CELL_THREADS), 0, st, a);
     return hipGetLastError();

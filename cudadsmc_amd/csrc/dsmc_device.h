@@ -97,15 +97,17 @@ struct Ctrl {
 // particles, a "chunk") by cell: each group is a run, counted with one 64-bit
 // atomic (runs << 32 | particles) per cell and recorded, compacted per chunk,
 // as (cell, its run slot in the cell, lane mask).  One scan covers both
-// counts; k_place lists every cell's runs contiguously, and k_scatter ranks
-// a cell's runs by chunk, which gives the stable (ascending index) order.
+// counts; k_place lists every cell's runs contiguously, k_rank ranks a
+// cell's runs by chunk and gives each its destination base, and k_cell
+// expands the runs of its tile into the permutation (every particle at its
+// run's base + its rank in the run): the stable (ascending index) order.
 struct Bins {
     unsigned long long *cnt;       // per cell, zeroed again by the scan
     unsigned long long *lstart;    // per cell: exclusive scan of cnt within its scan tile
     unsigned long long *tsum, *toff;   // per scan tile: total, and exclusive scan of the totals
     int4 *runrec;                  // chunk q's runs at [64q, 64q + chunk_runs[q]): cell, slot, mask lo, hi
     int *chunk_runs;               // runs per chunk
-    int4 *runlist;                 // runs in cell order: run id (64q + k), unused, mask lo, hi
+    int4 *runlist;                 // runs in cell order: run id (64q + k), destination base (k_rank), mask lo, hi
 };
 
 // Batched runs: every instance keeps its own counters (the global Ctrl holds

@@ -16,8 +16,8 @@ constexpr int CELL_WAVES = 4;                // waves per k_cell block
 constexpr int SAMPLE_G = 8;                  // lanes per cell in the FP64 sampling sums (k_cell, k_sample)
 
 // The step's binning: k_move records runs into bn; launch_scan_runs scans
-// the packed counts; launch_scatter runs k_place and k_scatter (perm, start,
-// tile_first).
+// the packed counts; launch_scatter runs k_place and k_rank (run bases in
+// runlist, start tables, tile table).
 hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, const Bins &bn, Ctrl *ctrl, int grid,
                        hipStream_t st);
 // Many instances in one launch: cases[ninst] and cbase[ninst+1] in device
@@ -25,17 +25,17 @@ hipError_t launch_move(const DevCase &cs, const SoA &A, const SoA &B, const Bins
 hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, const SoA &A, const SoA &B,
                              const Bins &bn, Ctrl *ctrl, InstCtrl *ictl, int grid, hipStream_t st);
 hipError_t launch_scan_runs(const Bins &bn, int ncell, Ctrl *ctrl, hipStream_t st);
-hipError_t launch_scatter(const Bins &bn, int *start, int *perm, int *tile_first, int ncell, int tile_p,
-                          const Ctrl *ctrl, int grid, hipStream_t st);
+hipError_t launch_scatter(const Bins &bn, int *start, int *tile_first, int ncell, int tile_p, const Ctrl *ctrl,
+                          int grid, hipStream_t st);
 // Plain exclusive scan of int counts (the fill and dsmc_bin_sort).
 hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell, Ctrl *ctrl,
                        hipStream_t st);
 hipError_t launch_cell(const DevCase &cs, const SoA &A, const SoA &B, const int *start, int *perm,
-                       const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
+                       const int4 *runlist, const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell,
                        double *acc, Ctrl *ctrl, int sample_start, int tile_p, int grid, hipStream_t st);
 hipError_t launch_cell_batch(const DevCase &all, const CellConst *cc, const int *cbase, int ninst,
                              InstCtrl *ictl, const SoA &A, const SoA &B, const int *start, int *perm,
-                             const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
+                             const int4 *runlist, const int *tile_first, float *sigcmax, float *mrem, int *ncoll_cell, double *acc,
                              Ctrl *ctrl, int tile_p, int grid, hipStream_t st);
 // Adaptive collision sub-cells: after launch_cell, the cells with sub-cells
 // (cs.sub) collide and sample in leaves; launch_amr_update sets the grids,

@@ -8,10 +8,12 @@
 //   k_scan_runs  exclusive scan of the per-cell particle and run counts,
 //                two-level, last-block finish      (GPUCell.cu:104-120)
 //   k_place      every cell's runs listed contiguously
-//   k_scatter    per cell, runs ranked by wave: the cell-sorted permutation
-//                in ascending index order (the stable order), the start
-//                table and the k_cell tile boundaries (GPUParticle.cu:365-373)
-//   k_cell       per tile of cells: gather into cell order (this is also the
+//   k_rank       per cell, runs ranked by wave: each run's destination base
+//                (the stable order: ascending particle index within a cell),
+//                the start tables and the k_cell tile boundaries
+//                                                  (GPUParticle.cu:365-373)
+//   k_cell       per tile of cells: the tile's runs expanded into the
+//                cell-sorted permutation, gather into cell order (this is also the
 //                outflow compaction), NTC/VHS collisions, FP64 sampling
 //                                                  (GPUCell.cu:200-402,
 //                                                   GPUParticle.cu:76-140)
@@ -333,7 +335,7 @@ __device__ __forceinline__ void xcd_range(int ntask, int &first, int &end, int &
 // k_place: every cell's runs listed contiguously -- record k of chunk q goes
 // to run start(cell) + its slot.  A wave takes up to 64 chunks, scans their
 // run counts and deals the runs out to its lanes, so every lane places a run.
-// Within a cell the list is in the atomics' arrival order, which k_scatter
+// Within a cell the list is in the atomics' arrival order, which k_rank
 // ranks away.
 // ---------------------------------------------------------------------------
 constexpr int PLACE_U = 4;
@@ -385,131 +387,150 @@ __global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__
     }
 }
 
-// Position of the r-th set bit of m (r < popcount(m)), by halving.
-__device__ __forceinline__ int select_bit(unsigned long long m, int r)
-{
-    int pos = 0;
-    uint32_t w = (uint32_t)m;
-    int c = __popc(w);
-    if (r >= c) { r -= c; w = (uint32_t)(m >> 32); pos = 32; }
-    c = __popc(w & 0xffffu);
-    if (r >= c) { r -= c; w >>= 16; pos += 16; }
-    c = __popc(w & 0xffu);
-    if (r >= c) { r -= c; w >>= 8; pos += 8; }
-    c = __popc(w & 0xfu);
-    if (r >= c) { r -= c; w >>= 4; pos += 4; }
-    c = __popc(w & 0x3u);
-    if (r >= c) { r -= c; w >>= 2; pos += 2; }
-    return pos + (r >= (int)(w & 1u) ? 1 : 0);
-}
-
 // ---------------------------------------------------------------------------
-// k_scatter: SC_G lanes per cell.  A run's id is 64q + k for the k-th run of
-// chunk q; the runs of one cell lie in distinct chunks, and chunk q holds
-// particles [64q, 64q + 64), so ordering a cell's runs by id orders them by
-// particle index.  Run j's members go to start(c) + (the lengths of the
-// cell's runs with smaller id) + (their rank in the mask): perm lists each
-// cell's particles in ascending index order -- the order of the serial
-// binning, bit for bit.  The runs are taken SC_G at a time: each lane ranks
-// one against the cell's others (shuffles within the group) and writes its
-// members into an LDS window holding the permutation of the wave's cells,
-// which the wave then copies out with consecutive addresses (a window too
-// small for the cells: straight to perm).  Also materialises start[0..C]
-// and, for every tile of tile_p particles, the first cell that starts in it
-// and that cell's start (tile_first holds int2 pairs).
+// k_rank: a wave takes RK_C consecutive cells (one per lane).  A run's id is
+// 64q + k for the k-th run of chunk q; the runs of one cell lie in distinct
+// chunks, and chunk q holds particles [64q, 64q + 64), so ordering a cell's
+// runs by id orders them by particle index.  Each run's destination base is
+// start(c) + (the lengths of the cell's runs with smaller id); it goes into
+// the unused field of the run's runlist entry, which then holds (id, base,
+// lane mask): k_cell expands the runs of its tile from there into the
+// cell-sorted permutation, every cell's particles in ascending index order --
+// the order of the serial binning, bit for bit.  The task's cells' runs are
+// contiguous in runlist: the wave stages them in LDS as (id, length) pairs
+// with coalesced loads (RK_LDS at a time, in segments of whole cells), then
+// each lane takes a run, finds its cell by a binary search over the cells'
+// first runs, and ranks it against that cell's runs.  Also materialises
+// start[0..C] (particles) and start[C+1 .. 2C+1] (runs), and, for every tile
+// of tile_p particles, the first cell that starts in it with that cell's
+// particle and run starts (tiles holds int4 (cell, start, run start, 0)).
 // ---------------------------------------------------------------------------
-constexpr int SC_G = 16;
-constexpr int SC_WIN = 1024;   // LDS window per wave: the permutation of its cells
+constexpr int RK_C = 16;                 // cells per wave task
+constexpr int RK_LDS = 1024;             // runs staged in LDS per wave
+static_assert(RK_C < 64, "a lane beyond the task's cells holds the cell after them");
+constexpr int RK_U = 8;                  // runs compared per step of the ranking loop
 
-__global__ __launch_bounds__(256) void k_scatter(Bins bn, int *__restrict__ start, int *__restrict__ perm,
-                                                 int *__restrict__ tile_first, int ncell, int tile_p,
-                                                 const Ctrl *__restrict__ ctrl)
+__global__ __launch_bounds__(256) void k_rank(Bins bn, int *__restrict__ start, int4 *__restrict__ tiles,
+                                              int ncell, int tile_p, const Ctrl *__restrict__ ctrl)
 {
-    constexpr int CPW = 64 / SC_G;
-    __shared__ int s_win[4][SC_WIN];
-    const int lane = lane_id(), g = lane / SC_G, sub = lane % SC_G;
-    int *const win = s_win[threadIdx.x >> 6];
+    __shared__ int2 s_run[4][RK_LDS];
+    int2 *const sr = s_run[threadIdx.x >> 6];
+    const int lane = lane_id();
     const uint32_t n_next = ctrl->n_next, n_runs = ctrl->n_runs;
     const int ntiles = ((int)n_next + tile_p - 1) / tile_p;
     int first, end, stride;
-    xcd_range(ncell / CPW + 1, first, end, stride);
-    // the packed starts of a task's cells (and, for the tile table, of the
-    // cell before each), loaded one task ahead: the next task's loads are in
-    // flight while this one waits for its runs
-    auto starts = [&](int task, unsigned long long &a0, unsigned long long &a1, unsigned long long &ap) {
-        const int cc = task * CPW + g;
-        a0 = cc <= ncell ? packed_start(bn, cc, ncell, n_next, n_runs) : 0ull;
-        a1 = cc < ncell ? packed_start(bn, cc + 1, ncell, n_next, n_runs) : 0ull;
-        ap = (sub == 0 && cc > 0 && cc <= ncell) ? packed_start(bn, cc - 1, ncell, n_next, n_runs) : 0ull;
+    xcd_range(ncell / RK_C + 1, first, end, stride);   // cells 0..ncell (the last: the totals)
+    // lane j < RK_C holds cell task RK_C + j; the lanes from RK_C on hold the
+    // cell after the task's last, whose starts end the task's runs.  The
+    // packed starts of a task's cells (and of the cell before the first, lane
+    // 0, for the tile table) are loaded one task ahead: the next task's loads
+    // are in flight while this one ranks its runs
+    auto starts = [&](int task, unsigned long long &a0, unsigned long long &ap) {
+        const int cc = task * RK_C + min(lane, RK_C);
+        a0 = packed_start(bn, min(cc, ncell), ncell, n_next, n_runs);
+        ap = (lane == 0 && cc > 0) ? packed_start(bn, cc - 1, ncell, n_next, n_runs) : 0ull;
     };
-    unsigned long long n0 = 0ull, n1 = 0ull, np0 = 0ull;
-    if (first < end) starts(first, n0, n1, np0);
+    unsigned long long n0 = 0ull, np0 = 0ull;
+    if (first < end) starts(first, n0, np0);
     for (int task = first; task < end; task += stride) {
-        const int c = task * CPW + g;
-        const unsigned long long v0 = n0, v1 = c < ncell ? n1 : v0, vp = np0;
+        const int c = task * RK_C + lane;
+        const bool own = lane < RK_C;
+        const unsigned long long v0 = n0;
+        const unsigned long long v1 = __shfl_down(v0, 1, 64), up = __shfl_up(v0, 1, 64);
+        const unsigned long long vp = lane == 0 ? np0 : up;
         const int st = (int)(uint32_t)v0, rs = (int)(v0 >> 32);
-        const int L = (int)(v1 >> 32) - rs;
-        if (sub == 0 && c <= ncell) {
+        const int L = own && c < ncell ? (int)(v1 >> 32) - rs : 0;
+        if (own && c <= ncell) {
             start[c] = st;
+            start[ncell + 1 + c] = rs;
             const int tlo = c > 0 ? (int)(uint32_t)vp / tile_p + 1 : 0;
             const int thi = min(st / tile_p, ntiles - 1);
-            for (int t = tlo; t <= thi; ++t) reinterpret_cast<int2 *>(tile_first)[t] = make_int2(c, st);
+            for (int t = tlo; t <= thi; ++t) tiles[t] = make_int4(c, st, rs, 0);
         }
-        // Lw: the wave's largest run count (the trip count of the wave-uniform
-        // loops); its cells hold perm[P0, P1), staged in LDS when that fits
-        int Lw = L, P1 = c < ncell ? (int)(uint32_t)v1 : 0;
+        if (task + stride < end) starts(task + stride, n0, np0);
+        const int R0 = __shfl(rs, 0, 64);
+        const int rel = rs - R0, rend = rel + L;      // the lane's cell's runs [rel, rend), from R0
+        // segments of consecutive cells whose runs fit the staging area (a
+        // cell of more runs than that alone, read from runlist itself)
+        for (int s0 = 0; s0 < RK_C;) {
+            const int base = __shfl(rel, s0, 64);
+            const unsigned long long fit = __ballot(lane >= s0 && rend - base <= RK_LDS);
+            const bool lds = fit != 0ull;                 // else cell s0 alone has too many runs
+            const int e = lds ? 64 - (int)__clzll((long long)fit) : s0 + 1;
+            const int nr = __shfl(rend, e - 1, 64) - base;
+            int4 *const rl = bn.runlist + R0 + base;
+            if (lds) {
+                for (int k0 = 0; k0 < nr; k0 += 256) {
+                    int4 ev[4];
 #pragma unroll
-        for (int d = 32; d > 0; d >>= 1) {
-            Lw = max(Lw, __shfl_xor(Lw, d, 64));
-            P1 = max(P1, __shfl_xor(P1, d, 64));
-        }
-        if (task + stride < end) starts(task + stride, n0, n1, np0);
-        const int P0 = __shfl(st, 0, 64);
-        const bool staged = P1 - P0 <= SC_WIN;
-        for (int b0 = 0; b0 < Lw; b0 += SC_G) {
-            // this lane's run: id, mask (none: id INT_MAX, empty mask)
-            int rid = 0x7fffffff, mlo = 0, mhi = 0;
-            if (b0 + sub < L) {
-                const int4 e = bn.runlist[rs + b0 + sub];
-                rid = e.x; mlo = e.z; mhi = e.w;
-            }
-            int off = 0;
-            for (int p2 = 0; p2 < Lw; p2 += SC_G) {
-                int fid = rid, flen = __popc((uint32_t)mlo) + __popc((uint32_t)mhi);
-                if (p2 != b0) {
-                    fid = 0x7fffffff;
-                    flen = 0;
-                    if (p2 + sub < L) {
-                        const int4 f = bn.runlist[rs + p2 + sub];
-                        fid = f.x;
-                        flen = __popc((uint32_t)f.z) + __popc((uint32_t)f.w);
+                    for (int u = 0; u < 4; ++u) {
+                        const int k = k0 + 64 * u + lane;
+                        if (k < nr) ev[u] = rl[k];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int k = k0 + 64 * u + lane;
+                        if (k < nr) sr[k] = make_int2(ev[u].x, __popc((uint32_t)ev[u].z) + __popc((uint32_t)ev[u].w));
                     }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            // run k of the segment: the lane whose first run is the last one
+            // <= k holds its cell; each lane ranks one run per pass against
+            // its cell's runs, RK_U at a time (their reads in flight
+            // together).  The staged and the unstaged form are separate
+            // loops, so the staged one has no global load whose wait would
+            // also wait for the stores of the passes before.
+            auto cell_of = [&](int k, int &a, int &Lj, int &sj) {
+                int j = 0;
 #pragma unroll
-                for (int t = 0; t < SC_G; ++t) {
-                    const int r2 = __shfl(fid, t, SC_G), l2 = __shfl(flen, t, SC_G);
-                    if (r2 < rid) off += l2;
+                for (int d = 32; d > 0; d >>= 1)
+                    if (__shfl(rel, j + d, 64) <= base + k) j += d;
+                a = __shfl(rel, j, 64) - base;
+                Lj = __shfl(L, j, 64);
+                sj = __shfl(st, j, 64);
+            };
+            if (lds) {
+                for (int k0 = 0; k0 < nr; k0 += 64) {
+                    const int k = k0 + lane;
+                    int a, Lj, sj;
+                    cell_of(k, a, Lj, sj);
+                    if (k < nr) {
+                        const int me = sr[k].x;
+                        int off = 0;
+                        for (int t = a; t < a + Lj; t += RK_U) {
+                            int2 f[RK_U];
+#pragma unroll
+                            for (int u = 0; u < RK_U; ++u) f[u] = sr[min(t + u, RK_LDS - 1)];
+#pragma unroll
+                            for (int u = 0; u < RK_U; ++u)
+                                if (t + u < a + Lj && f[u].x < me) off += f[u].y;
+                        }
+                        reinterpret_cast<int *>(rl + k)[1] = sj + off;
+                    }
+                }
+            } else {
+                for (int k0 = 0; k0 < nr; k0 += 64) {
+                    const int k = k0 + lane;
+                    int a, Lj, sj;
+                    cell_of(k, a, Lj, sj);
+                    if (k < nr) {
+                        const int me = rl[k].x;
+                        int off = 0;
+                        for (int t = a; t < a + Lj; ++t) {
+                            const int4 f = rl[t];
+                            if (f.x < me) off += __popc((uint32_t)f.z) + __popc((uint32_t)f.w);
+                        }
+                        reinterpret_cast<int *>(rl + k)[1] = sj + off;
+                    }
                 }
             }
-            // every lane writes its own run's members, in ascending index order
-            if (b0 + sub < L) {
-                const int q0 = rid & ~63;
-                unsigned long long m = ((unsigned long long)(uint32_t)mhi << 32) | (uint32_t)mlo;
-                if (staged) {
-                    int *o = win + (st - P0) + off;
-                    for (int r = 0; m; ++r, m &= m - 1ull) o[r] = q0 + (int)__builtin_ctzll(m);
-                } else {
-                    int *o = perm + st + off;
-                    for (int r = 0; m; ++r, m &= m - 1ull) o[r] = q0 + (int)__builtin_ctzll(m);
-                }
+            if (lds) {   // the staging area is free again
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
-        }
-        if (staged) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            for (int k = lane; k < P1 - P0; k += 64) perm[P0 + k] = win[k];
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            s0 = e;
         }
     }
 }
@@ -817,14 +838,15 @@ hipError_t launch_scan_runs(const Bins &bn, int ncell, Ctrl *ctrl, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const Bins &bn, int *start, int *perm, int *tile_first, int ncell, int tile_p,
-                          const Ctrl *ctrl, int grid, hipStream_t st)
+hipError_t launch_scatter(const Bins &bn, int *start, int *tile_first, int ncell, int tile_p, const Ctrl *ctrl,
+                          int grid, hipStream_t st)
 {
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_place, dim3(grid), dim3(256), 0, st, bn, ctrl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, bn, start, perm, tile_first, ncell, tile_p, ctrl);
+    hipLaunchKernelGGL(k_rank, dim3(grid), dim3(256), 0, st, bn, start, reinterpret_cast<int4 *>(tile_first), ncell,
+                       tile_p, ctrl);
     return hipGetLastError();
 }
 
@@ -918,6 +940,6 @@ static int occ(const void *fn)
 }
 int occupancy_move() { return occ((const void *)k_move); }
 int occupancy_move_batch() { return occ((const void *)k_move_batch); }
-int occupancy_scatter() { return occ((const void *)k_scatter); }
+int occupancy_scatter() { return occ((const void *)k_rank); }
 
 }  // namespace dsmc

@@ -327,7 +327,7 @@ int dsmc_bin_sort_noalloc(const dsmc_grid *g, const float *x, const float *y, in
  * Changing them drops the captured graph. */
 typedef struct dsmc_launch {
     int move_blocks;     /* k_move grid, blocks of 256 threads */
-    int scatter_blocks;  /* k_scatter grid */
+    int scatter_blocks;  /* k_place and k_rank grid */
     int cell_blocks;     /* k_cell grid (persistent blocks pulling tiles) */
     int cell_tile;       /* particles per k_cell tile, 256..2048 */
     int graph_steps;     /* steps per captured hipGraph, 1..64 */
