@@ -861,13 +861,18 @@ __global__ __launch_bounds__(AMR_THREADS) void k_amr_cell(AmrArgs a)
     const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
     const size_t C = (size_t)cs.ncell;
     uint32_t bcoll = 0, bcand = 0;
-    for (int c0 = blockIdx.x * AMR_THREADS; c0 < cs.ncell; c0 += gridDim.x * AMR_THREADS) {
-        // this block's cells with sub-cells, found AMR_THREADS at a time
+    // block b takes the cells b, b + G, b + 2G, ... (G blocks): cells with
+    // sub-cells cluster in space, so a cyclic deal spreads them over every
+    // block; each pass checks AMR_THREADS of the block's cells and lists
+    // those with sub-cells
+    const int G = (int)gridDim.x;
+    for (int j0 = 0; (long long)j0 * G + (int)blockIdx.x < cs.ncell; j0 += AMR_THREADS) {
         if (tid == 0) s_nlist = 0;
         __syncthreads();
         {
-            const int c = c0 + tid;
-            if (c < cs.ncell && amr_cell(cs, c, a.start[c + 1] - a.start[c])) s_list[atomicAdd(&s_nlist, 1)] = c;
+            const long long cl = (long long)(j0 + tid) * G + (int)blockIdx.x;
+            const int c = cl < cs.ncell ? (int)cl : -1;
+            if (c >= 0 && amr_cell(cs, c, a.start[c + 1] - a.start[c])) s_list[atomicAdd(&s_nlist, 1)] = c;
         }
         __syncthreads();
         const int nlist = s_nlist;
