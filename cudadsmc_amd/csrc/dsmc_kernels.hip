@@ -730,15 +730,15 @@ __device__ __forceinline__ int find_inst(const uint32_t *tab, int ninst, uint32_
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ cases, int ninst,
-                                                    const int *__restrict__ cbase, SoA A, SoA B,
-                                                    Bins bn, Ctrl *__restrict__ ctrl, InstCtrl *__restrict__ ictl)
+// The batch's instance tables of a step in LDS: s_p = exclusive prefix of
+// the live counts, s_q = of the (clamped) inflow counts, each with the total
+// at [ninst].  The writer (block 0 of k_move_batch) also sets the move
+// counts and the capacity flags.
+__device__ void batch_tables(const DevCase *__restrict__ cases, int ninst, Ctrl *__restrict__ ctrl,
+                             InstCtrl *__restrict__ ictl, uint32_t step, uint32_t *s_p, uint32_t *s_q, int *s_w,
+                             bool writer)
 {
-    __shared__ uint32_t s_p[MAX_INST + 1], s_q[MAX_INST + 1];
-    __shared__ int s_w[4];
     const int t = threadIdx.x;
-    const uint32_t step = ctrl->step;
-    const SoA S = (step & 1u) ? B : A;
     uint32_t nl = 0, jt = 0;
     bool clamped = false;
     if (t < ninst) {
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
     const int tot_q = block_excl_scan((int)jt, ex_q, s_w);
     if (t < ninst) { s_p[t] = (uint32_t)ex_p; s_q[t] = (uint32_t)ex_q; }
     if (t == 0) { s_p[ninst] = (uint32_t)tot_p; s_q[ninst] = (uint32_t)tot_q; }
-    if (blockIdx.x == 0) {
+    if (writer) {
         if (t < ninst) {
             ictl[t].n_move = nl + jt;
             if (clamped) atomicOr(&ictl[t].err, (uint32_t)ERR_CAPACITY);
@@ -762,51 +762,125 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
         if (t == 0) ctrl->n_move = (uint32_t)(tot_p + tot_q);
     }
     __syncthreads();
-    const uint32_t n_live = (uint32_t)tot_p, ntot = n_live + (uint32_t)tot_q;
+}
+
+// k_move_batch: the chunks (64 consecutive particles, one wave) that hold
+// live particles of one instance only -- all but a few.  The instance is
+// then the same for the whole wave, so its case is read with scalar loads
+// and the wave needs no loop over instances; the chunks across an instance
+// boundary and those holding inflow particles are k_move_batch_rest's.
+// Keeping the instance loop and the inflow out of this kernel keeps its
+// registers low enough for more waves per SIMD.
+__global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ cases, int ninst,
+                                                    const int *__restrict__ cbase, SoA A, SoA B,
+                                                    Bins bn, Ctrl *__restrict__ ctrl, InstCtrl *__restrict__ ictl)
+{
+    __shared__ uint32_t s_p[MAX_INST + 1], s_q[MAX_INST + 1];
+    __shared__ int s_w[4];
+    const uint32_t step = ctrl->step;
+    const SoA S = (step & 1u) ? B : A;
+    batch_tables(cases, ninst, ctrl, ictl, step, s_p, s_q, s_w, blockIdx.x == 0);
+    const int lane = lane_id();
+    const int nq = (int)(s_p[ninst] >> 6);            // chunks wholly of live particles
+    const int nw = (int)(gridDim.x * (blockDim.x >> 6));
+    int q = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     uint32_t err = 0;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t i = blockIdx.x * blockDim.x + t;
-    // next live particle's state loaded one iteration ahead (as in k_move)
+    int last = 0;                                      // the previous chunk's instance
+    // the next chunk's state loaded one chunk ahead
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
-    if (i < n_live) { px = S.x[i]; py = S.y[i]; pvx = S.vx[i]; pvy = S.vy[i]; pvz = S.vz[i]; }
-    int last = -1;          // the previous particle's instance: kept while its range holds the key
-    bool last_inflow = false;
-    for (; i < ntot; i += stride) {
-        float lx = px, ly = py, lvx = pvx, lvy = pvy, lvz = pvz;
-        const uint32_t jn = i + stride;
-        if (jn < n_live) { px = S.x[jn]; py = S.y[jn]; pvx = S.vx[jn]; pvy = S.vy[jn]; pvz = S.vz[jn]; }
-        const bool inflow = i >= n_live;
-        const uint32_t key = inflow ? i - n_live : i;
-        const uint32_t *tab = inflow ? s_q : s_p;
-        const int inst = (last >= 0 && inflow == last_inflow && tab[last] <= key && key < tab[last + 1])
-                             ? last : find_inst(tab, ninst, key);
-        last = inst;
-        last_inflow = inflow;
+    if (q < nq) {
+        const uint32_t i = (uint32_t)q * 64u + (uint32_t)lane;
+        px = S.x[i]; py = S.y[i]; pvx = S.vx[i]; pvy = S.vy[i]; pvz = S.vz[i];
+    }
+    for (; q < nq; q += nw) {
+        const uint32_t i = (uint32_t)q * 64u + (uint32_t)lane;
+        float x = px, y = py, vx = pvx, vy = pvy, vz = pvz, er = 0.0f;
+        if (q + nw < nq) {
+            const uint32_t j = i + 64u * (uint32_t)nw;
+            px = S.x[j]; py = S.y[j]; pvx = S.vx[j]; pvy = S.vy[j]; pvz = S.vz[j];
+        }
+        const uint32_t lo = (uint32_t)q * 64u;
+        int u = (s_p[last] <= lo && lo < s_p[last + 1]) ? last : find_inst(s_p, ninst, lo);
+        u = __builtin_amdgcn_readfirstlane(u);
+        last = u;
+        if (s_p[u + 1] < lo + 64u) continue;           // an instance boundary inside: the rest kernel's
+        const DevCase &cs = cases[u];
+        bool vchg = false, echg = false;
+        // batched instances are monatomic (zeta = 0): er is never used
+        const bool dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
+        S.x[i] = x; S.y[i] = y;
+        if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
+        if (err) { atomicOr(&ictl[u].err, err); err = 0; }
+        agg_run(bn, dead ? -1 : cbase[u] + cell_of(cs, x, y), i);
+    }
+}
+
+// k_move_batch_rest: the chunks k_move_batch leaves -- those across an
+// instance boundary (listed once, by the first instance that starts inside
+// them) and those at and past the end of the live particles, where the
+// inflow particles are created.  A wave takes a chunk and runs its lanes one
+// instance at a time.
+__global__ __launch_bounds__(256) void k_move_batch_rest(const DevCase *__restrict__ cases, int ninst,
+                                                         const int *__restrict__ cbase, SoA A, SoA B,
+                                                         Bins bn, Ctrl *__restrict__ ctrl,
+                                                         InstCtrl *__restrict__ ictl)
+{
+    __shared__ uint32_t s_p[MAX_INST + 1], s_q[MAX_INST + 1];
+    __shared__ int s_w[4];
+    __shared__ int s_chunk[MAX_INST];
+    __shared__ int s_nb;
+    const int t = threadIdx.x;
+    const uint32_t step = ctrl->step;
+    const SoA S = (step & 1u) ? B : A;
+    batch_tables(cases, ninst, ctrl, ictl, step, s_p, s_q, s_w, false);
+    const uint32_t n_live = s_p[ninst], ntot = n_live + s_q[ninst];
+    const int nq = (int)(n_live >> 6), nend = (int)((ntot + 63u) >> 6);
+    // the boundary chunks below nq, in instance order
+    if (t == 0) s_nb = 0;
+    __syncthreads();
+    if (t >= 1 && t < ninst) {
+        const uint32_t s = s_p[t], sp = s_p[t - 1];
+        const bool first = (s & 63u) != 0u && (int)(s >> 6) < nq &&
+                           !(t >= 2 && (sp & 63u) != 0u && (sp >> 6) == (s >> 6));
+        if (first) s_chunk[atomicAdd(&s_nb, 1)] = (int)(s >> 6);
+    }
+    __syncthreads();
+    const int nb = s_nb, nitems = nb + (nend - nq);
+    const int lane = lane_id();
+    const int nw = (int)(gridDim.x * (blockDim.x >> 6));
+    uint32_t err = 0;
+    for (int it = (int)(blockIdx.x * (blockDim.x >> 6) + (t >> 6)); it < nitems; it += nw) {
+        const int q = it < nb ? s_chunk[it] : nq + (it - nb);
+        const uint32_t i = (uint32_t)q * 64u + (uint32_t)lane;
         int c = -1;
-        for (bool todo = true; todo;) {
-            const int u = __builtin_amdgcn_readfirstlane(inst);
-            if (inst != u) continue;
-            todo = false;
-            const DevCase &cs = cases[u];
-            // batched instances are monatomic (zeta = 0): er is never used
-            float x, y, vx, vy, vz, er = 0.0f;
-            bool dead;
-            if (!inflow) {
-                x = lx; y = ly; vx = lvx; vy = lvy; vz = lvz;
-                bool vchg = false, echg = false;
-                dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
-                S.x[i] = x; S.y[i] = y;
-                if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
-            } else {
-                const uint32_t q = key - s_q[u];
-                int J[4];
-                inject_counts(cs, step, J);
-                const uint32_t pid = ictl[u].next_pid + q;
-                dead = inject_one(cs, inflow_face(J, q), pid, step, x, y, vx, vy, vz, er, err);
-                S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
+        if (i < ntot) {
+            const bool inflow = i >= n_live;
+            const uint32_t key = inflow ? i - n_live : i;
+            const int inst = find_inst(inflow ? s_q : s_p, ninst, key);
+            for (bool todo = true; todo;) {
+                const int u = __builtin_amdgcn_readfirstlane(inst);
+                if (inst != u) continue;
+                todo = false;
+                const DevCase &cs = cases[u];
+                float x, y, vx, vy, vz, er = 0.0f;
+                bool dead;
+                if (!inflow) {
+                    x = S.x[i]; y = S.y[i]; vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
+                    bool vchg = false, echg = false;
+                    dead = move_one(cs, x, y, vx, vy, vz, er, S.pid + i, 0u, false, step, cs.dt, err, vchg, echg);
+                    S.x[i] = x; S.y[i] = y;
+                    if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
+                } else {
+                    const uint32_t qi = key - s_q[u];
+                    int J[4];
+                    inject_counts(cs, step, J);
+                    const uint32_t pid = ictl[u].next_pid + qi;
+                    dead = inject_one(cs, inflow_face(J, qi), pid, step, x, y, vx, vy, vz, er, err);
+                    S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
+                }
+                c = dead ? -1 : cbase[u] + cell_of(cs, x, y);
+                if (err) { atomicOr(&ictl[u].err, err); err = 0; }
             }
-            c = dead ? -1 : cbase[u] + cell_of(cs, x, y);
-            if (err) { atomicOr(&ictl[u].err, err); err = 0; }
         }
         agg_run(bn, c, i);
     }
@@ -871,6 +945,9 @@ hipError_t launch_move_batch(const DevCase *cases, int ninst, const int *cbase, 
 {
     if (ninst < 1 || ninst > MAX_INST) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_move_batch, dim3(grid), dim3(256), 0, st, cases, ninst, cbase, A, B, bn, ctrl, ictl);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_move_batch_rest, dim3(grid), dim3(256), 0, st, cases, ninst, cbase, A, B, bn, ctrl, ictl);
     return hipGetLastError();
 }
 
