@@ -439,7 +439,8 @@ struct KBatch {
 template <bool LDS, bool ROT, class KF>
 __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const SoA &nxt, uint32_t step,
                           int ca, int cb, int p0, int n, int ra, int rb, int *Sp, float *Vx, float *Vy, float *Vz,
-                          float *Er, CellShared &sh, uint32_t &tcoll, uint32_t &tcand, int *lmarks, int lmark_cap)
+                          float *Er, CellShared &sh, uint32_t &tcoll, uint32_t &tcand, int *lmarks, int lmark_cap,
+                          bool claim, int carry, int *ahead)
 {
     const int tid = threadIdx.x, bs = blockDim.x, wave = tid >> 6, lane = lane_id();
     const int nwaves = bs >> 6;
@@ -454,6 +455,11 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
     // collisions); larger tiles stage each chunk's metadata at the
     // collisions.
     const bool one = cb - ca <= CELL_MAXC;
+    // the block's next tile claimed ahead (its queue position goes to
+    // *ahead; carry = one claimed before and not yet used): the atomic is
+    // issued before this phase's loads, so their wait covers its return
+    int qa = carry;
+    if (claim && tid == 0) qa = (int)atomicAdd(&a.ctrl->tile_next, 1u) + (int)gridDim.x;
     {
         const int nc = cb - ca, nr = rb - ra;
         const int lp = nr <= CELL_THREADS / 4 ? 2 : nr <= CELL_THREADS / 2 ? 1 : 0;   // log2 P
@@ -497,6 +503,7 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
             }
         }
     }
+    if (tid == 0) *ahead = qa;
     if (!LDS) __threadfence();
     __syncthreads();
 
@@ -688,6 +695,7 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     const int ntiles = (n_next + a.tile_p - 1) / a.tile_p;
     const bool sampling = a.sample_start >= 0 && step >= (uint32_t)a.sample_start;
     __shared__ int s_tile;
+    __shared__ int s_ahead[2];    // a tile claimed ahead, by round parity
     if (tid == 0) { sh.coll = 0u; sh.cand = 0u; }
     if constexpr (BATCH) {
         for (int t = tid; t <= a.ninst; t += blockDim.x) bsh.cbase[t] = a.cbase[t];
@@ -696,8 +704,10 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     uint32_t tcoll = 0, tcand = 0;
     const int4 *tiles = reinterpret_cast<const int4 *>(a.tile_first);
     int rest_c = 0, rest_cb = 0;      // the part of a cut tile still to do
+    int pending = -1;                 // a queue position claimed ahead and not yet used
     for (int round = 0;; ++round) {
         int ca, cb, p0, n, ra, rb;
+        int qcur = -1;
         const int *const rstart = a.start + a.cs.ncell + 1;
         if (rest_c < rest_cb) {
             ca = rest_c; cb = rest_cb;
@@ -711,12 +721,18 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
             // left to fill the end of the launch
             int q = (int)blockIdx.x;
             if (round > 0) {
-                if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u) + (int)gridDim.x;
-                __syncthreads();
-                q = s_tile;
-                __syncthreads();
+                if (pending >= 0) {
+                    q = pending;
+                    pending = -1;
+                } else {
+                    if (tid == 0) s_tile = (int)atomicAdd(&ctrl->tile_next, 1u) + (int)gridDim.x;
+                    __syncthreads();
+                    q = s_tile;
+                    __syncthreads();
+                }
             }
             if (q >= ntiles) break;
+            qcur = q;
             const int t = ntiles - 1 - q;
             // tile t: cells [ca, cb), particles [p0, p0 + n), runs [ra, rb),
             // from the table of (first cell, its start, its run start)
@@ -743,17 +759,23 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
             rest_c = c2; rest_cb = cb;
             cb = c2;
         }
+        // the next tile is claimed during this one while the queue is far
+        // from its end (a block holding a claimed tile there would leave the
+        // others idle at the end of the launch)
+        const bool claim = pending < 0 && qcur >= 0 && qcur + 2 * (int)gridDim.x < ntiles;
+        int *const ahead = &s_ahead[round & 1];
         auto run = [&](const auto &kf) {
             if (n <= CELL_CAP)
                 cell_tile<true, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, ra, rb, s_perm, s_vx, s_vy, s_vz, s_er, sh,
-                                     tcoll, tcand, nullptr, 0);
+                                     tcoll, tcand, nullptr, 0, claim, pending, ahead);
             else    // a single cell: its collision rounds alone use the idle staging LDS for marks
                 cell_tile<false, ROT>(a, kf, cur, nxt, step, ca, cb, p0, n, ra, rb, a.perm + p0, nxt.vx + p0,
                                       nxt.vy + p0, nxt.vz + p0, ROT ? nxt.er + p0 : nullptr, sh, tcoll, tcand, stage.i,
-                                      4 * CELL_CAP);
+                                      4 * CELL_CAP, claim, pending, ahead);
         };
         if constexpr (BATCH) run(KBatch{ bsh, sh.inst, a.ninst, step, a.cc, a.ictl });
         else run(KSingle{ a.cs, sampling });
+        pending = *ahead;
     }
     if (tcoll) atomicAdd(&sh.coll, tcoll);
     if (tcand) atomicAdd(&sh.cand, tcand);
