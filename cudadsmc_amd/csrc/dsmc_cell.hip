@@ -866,15 +866,208 @@ struct AmrArgs {
 };
 
 constexpr int AMR_THREADS = 256;
+constexpr int AMR_WCAP = 256;        // a cell of at most this many particles is taken by one wave
+
+// LDS of one wave's cell, and of the block's cell: leaf order, leaf of each
+// particle, velocities (and rotational energies) in leaf order, and per
+// leaf its start, cursor, candidate count and first candidate number.
+template <int CAP, int NV>
+struct AmrLds {
+    int idx[CAP];
+    int leaf[CAP];
+    float v[NV][CAP];
+    int ls[AMR_LEAVES + 1], pos[AMR_LEAVES], M[AMR_LEAVES], m0[AMR_LEAVES];
+};
+template <int NV>
+union AmrShared {
+    AmrLds<AMR_WCAP, NV> w[AMR_THREADS / 64];
+    AmrLds<AMR_CAP, NV> b;
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One cell with sub-cells, np <= AMR_WCAP particles, by one wave: the steps
+// of the block form below, with the leaves dealt to the wave's lanes and
+// wave barriers only.  The arithmetic and every order (leaf order, candidate
+// numbering, the lane-strided FP64 sampling sums and their butterfly) are
+// the block form's, so both give the same bits.
+template <bool ROT>
+__device__ void amr_cell_wave(const AmrArgs &a, const SoA &S, uint32_t step, bool sampling, int c,
+                              AmrLds<AMR_WCAP, ROT ? 4 : 3> &L, uint32_t &wcoll, uint32_t &wcand)
+{
+    const DevCase &cs = a.cs;
+    const int lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int o = a.start[c], np = a.start[c + 1] - o;
+    int nsx, nsy;
+    const int nl = amr_leaves(cs.sub[c], nsx, nsy);
+    for (int l = lane; l < nl; l += 64) L.pos[l] = 0;
+    wave_sync();
+    for (int k = lane; k < np; k += 64) {
+        const int lf = amr_leaf(cs, c, S.x[o + k], S.y[o + k], nsx, nsy);
+        L.leaf[k] = lf;
+        atomicAdd(&L.pos[lf], 1);
+    }
+    wave_sync();
+    // leaf starts, candidate numbers with the remainder carried in leaf order
+    if (lane == 0) {
+        const float vol = cs.vol_ntc[c];
+        const float vleaf = vol / (float)nl;
+        const float smax0 = a.sigcmax[c];
+        float mr = a.mrem[c];
+        int st = 0, m0 = 0;
+        for (int l = 0; l < nl; ++l) {
+            const int npl = L.pos[l];
+            L.ls[l] = st;
+            L.pos[l] = st;
+            st += npl;
+            int M = 0;
+            if (vol > 0.0f && npl >= 2) M = ntc_count(cs.ntc_coef, npl, vleaf, smax0, mr);
+            L.M[l] = M;
+            L.m0[l] = m0;
+            m0 += M;
+        }
+        L.ls[nl] = st;
+        a.mrem[c] = mr;
+        wcand += (uint32_t)m0;
+    }
+    wave_sync();
+    // stable order by leaf, chunk after chunk in index order
+    for (int k0 = 0; k0 < np; k0 += 64) {
+        const int k = k0 + lane;
+        const int lf = k < np ? L.leaf[k] : -1;
+        unsigned long long todo = __ballot(lf >= 0);
+        while (todo) {
+            const int l0 = (int)__ffsll((long long)todo) - 1;
+            const int ll = __shfl(lf, l0, 64);
+            const unsigned long long m = __ballot(lf == ll);
+            int base = 0;
+            if (lane == l0) base = L.pos[ll];
+            base = __shfl(base, l0, 64);
+            if (lf == ll) L.idx[base + __popcll(m & lt)] = k;
+            if (lane == l0) L.pos[ll] = base + __popcll(m);
+            wave_sync();
+            todo &= ~m;
+        }
+    }
+    for (int p = lane; p < np; p += 64) {
+        const int k = o + L.idx[p];
+        L.v[0][p] = S.vx[k];
+        L.v[1][p] = S.vy[k];
+        L.v[2][p] = S.vz[k];
+        if constexpr (ROT) L.v[3][p] = S.er[k];
+    }
+    wave_sync();
+    // one lane per leaf
+    const CK key{ cs.K1, cs.e_exp, cs.seed, c, cs.rot_p, cs.lb_ia, cs.lb_ib };
+    const float smax0 = a.sigcmax[c];
+    float best = smax0;
+    uint32_t nc = 0;
+    for (int l = lane; l < nl; l += 64) {
+        const int M = L.M[l];
+        if (M == 0) continue;
+        const int b = L.ls[l], npl = L.ls[l + 1] - b, m0 = L.m0[l];
+        float smax = smax0;
+        for (int m = m0; m < m0 + M; ++m) {
+            const uint4 w = philox((uint32_t)m, step, SUB_COLLIDE, 0u, (uint32_t)c, cs.seed);
+            const int i1 = mulhi_idx(w.x, npl);
+            int i2 = mulhi_idx(w.y, npl - 1);
+            if (i2 >= i1) i2 = i2 + 1;
+            const int p1 = b + i1, p2 = b + i2;
+            float v1x = L.v[0][p1], v1y = L.v[1][p1], v1z = L.v[2][p1];
+            float v2x = L.v[0][p2], v2y = L.v[1][p2], v2z = L.v[2][p2];
+            const float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
+            const float g2 = gx * gx + gy * gy + gz * gz;
+            const float sc = vhs_sigc(key, g2);
+            if (sc > smax) smax = sc;
+            if (u01(w.z) < sc / smax) {
+                Scatter sd = scatter_dir(key, step, m, w.w, g2);
+                if constexpr (ROT) lb_exchange(key, step, m, g2, L.v[3][p1], L.v[3][p2], sd);
+                scatter_pair(sd, v1x, v1y, v1z, v2x, v2y, v2z);
+                L.v[0][p1] = v1x; L.v[1][p1] = v1y; L.v[2][p1] = v1z;
+                L.v[0][p2] = v2x; L.v[1][p2] = v2y; L.v[2][p2] = v2z;
+                ++nc;
+            }
+        }
+        best = fmaxf(best, smax);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        nc += __shfl_xor(nc, d, 64);
+        best = fmaxf(best, __shfl_xor(best, d, 64));
+    }
+    wave_sync();
+    // the cell in leaf order: positions and ids through registers
+    constexpr int PR = AMR_WCAP / 64;
+    float rx[PR], ry[PR];
+    uint32_t rp[PR];
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+        const int p = lane + j * 64;
+        if (p < np) {
+            const int k = o + L.idx[p];
+            rx[j] = S.x[k]; ry[j] = S.y[k]; rp[j] = S.pid[k];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+        const int p = lane + j * 64;
+        if (p < np) {
+            S.x[o + p] = rx[j]; S.y[o + p] = ry[j]; S.pid[o + p] = rp[j];
+            S.vx[o + p] = L.v[0][p]; S.vy[o + p] = L.v[1][p]; S.vz[o + p] = L.v[2][p];
+            if constexpr (ROT) S.er[o + p] = L.v[3][p];
+        }
+    }
+    if (lane == 0) {
+        a.sigcmax[c] = best;
+        a.ncoll_cell[c] = (int)nc;
+        wcoll += nc;
+    }
+    if (sampling) {
+        double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
+        for (int p = lane; p < np; p += 64) {
+            const double u = (double)L.v[0][p], v = (double)L.v[1][p], w = (double)L.v[2][p];
+            sx += u; sy += v; sz += w;
+            s2 += u * u + v * v + w * w;
+            if constexpr (ROT) se += (double)L.v[3][p];
+        }
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            sx += __shfl_xor(sx, d, 64);
+            sy += __shfl_xor(sy, d, 64);
+            sz += __shfl_xor(sz, d, 64);
+            s2 += __shfl_xor(s2, d, 64);
+            if constexpr (ROT) se += __shfl_xor(se, d, 64);
+        }
+        if (lane == 0) {
+            const size_t C = (size_t)cs.ncell;
+            unsafeAtomicAdd(&a.acc[c], (double)np);
+            unsafeAtomicAdd(&a.acc[C + c], sx);
+            unsafeAtomicAdd(&a.acc[2 * C + c], sy);
+            unsafeAtomicAdd(&a.acc[3 * C + c], sz);
+            unsafeAtomicAdd(&a.acc[4 * C + c], s2);
+            unsafeAtomicAdd(&a.acc[5 * C + c], (double)nc);
+            if constexpr (ROT) unsafeAtomicAdd(&a.acc[6 * C + c], se);
+        }
+    }
+    wave_sync();
+}
 
 template <bool ROT>
 __global__ __launch_bounds__(AMR_THREADS) void k_amr_cell(AmrArgs a)
 {
-    __shared__ int s_idx[AMR_CAP];          // leaf order -> particle in the cell
-    __shared__ int s_leaf[AMR_CAP];         // leaf of each particle (cell order)
-    __shared__ float s_v[ROT ? 4 : 3][AMR_CAP];
-    __shared__ int s_ls[AMR_LEAVES + 1], s_pos[AMR_LEAVES], s_M[AMR_LEAVES], s_m0[AMR_LEAVES];
-    __shared__ int s_list[AMR_THREADS], s_nlist;
+    __shared__ AmrShared<ROT ? 4 : 3> lds;
+    int *const s_idx = lds.b.idx;           // leaf order -> particle in the cell
+    int *const s_leaf = lds.b.leaf;         // leaf of each particle (cell order)
+    float (*const s_v)[AMR_CAP] = lds.b.v;
+    int *const s_ls = lds.b.ls, *const s_pos = lds.b.pos, *const s_M = lds.b.M, *const s_m0 = lds.b.m0;
+    __shared__ int s_list[AMR_THREADS], s_nlist;      // cells for the block
+    __shared__ int s_wlist[AMR_THREADS], s_nwlist;    // cells for a wave each
     __shared__ uint32_t s_nc, s_best;
     const DevCase &cs = a.cs;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -889,12 +1082,23 @@ __global__ __launch_bounds__(AMR_THREADS) void k_amr_cell(AmrArgs a)
     // those with sub-cells
     const int G = (int)gridDim.x;
     for (int j0 = 0; (long long)j0 * G + (int)blockIdx.x < cs.ncell; j0 += AMR_THREADS) {
-        if (tid == 0) s_nlist = 0;
+        if (tid == 0) { s_nlist = 0; s_nwlist = 0; }
         __syncthreads();
         {
             const long long cl = (long long)(j0 + tid) * G + (int)blockIdx.x;
             const int c = cl < cs.ncell ? (int)cl : -1;
-            if (c >= 0 && amr_cell(cs, c, a.start[c + 1] - a.start[c])) s_list[atomicAdd(&s_nlist, 1)] = c;
+            const int k = c >= 0 ? a.start[c + 1] - a.start[c] : 0;
+            if (c >= 0 && amr_cell(cs, c, k)) {
+                if (k <= AMR_WCAP) s_wlist[atomicAdd(&s_nwlist, 1)] = c;
+                else s_list[atomicAdd(&s_nlist, 1)] = c;
+            }
+        }
+        __syncthreads();
+        // the small cells, a wave each
+        {
+            const int wave = tid >> 6, nwl = s_nwlist;
+            for (int q = wave; q < nwl; q += AMR_THREADS / 64)
+                amr_cell_wave<ROT>(a, S, step, sampling, s_wlist[q], lds.w[wave], bcoll, bcand);
         }
         __syncthreads();
         const int nlist = s_nlist;
@@ -1059,7 +1263,7 @@ __global__ __launch_bounds__(AMR_THREADS) void k_amr_cell(AmrArgs a)
             __syncthreads();
         }
     }
-    if (tid == 0 && (bcoll || bcand)) {
+    if ((tid & 63) == 0 && (bcoll || bcand)) {   // lane 0 of every wave (the block's cells: thread 0)
         atomicAdd(&a.ctrl->ncoll, (unsigned long long)bcoll);
         atomicAdd(&a.ctrl->last_coll, bcoll);
         atomicAdd(&a.ctrl->ncand, (unsigned long long)bcand);
