@@ -627,7 +627,10 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                 const int k = i < nc ? sh.st[i + 1] - o : 0;
                 const bool on = k > 0 && kf.sample(i) && !kf.amr(c0 + i, k);
                 double sx = 0.0, sy = 0.0, sz = 0.0, s2 = 0.0, se = 0.0;
+                // unrolled so that several particles' loads are in flight
+                // before the adds, which keep their order
                 if (on)
+#pragma unroll 4
                     for (int q = sub; q < k; q += SAMPLE_G) {
                         const double u = (double)Vx[o + q], v = (double)Vy[o + q], w = (double)Vz[o + q];
                         sx += u; sy += v; sz += w;
