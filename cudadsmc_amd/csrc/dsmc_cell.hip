@@ -32,7 +32,7 @@ constexpr int CELL_THREADS = 64 * CELL_WAVES;   // k_cell's block
 static_assert(CELL_MAXC + 1 <= CELL_THREADS, "one thread per staged cell start");
 constexpr int NTC_G = 8;             // lanes per cell in the grouped NTC rounds
 constexpr int NTC_GROUP_MAX = 64;    // above this many candidates a whole wave takes the cell
-constexpr int GATHER_UNROLL = 2;     // particles per thread with loads in flight
+constexpr int GATHER_UNROLL = 4;     // particles per thread with loads in flight
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
