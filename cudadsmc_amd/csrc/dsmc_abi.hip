@@ -542,8 +542,9 @@ int build_graph(dsmc_handle h)
 // live at the end, runs.  A run record is 16 B (written by k_move, read and
 // written again by k_place, read by k_rank, its base written back 4 B, read
 // again by k_cell, which expands it into the tile's permutation), a packed
-// cell counter 8 B, a cell's particle and run start 4 B each, a chunk's run
-// count 4 B per 64 moved particles.  A diatomic gas
+// cell counter 8 B (read and zeroed by the scan, which writes its 8 B tile
+// scan), a cell's particle and run start 4 B each, a chunk's run count 4 B
+// per 64 moved particles.  A diatomic gas
 // adds its rotational energy: written for every injected particle by the
 // move, gathered and written back by the cell kernel, one more sample block.
 void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
@@ -557,7 +558,7 @@ void step_bytes(dsmc_handle h, const Ctrl &before, const Ctrl &after, int K)
     const double S = (double)(after.nsamples - before.nsamples);      // sampled steps
     const double R = (double)(after.nruns - before.nruns);            // (cell, wave) runs binned
     h->t_bytes[0] += 40.0 * snl + (24.0 + 4.0 * zr) * sJ + (4.0 + 1.0 / 16.0) * snm + 16.0 * R + 8.0 * C * k;
-    h->t_bytes[1] += 16.0 * C * k;
+    h->t_bytes[1] += 24.0 * C * k;
     h->t_bytes[2] += 52.0 * R + snm / 16.0 + 16.0 * C * k;
     h->t_bytes[3] += (48.0 + 8.0 * zr) * snn + 16.0 * R + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * k +

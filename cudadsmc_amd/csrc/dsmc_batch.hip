@@ -510,7 +510,7 @@ int timed_steps(dsmc_batch b, int nsteps)
         sampled += 16.0 * nn_i * (S / K) + 96.0 * (double)(b->cbase[i + 1] - b->cbase[i]) * S;
     }
     b->t_bytes[0] += 40.0 * snl + 24.0 * sJ + (4.0 + 1.0 / 16.0) * snm + 16.0 * R + 8.0 * C * K;
-    b->t_bytes[1] += 16.0 * C * K;
+    b->t_bytes[1] += 24.0 * C * K;
     b->t_bytes[2] += 52.0 * R + snm / 16.0 + 16.0 * C * K;
     b->t_bytes[3] += 48.0 * snn + 16.0 * R + 32.0 * (double)(after.ncand - before.ncand) +
                      24.0 * (double)(after.ncoll - before.ncoll) + 16.0 * C * K + sampled;

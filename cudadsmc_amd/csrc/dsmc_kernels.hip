@@ -5,9 +5,10 @@
 //                the binning: per wave (64 consecutive particles) one run per
 //                cell, one packed atomic per run  (cudaDSMC GPUParticle.cu:229-363,
 //                                                   :375-422, GPUCell.cu:169-198)
-//   k_scan_runs  exclusive scan of the per-cell particle and run counts,
-//                two-level, last-block finish      (GPUCell.cu:104-120)
-//   k_place      every cell's runs listed contiguously
+//   k_scan_runs  exclusive scan of the per-cell particle and run counts
+//                within tiles of cells             (GPUCell.cu:104-120)
+//   k_place      the tile offsets (every block, in LDS), then every cell's
+//                runs listed contiguously
 //   k_rank       per cell, runs ranked by wave: each run's destination base
 //                (the stable order: ascending particle index within a cell),
 //                the start tables and the k_cell tile boundaries
@@ -224,10 +225,14 @@ __global__ __launch_bounds__(256) void k_scan(int *__restrict__ count, int *__re
 }
 
 // ---------------------------------------------------------------------------
-// k_scan_runs: k_scan over the packed per-cell counters (runs << 32 |
-// particles).  Neither half can carry into the other -- both totals stay
-// below 2^32 -- so one 64-bit scan gives the particle starts in the low half
-// and the run starts in the high half.  cnt is zeroed for the next step.
+// k_scan_runs: a tile-local scan of the packed per-cell counters (runs << 32
+// | particles) and the tile totals.  Neither half can carry into the other --
+// both totals stay below 2^32 -- so one 64-bit scan gives the particle starts
+// in the low half and the run starts in the high half.  cnt is zeroed for the
+// next step.  The tile offsets are scanned by every block of k_place in LDS
+// (block 0 publishes them), so no block here waits on another: a last-block
+// finish needs a device-scope release per block, which writes back the L2
+// of its XCD.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long long v, unsigned long long &excl,
                                                                 unsigned long long *s_w)
@@ -253,10 +258,9 @@ __device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long lo
     return total;
 }
 
-__global__ __launch_bounds__(256) void k_scan_runs(Bins bn, int ncell, int ntile, Ctrl *__restrict__ ctrl)
+__global__ __launch_bounds__(256) void k_scan_runs(Bins bn, int ncell)
 {
     __shared__ unsigned long long s_w[4];
-    __shared__ int s_last;
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_IPT;
     unsigned long long v[SCAN_IPT];
     unsigned long long run = 0;
@@ -278,17 +282,46 @@ __global__ __launch_bounds__(256) void k_scan_runs(Bins bn, int ncell, int ntile
         excl += v[k];
     }
     if (threadIdx.x == 0) bn.tsum[blockIdx.x] = total;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = (atomicAdd(&ctrl->scan_done, 1u) == (uint32_t)(ntile - 1));
-    __syncthreads();
-    if (!s_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// The tile offsets: exclusive scan of the tile totals into s_toff (n <=
+// TOFF_LDS tiles), by one block; the totals end the step's particle and run
+// counts.  With `publish` the block also writes them for the later kernels
+// (bn.toff, ctrl n_next, n_runs, nruns).
+constexpr int TOFF_LDS = 2048;
+
+__device__ void tile_offsets(const Bins &bn, int ntile, unsigned long long *s_toff, unsigned long long *s_w,
+                             bool publish, Ctrl *__restrict__ ctrl)
+{
     unsigned long long carry = 0;
     for (int b0 = 0; b0 < ntile; b0 += 256) {
         const int b = b0 + (int)threadIdx.x;
-        const unsigned long long t =
-            b < ntile ? __hip_atomic_load(&bn.tsum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        const unsigned long long t = b < ntile ? bn.tsum[b] : 0ull;
+        unsigned long long e;
+        const unsigned long long tot = block_excl_scan64(t, e, s_w);
+        if (b < ntile) {
+            s_toff[b] = carry + e;
+            if (publish) bn.toff[b] = carry + e;
+        }
+        carry += tot;
+    }
+    if (publish && threadIdx.x == 0) {
+        ctrl->n_next = (uint32_t)carry;
+        ctrl->n_runs = (uint32_t)(carry >> 32);
+        ctrl->nruns += carry >> 32;
+    }
+    __syncthreads();
+}
+
+// k_scan_tiles: the same in global memory, for grids of more than TOFF_LDS
+// tiles (one block).
+__global__ __launch_bounds__(256) void k_scan_tiles(Bins bn, int ntile, Ctrl *__restrict__ ctrl)
+{
+    __shared__ unsigned long long s_w[4];
+    unsigned long long carry = 0;
+    for (int b0 = 0; b0 < ntile; b0 += 256) {
+        const int b = b0 + (int)threadIdx.x;
+        const unsigned long long t = b < ntile ? bn.tsum[b] : 0ull;
         unsigned long long e;
         const unsigned long long tot = block_excl_scan64(t, e, s_w);
         if (b < ntile) bn.toff[b] = carry + e;
@@ -298,7 +331,6 @@ __global__ __launch_bounds__(256) void k_scan_runs(Bins bn, int ncell, int ntile
         ctrl->n_next = (uint32_t)carry;
         ctrl->n_runs = (uint32_t)(carry >> 32);
         ctrl->nruns += carry >> 32;
-        ctrl->scan_done = 0u;
     }
 }
 
@@ -340,8 +372,14 @@ __device__ __forceinline__ void xcd_range(int ntask, int &first, int &end, int &
 // ---------------------------------------------------------------------------
 constexpr int PLACE_U = 4;
 
-__global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__ ctrl)
+__global__ __launch_bounds__(256) void k_place(Bins bn, int ncell, Ctrl *__restrict__ ctrl)
 {
+    __shared__ unsigned long long s_toff[TOFF_LDS];
+    __shared__ unsigned long long s_w[4];
+    const int ntile = (ncell + SCAN_TILE - 1) / SCAN_TILE;
+    const bool lds = ntile <= TOFF_LDS;     // else k_scan_tiles has published them
+    if (lds) tile_offsets(bn, ntile, s_toff, s_w, blockIdx.x == 0, ctrl);
+    const unsigned long long *const toff = lds ? s_toff : bn.toff;
     const int lane = lane_id();
     const uint32_t nch = (ctrl->n_move + 63u) >> 6;
     // chunks per task: 64, halved (down to 4) while that leaves fewer than
@@ -380,7 +418,7 @@ __global__ __launch_bounds__(256) void k_place(Bins bn, const Ctrl *__restrict__
 #pragma unroll
             for (int u = 0; u < PLACE_U; ++u)
                 if (ri[u] != ~(size_t)0) {
-                    const unsigned long long v = bn.lstart[r[u].x] + bn.toff[r[u].x / SCAN_TILE];
+                    const unsigned long long v = bn.lstart[r[u].x] + toff[r[u].x / SCAN_TILE];
                     bn.runlist[(uint32_t)(v >> 32) + (uint32_t)r[u].y] = make_int4((int)ri[u], 0, r[u].z, r[u].w);
                 }
         }
@@ -908,7 +946,10 @@ hipError_t launch_scan(int *count, int *lstart, int *tsum, int *toff, int ncell,
 hipError_t launch_scan_runs(const Bins &bn, int ncell, Ctrl *ctrl, hipStream_t st)
 {
     const int ntile = (ncell + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_runs, dim3(ntile), dim3(256), 0, st, bn, ncell, ntile, ctrl);
+    hipLaunchKernelGGL(k_scan_runs, dim3(ntile), dim3(256), 0, st, bn, ncell);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ntile <= TOFF_LDS) return e;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, st, bn, ntile, ctrl);
     return hipGetLastError();
 }
 
@@ -916,7 +957,7 @@ hipError_t launch_scatter(const Bins &bn, int *start, int *tile_first, int ncell
                           int grid, hipStream_t st)
 {
     if (tile_p < CELL_P_MIN || tile_p > CELL_CAP) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_place, dim3(grid), dim3(256), 0, st, bn, ctrl);
+    hipLaunchKernelGGL(k_place, dim3(grid), dim3(256), 0, st, bn, ncell, const_cast<Ctrl *>(ctrl));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_rank, dim3(grid), dim3(256), 0, st, bn, start, reinterpret_cast<int4 *>(tile_first), ncell,
