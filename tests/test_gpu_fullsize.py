@@ -95,6 +95,24 @@ def test_box_64x64_bitexact(gpu_lib):
     r.close()
 
 
+def test_grid_beyond_lds_tile_offsets(gpu_lib):
+    """A 2,064 x 2,064 box (4.26M cells, 2,081 scan tiles of 2,048 cells:
+    more tile offsets than k_place scans in LDS, so the one-block tile scan
+    publishes them): a few steps, then one step phase by phase -- counts,
+    scan offsets, stable permutation and state bit-exact against the
+    oracle."""
+    p, body = T.small_box(2064, 2, seed=3)
+    print("[fullsize] box 2064x2064, 8.5M particles, scan tiles beyond the LDS", flush=True)
+    r = Run(p, body)
+    r.initialize()
+    r.solver(3, graph=False)
+    oc, info, _ = O.case_from_params(p, body)
+    assert info.ncell > 2048 * 2048
+    ref = _phase_step(r, oc, info, "box 2064x2064 ")
+    assert int(ref["count"].sum()) == r.status().n_live
+    r.close()
+
+
 def test_cylinder_400x200_bitexact(gpu_lib):
     cfg = cases.cylinder_mach10()
     p = cfg["params"]
