@@ -13,7 +13,8 @@
 //   2. NTC/VHS collisions (collisionInCells, GPUCell.cu:200-333, with the
 //      host semantics of Cell::collision, Cell.cpp:335-409): candidate counts
 //      one lane per cell; cells with up to 64 candidates run in groups of 8
-//      lanes, heavier cells by a whole wave, both in conflict-free rounds
+//      lanes, heavier cells by a whole wave, both in rounds that end at the
+//      first candidate sharing a particle with an earlier accepted one,
 //      whose results equal the serial candidate loop bit for bit;
 //   3. FP64 sampling sums, 8 lanes per cell joined by a fixed butterfly
 //      (samplingInCells, GPUCell.cu:336-402, as time-averaged accumulators);
@@ -135,8 +136,8 @@ __device__ __forceinline__ float vhs_sigc(const CK &k, float g2)
 }
 
 // The serial candidate loop of many cells at once: a wave takes 64/G cells,
-// G lanes per cell, and runs conflict-free rounds of up to G candidates in
-// each cell (the same rule as ntc_rounds, within G-lane segments).  list[]
+// G lanes per cell, and runs rounds of up to G candidates in each cell (the
+// same rule as ntc_rounds, within G-lane segments).  list[]
 // holds the chunk-local cell numbers; results go to sh.sm / sh.nc.
 template <int G, bool GLOBAL, bool ROT, class KF>
 __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list, int nlist, CellShared &sh,
@@ -169,19 +170,10 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
                 i2 = mulhi_idx(w.y, np - 1);
                 if (i2 >= i1) i2 = i2 + 1;
             }
-            bool conf = false;
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                const int a = __shfl(i1, j, G), b = __shfl(i2, j, G);
-                if (j < sub && j < nact && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
-            }
-            const unsigned long long cm = __ballot(act && conf) & gmask;
-            const int f = cm ? (int)__ffsll((long long)cm) - 1 - g * G : nact;
-            const bool proc = sub < f;
             float sc = 0.0f, g2 = 0.0f;
             float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
             Scatter sd{ 0.f, 0.f, 0.f, 0.f };
-            if (proc) {
+            if (act) {
                 v1x = vx[i1]; v1y = vy[i1]; v1z = vz[i1];
                 v2x = vx[i2]; v2y = vy[i2]; v2z = vz[i2];
                 const float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
@@ -196,7 +188,20 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
                 if (sub >= d) pm = fmaxf(pm, t);
             }
             const float sml = fmaxf(smax, pm);
-            const bool accd = proc && (u01(w.z) < sc / sml);
+            const bool acc = act && (u01(w.z) < sc / sml);
+            // the round ends before the first candidate sharing a particle
+            // with an earlier accepted one (a rejected candidate changes no
+            // velocity, so its neighbours in the round read current values)
+            const int a1 = acc ? i1 : -3, a2 = acc ? i2 : -4;
+            bool conf = false;
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const int a = __shfl(a1, j, G), b = __shfl(a2, j, G);
+                if (j < sub && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
+            }
+            const unsigned long long cm = __ballot(act && conf) & gmask;
+            const int f = cm ? (int)__ffsll((long long)cm) - 1 - g * G : nact;
+            const bool accd = acc && sub < f;
             if (accd) {
                 if constexpr (ROT) {
                     float e1 = er[i1], e2 = er[i2];
@@ -218,8 +223,10 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
 }
 
 // The same loop by a whole wave: each round takes the longest prefix of the
-// next <= 64 candidates sharing no particle with an earlier candidate of the
-// round; (sigma c)max becomes a prefix max.  Identical results.
+// next <= 64 candidates sharing no particle with an earlier accepted
+// candidate of the round; (sigma c)max becomes a prefix max.  Within that
+// prefix every candidate reads the velocities the serial loop would, so the
+// results are identical.
 // GLOBAL: velocities in HBM; LMARK: conflict marks in LDS (else found by
 // shuffles).
 template <bool GLOBAL, bool ROT, bool LMARK = !GLOBAL>
@@ -229,7 +236,8 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
     const int lane = lane_id();
     int nc = 0;
     if (LMARK) {
-        // mark[q] = lowest lane of the round that uses particle q (64 = none)
+        // mark[q] = lowest accepted lane of the round that uses particle q
+        // (64 = none)
         for (int q = lane; q < np; q += 64) mark[q] = 64;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -246,28 +254,10 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
             i2 = mulhi_idx(w.y, np - 1);
             if (i2 >= i1) i2 = i2 + 1;
         }
-        bool conf = false;
-        if (LMARK) {
-            if (act) { atomicMin(&mark[i1], lane); atomicMin(&mark[i2], lane); }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (act) conf = mark[i1] < lane || mark[i2] < lane;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (act) { mark[i1] = 64; mark[i2] = 64; }
-        } else {
-            for (int j = 0; j < nact; ++j) {
-                int a = __shfl(i1, j, 64), b = __shfl(i2, j, 64);
-                if (j < lane && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
-            }
-        }
-        const unsigned long long cm = __ballot(act && conf);
-        const int f = cm ? (int)__ffsll((long long)cm) - 1 : nact;
-        const bool proc = lane < f;
         float sc = 0.0f, g2 = 0.0f;
         float v1x = 0.f, v1y = 0.f, v1z = 0.f, v2x = 0.f, v2y = 0.f, v2z = 0.f;
         Scatter sd{ 0.f, 0.f, 0.f, 0.f };
-        if (proc) {
+        if (act) {
             v1x = Vx[i1]; v1y = Vy[i1]; v1z = Vz[i1];
             v2x = Vx[i2]; v2y = Vy[i2]; v2z = Vz[i2];
             float gx = v1x - v2x, gy = v1y - v2y, gz = v1z - v2z;
@@ -282,7 +272,26 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
             if (lane >= d) pm = fmaxf(pm, t);
         }
         const float sml = fmaxf(smax, pm);
-        const bool accd = proc && (u01(w.z) < sc / sml);
+        const bool acc = act && (u01(w.z) < sc / sml);
+        bool conf = false;
+        if (LMARK) {
+            if (acc) { atomicMin(&mark[i1], lane); atomicMin(&mark[i2], lane); }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (act) conf = mark[i1] < lane || mark[i2] < lane;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (acc) { mark[i1] = 64; mark[i2] = 64; }
+        } else {
+            const int a1 = acc ? i1 : -3, a2 = acc ? i2 : -4;
+            for (int j = 0; j < nact; ++j) {
+                int a = __shfl(a1, j, 64), b = __shfl(a2, j, 64);
+                if (j < lane && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
+            }
+        }
+        const unsigned long long cm = __ballot(act && conf);
+        const int f = cm ? (int)__ffsll((long long)cm) - 1 : nact;
+        const bool accd = acc && lane < f;
         if (accd) {
             if constexpr (ROT) {
                 float e1 = Er[i1], e2 = Er[i2];
