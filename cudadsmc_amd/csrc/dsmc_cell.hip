@@ -37,6 +37,15 @@ constexpr int GATHER_UNROLL = 4;     // particles per thread with loads in fligh
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// 0, computed from v by an instruction the compiler cannot see through: a
+// use of v that makes the wave wait for v (an atomic's old value)
+__device__ __forceinline__ uint32_t opaque_zero(uint32_t v)
+{
+    uint32_t z;
+    asm volatile("v_mul_lo_u32 %0, %1, 0" : "=v"(z) : "v"(v));
+    return z;
+}
+
 struct CellShared {
     int st[CELL_MAXC + 1];           // cell starts relative to the tile
     float sm[CELL_MAXC], mr[CELL_MAXC], vol[CELL_MAXC];
@@ -380,15 +389,19 @@ struct KBatch {
     uint32_t step;
     const CellConst *gcc;    // every instance's constants (global)
     InstCtrl *ictl;
-    // Adds the window's sums to the instance counters and clears them.
-    // Called by wave 0 only, after a block barrier that ended every add.
-    __device__ void flush() const
+    // Adds the window's sums to the instance counters.  Called by wave 0
+    // only, after a block barrier that ended every add.  Returns the OR of
+    // the atomics' old values: a caller that uses it has waited for every
+    // add of the wave to be performed.
+    __device__ uint32_t flush() const
     {
         const int t = (int)threadIdx.x;
+        uint32_t dep = 0u;
         if (t < b.wn) {
-            if (b.icoll[t]) atomicAdd(&ictl[b.w0 + t].step_coll, b.icoll[t]);
-            if (b.icand[t]) atomicAdd(&ictl[b.w0 + t].step_cand, b.icand[t]);
+            if (b.icoll[t]) dep |= atomicAdd(&ictl[b.w0 + t].step_coll, b.icoll[t]);
+            if (b.icand[t]) dep |= atomicAdd(&ictl[b.w0 + t].step_cand, b.icand[t]);
         }
+        return dep;
     }
     // The previous chunk ended in a block barrier.  Wave 0 flushes the old
     // window and then stages the new one: one wave runs in order, so its
@@ -791,18 +804,23 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     }
     if (tcoll) atomicAdd(&sh.coll, tcoll);
     if (tcand) atomicAdd(&sh.cand, tcand);
+    // The block's counter adds are returning atomics whose old values feed
+    // the cell_done increment (through a zero the compiler cannot fold): an
+    // atomic's value returns once it is performed device-wide, so the last
+    // block sees every add.  A device-scope release here instead writes
+    // back the XCD's L2 once per exiting block, which slowed every tile
+    // still running while the blocks ran out of tiles.
+    uint32_t dep = 0u;
     if constexpr (BATCH) {
         __syncthreads();
-        if (tid < 64) KBatch{ bsh, sh.inst, a.ninst, step, a.cc, a.ictl }.flush();
-        __threadfence();
+        if (tid < 64) dep = KBatch{ bsh, sh.inst, a.ninst, step, a.cc, a.ictl }.flush();
     }
     __syncthreads();
     bool last = false;
     if (tid == 0) {
-        if (sh.coll) atomicAdd(&ctrl->step_coll, sh.coll);
-        if (sh.cand) atomicAdd(&ctrl->step_cand, sh.cand);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        last = atomicAdd(&ctrl->cell_done, 1u) == gridDim.x - 1;
+        if (sh.coll) dep |= atomicAdd(&ctrl->step_coll, sh.coll);
+        if (sh.cand) dep |= atomicAdd(&ctrl->step_cand, sh.cand);
+        last = atomicAdd(&ctrl->cell_done, 1u + opaque_zero(dep)) == gridDim.x - 1;
         if constexpr (BATCH) bsh.last = last ? 1 : 0;
     }
     if constexpr (BATCH) {
