@@ -284,7 +284,7 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
         (e = h->alloc(&h->toff, h->ntile_scan)) || (e = h->alloc(&h->start, 2 * ((size_t)C + 1))) ||
         (e = h->alloc(&h->tile_first, 4 * (size_t)ntiles_cell)) || (e = h->alloc(&h->ncoll_cell, C)) ||
         (e = h->alloc(&h->sigcmax, C)) || (e = h->alloc(&h->mrem, C)) || (e = h->alloc(&h->acc, (size_t)P.sample_blocks * C)) ||
-        (e = h->alloc(&h->ctrl, 1)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
+        (e = h->alloc(&h->ctrl, CTRL_ALLOC_COUNT)) || (e = h->alloc(&h->d_seg, T.seg.size())) ||
         (e = h->alloc(&h->d_cseg_start, C + 1)) || (e = h->alloc(&h->d_cseg_idx, T.cseg_idx.size())) ||
         (e = h->alloc(&h->d_vol_ntc, C)) || (e = h->alloc(&h->d_cut, C)) || (e = h->alloc(&h->d_fbox, 4 * C)) || (e = h->alloc(&h->d_fmask, C)) ||
         (e = h->alloc(&h->d_expected, C)) || (e = h->alloc(&h->d_ux, C)) || (e = h->alloc(&h->d_uy, C)) ||
@@ -299,7 +299,7 @@ int create_handle(const dsmc_params *p, Build build, dsmc_handle *out)
         (e = up(h->d_vol_ntc, T.vol_ntc.data(), sizeof(float) * C)) ||
         (e = up(h->d_cut, T.cut.data(), C)) || (e = up(h->d_fbox, T.fbox.data(), sizeof(float) * 4 * C)) ||
         (e = up(h->d_fmask, T.fmask.data(), sizeof(uint64_t) * C)) ||
-        (e = hipMemset(h->ctrl, 0, sizeof(Ctrl))))
+        (e = hipMemset(h->ctrl, 0, CTRL_ALLOC_BYTES)))
         return bad(e, "table upload");
 
     DevCase &cs = h->cs;

@@ -312,7 +312,7 @@ int setup_device(dsmc_batch b, dsmc_batch *out)
         (e = b->alloc(&b->toff, b->ntile_scan)) || (e = b->alloc(&b->start, 2 * ((size_t)Cs + 1))) ||
         (e = b->alloc(&b->tile_first, 4 * (size_t)ntiles_cell)) || (e = b->alloc(&b->ncoll_cell, Cs)) ||
         (e = b->alloc(&b->sigcmax, Cs)) || (e = b->alloc(&b->mrem, Cs)) || (e = b->alloc(&b->acc, 6 * Cs)) ||
-        (e = b->alloc(&b->ctrl, 1)) || (e = b->alloc(&b->fill_ctrl, 1)) || (e = b->alloc(&b->ictl, n)) ||
+        (e = b->alloc(&b->ctrl, CTRL_ALLOC_COUNT)) || (e = b->alloc(&b->fill_ctrl, 1)) || (e = b->alloc(&b->ictl, n)) ||
         (e = b->alloc(&b->d_cases, n)) || (e = b->alloc(&b->d_cc, n)) || (e = b->alloc(&b->d_cbase, n + 1)) ||
         (e = b->alloc(&b->d_seg, segs)) || (e = b->alloc(&b->d_cseg_start, Cs + n)) ||
         (e = b->alloc(&b->d_cseg_idx, idx)) || (e = b->alloc(&b->d_vol_ntc, Cs)) || (e = b->alloc(&b->d_cut, Cs)) ||
@@ -356,7 +356,7 @@ int setup_device(dsmc_batch b, dsmc_batch *out)
     if ((e = hipMemcpy(b->d_cases, b->cs.data(), sizeof(DevCase) * n, hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(b->d_cc, cc.data(), sizeof(CellConst) * n, hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(b->d_cbase, b->cbase.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice)) ||
-        (e = hipMemset(b->ctrl, 0, sizeof(Ctrl))) || (e = hipMemset(b->ictl, 0, sizeof(InstCtrl) * n)))
+        (e = hipMemset(b->ctrl, 0, CTRL_ALLOC_BYTES)) || (e = hipMemset(b->ictl, 0, sizeof(InstCtrl) * n)))
         return bad(e, "table upload");
 
     int cu = 0;

@@ -804,12 +804,16 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     }
     if (tcoll) atomicAdd(&sh.coll, tcoll);
     if (tcand) atomicAdd(&sh.cand, tcand);
-    // The block's counter adds are returning atomics whose old values feed
-    // the cell_done increment (through a zero the compiler cannot fold): an
-    // atomic's value returns once it is performed device-wide, so the last
-    // block sees every add.  A device-scope release here instead writes
-    // back the XCD's L2 once per exiting block, which slowed every tile
-    // still running while the blocks ran out of tiles.
+    // End of the step.  Each block adds its sums into its group's line and
+    // counts itself done there; the last block of a group moves the group's
+    // sums into the step counters and counts the group done in cell_done.
+    // Every add is a returning atomic whose old value feeds the next
+    // increment (through a zero the compiler cannot fold): an atomic's value
+    // returns once it is performed device-wide, so each counter is complete
+    // when the increment after it is seen, with no device-scope release
+    // (which writes back the XCD's L2, once per exiting block).  The groups
+    // keep the ~1,000 exiting blocks off one line: there their atomics
+    // queued for tens of microseconds and held up the tiles still running.
     uint32_t dep = 0u;
     if constexpr (BATCH) {
         __syncthreads();
@@ -818,9 +822,21 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     __syncthreads();
     bool last = false;
     if (tid == 0) {
-        if (sh.coll) dep |= atomicAdd(&ctrl->step_coll, sh.coll);
-        if (sh.cand) dep |= atomicAdd(&ctrl->step_cand, sh.cand);
-        last = atomicAdd(&ctrl->cell_done, 1u + opaque_zero(dep)) == gridDim.x - 1;
+        const uint32_t ng = min(gridDim.x, (uint32_t)EXIT_GROUPS);
+        const uint32_t g = blockIdx.x % ng;
+        const uint32_t gsize = (gridDim.x - g + ng - 1u) / ng;
+        unsigned long long *eg = exit_group(ctrl, (int)g);
+        uint32_t *const gdone = reinterpret_cast<uint32_t *>(eg + 1);
+        const unsigned long long o = atomicAdd(eg, (unsigned long long)sh.coll | ((unsigned long long)sh.cand << 32));
+        dep |= (uint32_t)o | (uint32_t)(o >> 32);
+        if (atomicAdd(gdone, 1u + opaque_zero(dep)) == gsize - 1u) {
+            const unsigned long long t = atomicExch(eg, 0ull);
+            atomicExch(gdone, 0u);
+            uint32_t dep2 = (uint32_t)t | (uint32_t)(t >> 32);
+            if ((uint32_t)t) dep2 |= atomicAdd(&ctrl->step_coll, (uint32_t)t);
+            if ((uint32_t)(t >> 32)) dep2 |= atomicAdd(&ctrl->step_cand, (uint32_t)(t >> 32));
+            last = atomicAdd(&ctrl->cell_done, 1u + opaque_zero(dep2)) == ng - 1u;
+        }
         if constexpr (BATCH) bsh.last = last ? 1 : 0;
     }
     if constexpr (BATCH) {

@@ -93,6 +93,21 @@ struct Ctrl {
     unsigned long long nruns;      // runs binned over all steps (algorithmic byte counts)
 };
 
+// k_cell counts the end of a step in EXIT_GROUPS groups of blocks, each
+// group on its own 4 KB line after the Ctrl (sums of collisions and
+// candidates, blocks done), so that no single line takes an atomic from
+// every block.  A Ctrl allocation is CTRL_ALLOC_BYTES, zeroed at creation;
+// the last block of a group clears its line.
+constexpr int EXIT_GROUPS = 16;
+constexpr size_t EXIT_OFFSET = 4096, EXIT_STRIDE = 4096;
+constexpr size_t CTRL_ALLOC_BYTES = EXIT_OFFSET + EXIT_GROUPS * EXIT_STRIDE;
+constexpr size_t CTRL_ALLOC_COUNT = (CTRL_ALLOC_BYTES + sizeof(Ctrl) - 1) / sizeof(Ctrl);
+static_assert(sizeof(Ctrl) <= EXIT_OFFSET, "the exit groups follow the Ctrl");
+__host__ __device__ inline unsigned long long *exit_group(Ctrl *c, int g)
+{
+    return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(c) + EXIT_OFFSET + (size_t)g * EXIT_STRIDE);
+}
+
 // Binning of one step.  k_move groups the lanes of each wave (64 consecutive
 // particles, a "chunk") by cell: each group is a run, counted with one 64-bit
 // atomic (runs << 32 | particles) per cell and recorded, compacted per chunk,
