@@ -37,6 +37,68 @@ constexpr int GATHER_UNROLL = 4;     // particles per thread with loads in fligh
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Cross-lane moves within a row of 16 lanes by DPP (a VALU operand
+// modifier, no LDS round trip).  row_shr:k (0x110 + k) gives lane i the
+// value of lane i - k of its row; row_bcast:15 / :31 (0x142 / 0x143, rows
+// chosen by the row mask) give a row lane 15 / 31.  A lane whose source is
+// outside the row, or whose row is masked, gets 0.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, true);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_f(float x)
+{
+    return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(x)));
+}
+
+// Inclusive prefix max over the wave of values >= 0 (a 0 from outside a
+// row leaves a max unchanged).
+__device__ __forceinline__ float wave_prefix_max(float x)
+{
+    x = fmaxf(x, dpp_f<0x111>(x));
+    x = fmaxf(x, dpp_f<0x112>(x));
+    x = fmaxf(x, dpp_f<0x114>(x));
+    x = fmaxf(x, dpp_f<0x118>(x));
+    x = fmaxf(x, dpp_f<0x142, 0xa>(x));
+    x = fmaxf(x, dpp_f<0x143, 0xc>(x));
+    return x;
+}
+
+// The same within aligned segments of G lanes (G = 8 or 16: within a row),
+// sub = lane % G.
+template <int G>
+__device__ __forceinline__ float seg_prefix_max(float x, int sub)
+{
+    static_assert(G == 8 || G == 16, "segments within a row of 16 lanes");
+    float t = dpp_f<0x111>(x);
+    if (sub >= 1) x = fmaxf(x, t);
+    t = dpp_f<0x112>(x);
+    if (sub >= 2) x = fmaxf(x, t);
+    t = dpp_f<0x114>(x);
+    if (sub >= 4) x = fmaxf(x, t);
+    if constexpr (G == 16) {
+        t = dpp_f<0x118>(x);
+        if (sub >= 8) x = fmaxf(x, t);
+    }
+    return x;
+}
+
+// Whether an earlier lane of the segment (lane - k, 1 <= k <= sub) holds a1
+// or a2 equal to i1 or i2.
+template <int K, int G>
+__device__ __forceinline__ bool seg_earlier_hit(int a1, int a2, int i1, int i2, int sub)
+{
+    if constexpr (K >= G) {
+        return false;
+    } else {
+        const int a = dpp_i<0x110 + K>(a1), b = dpp_i<0x110 + K>(a2);
+        const bool hit = K <= sub && (a == i1 || a == i2 || b == i1 || b == i2);
+        return hit || seg_earlier_hit<K + 1, G>(a1, a2, i1, i2, sub);
+    }
+}
+
 // 0, computed from v by an instruction the compiler cannot see through: a
 // use of v that makes the wave wait for v (an atomic's old value)
 __device__ __forceinline__ uint32_t opaque_zero(uint32_t v)
@@ -190,24 +252,14 @@ __device__ void ntc_groups(const KF &kf, uint32_t step, int c0, const int *list,
                 sc = vhs_sigc(k, g2);
                 sd = scatter_dir(k, step, m, w.w, g2);
             }
-            float pm = sc;
-#pragma unroll
-            for (int d = 1; d < G; d <<= 1) {
-                const float t = __shfl_up(pm, d, G);
-                if (sub >= d) pm = fmaxf(pm, t);
-            }
+            const float pm = seg_prefix_max<G>(sc, sub);
             const float sml = fmaxf(smax, pm);
             const bool acc = act && (u01(w.z) < sc / sml);
             // the round ends before the first candidate sharing a particle
             // with an earlier accepted one (a rejected candidate changes no
             // velocity, so its neighbours in the round read current values)
             const int a1 = acc ? i1 : -3, a2 = acc ? i2 : -4;
-            bool conf = false;
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                const int a = __shfl(a1, j, G), b = __shfl(a2, j, G);
-                if (j < sub && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
-            }
+            const bool conf = seg_earlier_hit<1, G>(a1, a2, i1, i2, sub);
             const unsigned long long cm = __ballot(act && conf) & gmask;
             const int f = cm ? (int)__ffsll((long long)cm) - 1 - g * G : nact;
             const bool accd = acc && sub < f;
@@ -274,12 +326,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
             sc = vhs_sigc(k, g2);
             sd = scatter_dir(k, step, m, w.w, g2);
         }
-        float pm = sc;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            float t = __shfl_up(pm, d, 64);
-            if (lane >= d) pm = fmaxf(pm, t);
-        }
+        const float pm = wave_prefix_max(sc);
         const float sml = fmaxf(smax, pm);
         const bool acc = act && (u01(w.z) < sc / sml);
         bool conf = false;
@@ -294,7 +341,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
         } else {
             const int a1 = acc ? i1 : -3, a2 = acc ? i2 : -4;
             for (int j = 0; j < nact; ++j) {
-                int a = __shfl(a1, j, 64), b = __shfl(a2, j, 64);
+                const int a = __builtin_amdgcn_readlane(a1, j), b = __builtin_amdgcn_readlane(a2, j);
                 if (j < lane && (a == i1 || a == i2 || b == i1 || b == i2)) conf = true;
             }
         }
@@ -312,7 +359,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
             Vx[i2] = v2x; Vy[i2] = v2y; Vz[i2] = v2z;
         }
         nc += __popcll(__ballot(accd));
-        smax = fmaxf(smax, __shfl(pm, f - 1, 64));
+        smax = fmaxf(smax, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pm), f - 1)));
         mb += f;
         if (GLOBAL) __threadfence_block();
         else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
