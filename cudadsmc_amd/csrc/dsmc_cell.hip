@@ -94,9 +94,33 @@ __device__ __forceinline__ bool seg_earlier_hit(int a1, int a2, int i1, int i2, 
         return false;
     } else {
         const int a = dpp_i<0x110 + K>(a1), b = dpp_i<0x110 + K>(a2);
-        const bool hit = K <= sub && (a == i1 || a == i2 || b == i1 || b == i2);
+        const bool hit = (K <= sub) & ((a == i1) | (a == i2) | (b == i1) | (b == i2));
         return hit || seg_earlier_hit<K + 1, G>(a1, a2, i1, i2, sub);
     }
+}
+
+// The value of lane ^ D (D = 1, 2 or 4: within a segment of 8 lanes), by
+// DPP with every lane active: quad_perm [1,0,3,2] (0xb1) and [2,3,0,1]
+// (0x4e) for 1 and 2; for 4, row_half_mirror (0x141: lane i of a segment
+// takes lane 7 - i) and then quad_perm [3,2,1,0] (0x1b: i takes i ^ 3),
+// since 7 - (i ^ 3) = i ^ 4.  Exactly what __shfl_xor(x, D, 8) returns.
+// (A DPP move must not sit in a branch: the lanes it reads from would be
+// inactive there and read as 0.)
+template <int D>
+__device__ __forceinline__ int xor_lane_i(int x)
+{
+    static_assert(D == 1 || D == 2 || D == 4, "a partner within 8 lanes");
+    if constexpr (D == 1) return dpp_i<0xb1>(x);
+    else if constexpr (D == 2) return dpp_i<0x4e>(x);
+    else return dpp_i<0x1b>(dpp_i<0x141>(x));
+}
+template <int D>
+__device__ __forceinline__ double xor_lane_d(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)xor_lane_i<D>((int)(unsigned)b);
+    const unsigned hi = (unsigned)xor_lane_i<D>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // 0, computed from v by an instruction the compiler cannot see through: a
@@ -706,14 +730,20 @@ __device__ void cell_tile(const CellArgs &a, const KF &kf, const SoA &cur, const
                         s2 += u * u + v * v + w * w;
                         if constexpr (ROT) se += (double)Er[o + q];
                     }
-#pragma unroll
-                for (int d = SAMPLE_G / 2; d > 0; d >>= 1) {
-                    sx += __shfl_xor(sx, d, SAMPLE_G);
-                    sy += __shfl_xor(sy, d, SAMPLE_G);
-                    sz += __shfl_xor(sz, d, SAMPLE_G);
-                    s2 += __shfl_xor(s2, d, SAMPLE_G);
-                    if constexpr (ROT) se += __shfl_xor(se, d, SAMPLE_G);
-                }
+                // the butterfly over 4, 2, 1 by DPP (each lane adds its
+                // partner's value, as a shuffle would give it)
+                static_assert(SAMPLE_G == 8, "the butterfly spans 8 lanes");
+                auto fold = [&](auto dc) {
+                    constexpr int D = decltype(dc)::value;
+                    sx += xor_lane_d<D>(sx);
+                    sy += xor_lane_d<D>(sy);
+                    sz += xor_lane_d<D>(sz);
+                    s2 += xor_lane_d<D>(s2);
+                    if constexpr (ROT) se += xor_lane_d<D>(se);
+                };
+                fold(std::integral_constant<int, 4>{});
+                fold(std::integral_constant<int, 2>{});
+                fold(std::integral_constant<int, 1>{});
                 // one lane per cell adds into the accumulators: the FP64
                 // adds are done at the memory side (the same IEEE sum, one
                 // writer per cell per step), so nothing waits for a load
