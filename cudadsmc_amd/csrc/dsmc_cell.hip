@@ -37,22 +37,6 @@ constexpr int GATHER_UNROLL = 4;     // particles per thread with loads in fligh
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// Cross-lane moves within a row of 16 lanes by DPP (a VALU operand
-// modifier, no LDS round trip).  row_shr:k (0x110 + k) gives lane i the
-// value of lane i - k of its row; row_bcast:15 / :31 (0x142 / 0x143, rows
-// chosen by the row mask) give a row lane 15 / 31.  A lane whose source is
-// outside the row, or whose row is masked, gets 0.
-template <int CTRL, int ROWS = 0xf>
-__device__ __forceinline__ int dpp_i(int x)
-{
-    return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, true);
-}
-template <int CTRL, int ROWS = 0xf>
-__device__ __forceinline__ float dpp_f(float x)
-{
-    return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(x)));
-}
-
 // Inclusive prefix max over the wave of values >= 0 (a 0 from outside a
 // row leaves a max unchanged).
 __device__ __forceinline__ float wave_prefix_max(float x)

@@ -494,4 +494,52 @@ __device__ __forceinline__ bool inject_one(const DevCase &cs, int f, uint32_t pi
     return move_one(cs, x, y, vx, vy, vz, er, nullptr, pid, true, step, ufrac * cs.dt, err, vchg, echg);
 }
 
+// Cross-lane moves within a row of 16 lanes by DPP (a VALU operand
+// modifier, no LDS round trip).  row_shr:k (0x110 + k) gives lane i the
+// value of lane i - k of its row; row_bcast:15 / :31 (0x142 / 0x143, rows
+// chosen by the row mask) give a row lane 15 / 31.  A lane whose source is
+// outside the row, or whose row is masked, gets 0.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, true);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_f(float x)
+{
+    return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(x)));
+}
+
+// Inclusive prefix sums over the wave (0 from outside a row adds nothing).
+__device__ __forceinline__ int wave_prefix_sum(int x)
+{
+    x += dpp_i<0x111>(x);
+    x += dpp_i<0x112>(x);
+    x += dpp_i<0x114>(x);
+    x += dpp_i<0x118>(x);
+    x += dpp_i<0x142, 0xa>(x);
+    x += dpp_i<0x143, 0xc>(x);
+    return x;
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x)
+{
+    const unsigned lo = (unsigned)dpp_i<CTRL, ROWS>((int)(unsigned)x);
+    const unsigned hi = (unsigned)dpp_i<CTRL, ROWS>((int)(unsigned)(x >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long wave_prefix_sum_u64(unsigned long long x)
+{
+    x += dpp_u64<0x111>(x);
+    x += dpp_u64<0x112>(x);
+    x += dpp_u64<0x114>(x);
+    x += dpp_u64<0x118>(x);
+    x += dpp_u64<0x142, 0xa>(x);
+    x += dpp_u64<0x143, 0xc>(x);
+    return x;
+}
+
+// A wave-uniform lane's value (readlane: no LDS round trip).
+__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
 }  // namespace dsmc

@@ -67,7 +67,7 @@ __device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i)
     unsigned long long mine = 0ull;
     while (todo) {
         const int l = (int)__ffsll((long long)todo) - 1;
-        const int lc = __shfl(c, l, 64);
+        const int lc = lane_value(c, l);
         const unsigned long long m = __ballot(c == lc);
         if (lane == l) mine = m;
         todo &= ~m;
@@ -146,16 +146,7 @@ __global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, Bins bn,
 // and the last block to finish scans the tile sums.  count is zeroed for the
 // next step.  start(c) = lstart[c] + toff[c / SCAN_TILE].
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int wave_incl_scan(int v)
-{
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
-}
+__device__ __forceinline__ int wave_incl_scan(int v) { return wave_prefix_sum(v); }
 
 // Exclusive scan over the 256 threads of a block; returns the block total.
 __device__ __forceinline__ int block_excl_scan(int v, int &excl, int *s_w)
@@ -238,12 +229,7 @@ __device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long lo
                                                                 unsigned long long *s_w)
 {
     const int lane = lane_id(), w = threadIdx.x >> 6;
-    unsigned long long inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned long long t = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += t;
-    }
+    const unsigned long long inc = wave_prefix_sum_u64(v);
     if (lane == 63) s_w[w] = inc;
     __syncthreads();
     unsigned long long wofs = 0, total = 0;
@@ -394,7 +380,7 @@ __global__ __launch_bounds__(256) void k_place(Bins bn, int ncell, Ctrl *__restr
         const uint32_t q0 = (uint32_t)task * cpw;
         const int nr = (uint32_t)lane < cpw && q0 + lane < nch ? bn.chunk_runs[q0 + lane] : 0;
         const int inc = wave_incl_scan(nr), excl = inc - nr;
-        const int total = __shfl(inc, 63, 64);
+        const int total = lane_value(inc, 63);
         // PLACE_U runs per lane per pass, their record loads in flight together
         for (int t0 = 0; t0 < total; t0 += 64 * PLACE_U) {
             size_t ri[PLACE_U];
@@ -486,16 +472,16 @@ __global__ __launch_bounds__(256) void k_rank(Bins bn, int *__restrict__ start, 
             for (int t = tlo; t <= thi; ++t) tiles[t] = make_int4(c, st, rs, 0);
         }
         if (task + stride < end) starts(task + stride, n0, np0);
-        const int R0 = __shfl(rs, 0, 64);
+        const int R0 = lane_value(rs, 0);
         const int rel = rs - R0, rend = rel + L;      // the lane's cell's runs [rel, rend), from R0
         // segments of consecutive cells whose runs fit the staging area (a
         // cell of more runs than that alone, read from runlist itself)
         for (int s0 = 0; s0 < RK_C;) {
-            const int base = __shfl(rel, s0, 64);
+            const int base = lane_value(rel, s0);
             const unsigned long long fit = __ballot(lane >= s0 && rend - base <= RK_LDS);
             const bool lds = fit != 0ull;                 // else cell s0 alone has too many runs
             const int e = lds ? 64 - (int)__clzll((long long)fit) : s0 + 1;
-            const int nr = __shfl(rend, e - 1, 64) - base;
+            const int nr = lane_value(rend, e - 1) - base;
             int4 *const rl = bn.runlist + R0 + base;
             if (lds) {
                 for (int k0 = 0; k0 < nr; k0 += 256) {
