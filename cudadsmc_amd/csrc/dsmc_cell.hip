@@ -116,6 +116,28 @@ __device__ __forceinline__ uint32_t opaque_zero(uint32_t v)
     return z;
 }
 
+// Called by one thread of every block at the block's end: adds (coll,
+// cand) into the block's exit group (blocks g, g + ng, ... with ng =
+// min(grid, EXIT_GROUPS), each group on its own line) and counts the block
+// done there.  dep carries old values of earlier atomics that must be
+// performed first.  In the last block of a group it returns true with the
+// group's sums in tot (coll | cand << 32) and the line cleared for the next
+// launch.
+__device__ bool exit_group_add(Ctrl *ctrl, uint32_t coll, uint32_t cand, uint32_t dep, unsigned long long &tot)
+{
+    const uint32_t ng = min(gridDim.x, (uint32_t)EXIT_GROUPS);
+    const uint32_t g = blockIdx.x % ng;
+    const uint32_t gsize = (gridDim.x - g + ng - 1u) / ng;
+    unsigned long long *eg = exit_group(ctrl, (int)g);
+    uint32_t *const gdone = reinterpret_cast<uint32_t *>(eg + 1);
+    const unsigned long long o = atomicAdd(eg, (unsigned long long)coll | ((unsigned long long)cand << 32));
+    dep |= (uint32_t)o | (uint32_t)(o >> 32);
+    if (atomicAdd(gdone, 1u + opaque_zero(dep)) != gsize - 1u) return false;
+    tot = atomicExch(eg, 0ull);
+    atomicExch(gdone, 0u);
+    return true;
+}
+
 struct CellShared {
     int st[CELL_MAXC + 1];           // cell starts relative to the tile
     float sm[CELL_MAXC], mr[CELL_MAXC], vol[CELL_MAXC];
@@ -883,20 +905,12 @@ __global__ __launch_bounds__(256, ROT ? 3 : 4) void k_cell(CellArgs a)
     __syncthreads();
     bool last = false;
     if (tid == 0) {
-        const uint32_t ng = min(gridDim.x, (uint32_t)EXIT_GROUPS);
-        const uint32_t g = blockIdx.x % ng;
-        const uint32_t gsize = (gridDim.x - g + ng - 1u) / ng;
-        unsigned long long *eg = exit_group(ctrl, (int)g);
-        uint32_t *const gdone = reinterpret_cast<uint32_t *>(eg + 1);
-        const unsigned long long o = atomicAdd(eg, (unsigned long long)sh.coll | ((unsigned long long)sh.cand << 32));
-        dep |= (uint32_t)o | (uint32_t)(o >> 32);
-        if (atomicAdd(gdone, 1u + opaque_zero(dep)) == gsize - 1u) {
-            const unsigned long long t = atomicExch(eg, 0ull);
-            atomicExch(gdone, 0u);
+        unsigned long long t;
+        if (exit_group_add(ctrl, sh.coll, sh.cand, dep, t)) {
             uint32_t dep2 = (uint32_t)t | (uint32_t)(t >> 32);
             if ((uint32_t)t) dep2 |= atomicAdd(&ctrl->step_coll, (uint32_t)t);
             if ((uint32_t)(t >> 32)) dep2 |= atomicAdd(&ctrl->step_cand, (uint32_t)(t >> 32));
-            last = atomicAdd(&ctrl->cell_done, 1u + opaque_zero(dep2)) == ng - 1u;
+            last = atomicAdd(&ctrl->cell_done, 1u + opaque_zero(dep2)) == min(gridDim.x, (uint32_t)EXIT_GROUPS) - 1u;
         }
         if constexpr (BATCH) bsh.last = last ? 1 : 0;
     }
@@ -1370,11 +1384,22 @@ __global__ __launch_bounds__(AMR_THREADS) void k_amr_cell(AmrArgs a)
             __syncthreads();
         }
     }
-    if ((tid & 63) == 0 && (bcoll || bcand)) {   // lane 0 of every wave (the block's cells: thread 0)
-        atomicAdd(&a.ctrl->ncoll, (unsigned long long)bcoll);
-        atomicAdd(&a.ctrl->last_coll, bcoll);
-        atomicAdd(&a.ctrl->ncand, (unsigned long long)bcand);
-        atomicAdd(&a.ctrl->last_cand, bcand);
+    // the block's sums (lane 0 of each wave; the block's cells: thread 0)
+    // go through the exit groups that k_cell left cleared: the last block
+    // of a group adds the group's sums to the totals, so the Ctrl line takes
+    // a few atomics per group instead of four from every wave
+    __shared__ uint32_t s_wc[AMR_THREADS / 64], s_wd[AMR_THREADS / 64];
+    if (lane == 0) { s_wc[tid >> 6] = bcoll; s_wd[tid >> 6] = bcand; }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t coll = 0, cand = 0;
+        for (int w = 0; w < AMR_THREADS / 64; ++w) { coll += s_wc[w]; cand += s_wd[w]; }
+        unsigned long long t;
+        if (exit_group_add(a.ctrl, coll, cand, 0u, t)) {
+            const uint32_t gc = (uint32_t)t, gd = (uint32_t)(t >> 32);
+            if (gc) { atomicAdd(&a.ctrl->ncoll, (unsigned long long)gc); atomicAdd(&a.ctrl->last_coll, gc); }
+            if (gd) { atomicAdd(&a.ctrl->ncand, (unsigned long long)gd); atomicAdd(&a.ctrl->last_cand, gd); }
+        }
     }
 }
 
