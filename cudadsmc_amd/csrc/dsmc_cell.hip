@@ -326,6 +326,10 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
 {
     const int lane = lane_id();
     int nc = 0;
+    // a whole-wave cell is the longest serial chain of its tile (on a
+    // body's stagnation side, of the launch): its wave issues first on its
+    // SIMD (the instruction priority only; no result depends on it)
+    __builtin_amdgcn_s_setprio(3);
     if (LMARK) {
         // mark[q] = lowest accepted lane of the round that uses particle q
         // (64 = none)
@@ -395,6 +399,7 @@ __device__ int ntc_rounds(const CK &k, uint32_t step, int np, int M, float *Vx, 
         else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
+    __builtin_amdgcn_s_setprio(0);
     return nc;
 }
 
