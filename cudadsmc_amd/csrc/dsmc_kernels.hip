@@ -59,7 +59,15 @@ __device__ __forceinline__ int wave_sort64(int v)
 // -- and records (cell, the run slot the atomic returned, lane mask) as the
 // chunk's k-th run, k counting the groups in lane order; lane 0 stores the
 // chunk's run count.  Lanes with c < 0 (left the domain) are in no run.
-__device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i)
+// agg_issue groups the lanes and sends the atomics; agg_finish writes the
+// run records, which need the atomics' values.  A loop over chunks can
+// finish a chunk after issuing the next one, so the atomic's round trip
+// overlaps the next chunk's move.
+struct RunPend {
+    unsigned long long mine, old;   // the lane's run (leaders only), the counter it got
+    int c, rec;                     // its cell, its record index (64 chunk + k)
+};
+__device__ __forceinline__ RunPend agg_issue(const Bins &bn, int c, uint32_t i)
 {
     const int lane = lane_id();
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -74,13 +82,18 @@ __device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i)
     }
     const unsigned long long leaders = __ballot(mine != 0ull);
     const uint32_t q = i >> 6;
-    if (mine) {
-        const unsigned long long old = atomicAdd(&bn.cnt[c], (1ull << 32) | (unsigned long long)__popcll(mine));
-        bn.runrec[(size_t)q * 64 + __popcll(leaders & lt)] =
-            make_int4(c, (int)(uint32_t)(old >> 32), (int)(uint32_t)mine, (int)(uint32_t)(mine >> 32));
-    }
+    RunPend r{ mine, 0ull, c, (int)(q * 64u) + (int)__popcll(leaders & lt) };
+    if (mine) r.old = atomicAdd(&bn.cnt[c], (1ull << 32) | (unsigned long long)__popcll(mine));
     if (lane == 0) bn.chunk_runs[q] = __popcll(leaders);
+    return r;
 }
+__device__ __forceinline__ void agg_finish(const Bins &bn, const RunPend &r)
+{
+    if (r.mine)
+        bn.runrec[(size_t)r.rec] =
+            make_int4(r.c, (int)(uint32_t)(r.old >> 32), (int)(uint32_t)r.mine, (int)(uint32_t)(r.mine >> 32));
+}
+__device__ __forceinline__ void agg_run(const Bins &bn, int c, uint32_t i) { agg_finish(bn, agg_issue(bn, c, i)); }
 
 // The face of the q-th inflow particle of a step (the faces' particles
 // follow one another, J[f] of face f): the number of faces whose cumulative
@@ -118,6 +131,7 @@ __global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, Bins bn,
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, pvz = 0.0f;
     if (i < n_live) { px = S.x[i]; py = S.y[i]; pvx = S.vx[i]; pvy = S.vy[i]; pvz = S.vz[i]; }
+    RunPend pend{ 0ull, 0ull, 0, 0 };
     for (; i < ntot; i += stride) {
         float x = px, y = py, vx = pvx, vy = pvy, vz = pvz, er = 0.0f;
         const uint32_t j = i + stride;
@@ -136,8 +150,11 @@ __global__ __launch_bounds__(256) void k_move(DevCase cs, SoA A, SoA B, Bins bn,
             S.x[i] = x; S.y[i] = y; S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.pid[i] = pid;
             if (rot) S.er[i] = er;
         }
-        agg_run(bn, dead ? -1 : cell_of(cs, x, y), i);
+        const RunPend r = agg_issue(bn, dead ? -1 : cell_of(cs, x, y), i);
+        agg_finish(bn, pend);
+        pend = r;
     }
+    agg_finish(bn, pend);
     if (err) atomicOr(&ctrl->err, err);
 }
 
@@ -816,6 +833,7 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
         const uint32_t i = (uint32_t)q * 64u + (uint32_t)lane;
         px = S.x[i]; py = S.y[i]; pvx = S.vx[i]; pvy = S.vy[i]; pvz = S.vz[i];
     }
+    RunPend pend{ 0ull, 0ull, 0, 0 };
     for (; q < nq; q += nw) {
         const uint32_t i = (uint32_t)q * 64u + (uint32_t)lane;
         float x = px, y = py, vx = pvx, vy = pvy, vz = pvz, er = 0.0f;
@@ -835,8 +853,11 @@ __global__ __launch_bounds__(256) void k_move_batch(const DevCase *__restrict__ 
         S.x[i] = x; S.y[i] = y;
         if (vchg) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
         if (err) { atomicOr(&ictl[u].err, err); err = 0; }
-        agg_run(bn, dead ? -1 : cbase[u] + cell_of(cs, x, y), i);
+        const RunPend r = agg_issue(bn, dead ? -1 : cbase[u] + cell_of(cs, x, y), i);
+        agg_finish(bn, pend);
+        pend = r;
     }
+    agg_finish(bn, pend);
 }
 
 // k_move_batch_rest: the chunks k_move_batch leaves -- those across an
